@@ -1,0 +1,178 @@
+/*
+ * rstsr_hip.h — C ABI of the MI355X (gfx950) device backend for RSTSR.
+ *
+ * Each entry point replaces one method of the rstsr-core device traits that
+ * DeviceFaer / DeviceOpenBLAS implement; the citation beside it names the
+ * reference trait (file:line, paths relative to the rstsr repository root).
+ * All pointers are plain device or host addresses, sizes are in bytes or
+ * elements as stated, layouts are (ndim, shape[], stride[], offset) in
+ * ELEMENTS exactly as rstsr's Layout<IxD> (strides may be negative or zero).
+ *
+ * Every function returns 0 on success and a nonzero rstsr_status_t on failure;
+ * rstsr_hip_last_error() returns the calling thread's last message. Work is
+ * enqueued on the context's stream; functions that hand a value to the host
+ * (*_all, memcpy_d2h) synchronise that stream before returning.
+ */
+#ifndef RSTSR_HIP_H
+#define RSTSR_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Error kinds, numbered as rstsr-common/src/error.rs:15-48 variants used here. */
+typedef enum {
+    RSTSR_OK = 0,
+    RSTSR_ERR_INVALID_LAYOUT = 1,
+    RSTSR_ERR_INVALID_VALUE = 2,
+    RSTSR_ERR_VALUE_OUT_OF_RANGE = 3,
+    RSTSR_ERR_DEVICE_MISMATCH = 4,
+    RSTSR_ERR_DEVICE = 5,
+    RSTSR_ERR_MEMORY = 6,
+    RSTSR_ERR_UNIMPLEMENTED = 7,
+    RSTSR_ERR_INDEX = 8,
+    RSTSR_ERR_RUNTIME = 9
+} rstsr_status_t;
+
+/* Element types (Rust: f32, f64, i32, i64, Complex<f32>, Complex<f64>, usize, bool). */
+typedef enum {
+    RSTSR_F32 = 0,
+    RSTSR_F64 = 1,
+    RSTSR_I32 = 2,
+    RSTSR_I64 = 3,
+    RSTSR_C64 = 4,
+    RSTSR_C128 = 5,
+    RSTSR_U64 = 6,
+    RSTSR_BOOL = 7
+} rstsr_dtype_t;
+
+/* Binary arithmetic (rstsr-core/src/operators/ops/op_ternary_arithmetic.rs:3-56). */
+typedef enum { RSTSR_OP_ADD = 0, RSTSR_OP_SUB = 1, RSTSR_OP_MUL = 2, RSTSR_OP_DIV = 3 } rstsr_binop_t;
+
+/* Reductions (rstsr-core/src/operators/reduction.rs:3-95). */
+typedef enum {
+    RSTSR_RED_SUM = 0,
+    RSTSR_RED_MAX = 1,
+    RSTSR_RED_MIN = 2,
+    RSTSR_RED_ARGMAX = 3,
+    RSTSR_RED_ARGMIN = 4
+} rstsr_redop_t;
+
+/* GEMM arithmetic: plain FP64 MFMA, or Ozaki-II split over INT8 MFMA with CRT. */
+typedef enum { RSTSR_GEMM_PLAIN = 0, RSTSR_GEMM_OZAKI_INT8 = 1 } rstsr_gemm_mode_t;
+
+/* Layout<IxD>: rstsr-common/src/layout/layoutbase.rs:15-23 (shape: usize, stride: isize, offset: usize). */
+typedef struct {
+    size_t ndim;
+    const size_t* shape;
+    const ptrdiff_t* stride;
+    size_t offset;
+} rstsr_layout_t;
+
+typedef struct rstsr_hip_ctx rstsr_hip_ctx_t;
+
+/* ---- device (DeviceBaseAPI, rstsr-core/src/storage/device.rs:3-7) ---- */
+/* Create a context bound to one GPU and one stream (DeviceFaer::new analogue, device_faer/device.rs:12-60). */
+int rstsr_hip_ctx_create(int device_id, rstsr_hip_ctx_t** out);
+int rstsr_hip_ctx_destroy(rstsr_hip_ctx_t* ctx);
+/* same_device (storage/device.rs:4): 1 if both contexts drive the same GPU. */
+int rstsr_hip_same_device(const rstsr_hip_ctx_t* a, const rstsr_hip_ctx_t* b);
+/* Thread-local message for the last failing call on this thread. */
+const char* rstsr_hip_last_error(void);
+int rstsr_hip_synchronize(rstsr_hip_ctx_t* ctx);
+/* hipStream_t of the context, for callers that enqueue their own work on it. */
+int rstsr_hip_get_stream(rstsr_hip_ctx_t* ctx, void** stream);
+int rstsr_hip_device_count(int* count);
+
+/* ---- storage (DeviceCreationAnyAPI storage/creation.rs:3-34, DeviceStorageAPI storage/device.rs:23-57) ---- */
+/* uninit_impl / empty_impl: nbytes of uninitialised device memory. */
+int rstsr_hip_malloc(rstsr_hip_ctx_t* ctx, size_t nbytes, void** ptr);
+/* Drop of Raw; the library frees only memory this call names. */
+int rstsr_hip_free(rstsr_hip_ctx_t* ctx, void* ptr);
+/* outof_cpu_vec / from_cpu_vec (host -> device). */
+int rstsr_hip_memcpy_h2d(rstsr_hip_ctx_t* ctx, void* dst, const void* src, size_t nbytes);
+/* to_cpu_vec / into_cpu_vec / get_index (device -> host, synchronous). */
+int rstsr_hip_memcpy_d2h(rstsr_hip_ctx_t* ctx, void* dst, const void* src, size_t nbytes);
+/* Raw: Clone (device -> device, stream ordered). */
+int rstsr_hip_memcpy_d2d(rstsr_hip_ctx_t* ctx, void* dst, const void* src, size_t nbytes);
+
+/* ---- assignment (rstsr-core/src/operators/assignment.rs:5-53) ---- */
+/* OpAssignArbitaryAPI::assign_arbitary(_uninit): same element COUNT, any shapes, copied in
+ * logical row-major order (cpu_rayon/assignment.rs:14-93); dtype cast c <- a. */
+int rstsr_hip_assign_arbitary(rstsr_hip_ctx_t* ctx, rstsr_dtype_t tc, void* c, const rstsr_layout_t* lc,
+                              rstsr_dtype_t ta, const void* a, const rstsr_layout_t* la);
+/* OpAssignAPI::assign: same shape (cpu_rayon/assignment.rs:95-171); dtype cast c <- a. */
+int rstsr_hip_assign(rstsr_hip_ctx_t* ctx, rstsr_dtype_t tc, void* c, const rstsr_layout_t* lc, rstsr_dtype_t ta,
+                     const void* a, const rstsr_layout_t* la);
+/* OpAssignAPI::fill (cpu_rayon/assignment.rs:181-225); *value is one host element of dtype t. */
+int rstsr_hip_fill(rstsr_hip_ctx_t* ctx, rstsr_dtype_t t, void* c, const rstsr_layout_t* lc, const void* value);
+
+/* ---- elementwise (operators/ops/op_ternary_arithmetic.rs:3-56, op_binary_arithmetic.rs:16-43) ---- */
+/* c = a (op) b; lc, la, lb already broadcast to one shape (tensor/operators/op_binary_arithmetic.rs:197-208). */
+int rstsr_hip_op_mutc_refa_refb(rstsr_hip_ctx_t* ctx, rstsr_binop_t op, rstsr_dtype_t t, void* c,
+                                const rstsr_layout_t* lc, const void* a, const rstsr_layout_t* la, const void* b,
+                                const rstsr_layout_t* lb);
+/* c = a (op) scalar b (host value). */
+int rstsr_hip_op_mutc_refa_numb(rstsr_hip_ctx_t* ctx, rstsr_binop_t op, rstsr_dtype_t t, void* c,
+                                const rstsr_layout_t* lc, const void* a, const rstsr_layout_t* la, const void* b);
+/* c = scalar a (op) b. */
+int rstsr_hip_op_mutc_numa_refb(rstsr_hip_ctx_t* ctx, rstsr_binop_t op, rstsr_dtype_t t, void* c,
+                                const rstsr_layout_t* lc, const void* a, const void* b, const rstsr_layout_t* lb);
+/* c (op)= b  (OpAddAssignAPI etc.). */
+int rstsr_hip_op_muta_refb(rstsr_hip_ctx_t* ctx, rstsr_binop_t op, rstsr_dtype_t t, void* c,
+                           const rstsr_layout_t* lc, const void* b, const rstsr_layout_t* lb);
+/* c (op)= scalar b. */
+int rstsr_hip_op_muta_numb(rstsr_hip_ctx_t* ctx, rstsr_binop_t op, rstsr_dtype_t t, void* c,
+                           const rstsr_layout_t* lc, const void* b);
+
+/* ---- reductions (operators/reduction.rs:3-95; auto_impl/reduction.rs:7-464) ---- */
+/* sum_all / max_all / min_all: one host value of dtype t (sum of f32 is accumulated in f64). */
+int rstsr_hip_reduce_all(rstsr_hip_ctx_t* ctx, rstsr_redop_t op, rstsr_dtype_t t, const void* a,
+                         const rstsr_layout_t* la, void* host_out);
+/* argmax_all / argmin_all: raveled row-major index into la's shape (reduction.rs:577). */
+int rstsr_hip_argreduce_all(rstsr_hip_ctx_t* ctx, rstsr_redop_t op, rstsr_dtype_t t, const void* a,
+                            const rstsr_layout_t* la, uint64_t* host_out);
+/* Output layout of a reduction over `axes` (reduce_axes_cpu_rayon, cpu_rayon/reduction.rs:153 =
+ * layout_for_array_copy(rest, K)); shape_out/stride_out need room for la->ndim entries. */
+int rstsr_hip_reduce_out_layout(const rstsr_layout_t* la, size_t naxes, const ptrdiff_t* axes, size_t* ndim_out,
+                                size_t* shape_out, ptrdiff_t* stride_out);
+/* sum_axes / max_axes / min_axes into caller-allocated `out` laid out as `lo` (same dtype t). */
+int rstsr_hip_reduce_axes(rstsr_hip_ctx_t* ctx, rstsr_redop_t op, rstsr_dtype_t t, const void* a,
+                          const rstsr_layout_t* la, size_t naxes, const ptrdiff_t* axes, void* out,
+                          const rstsr_layout_t* lo);
+/* argmax_axes / argmin_axes: u64 indices raveled row-major over the reduced axes in the given
+ * order (cpu_serial/reduction.rs:553-580), into `out` laid out as `lo`. */
+int rstsr_hip_argreduce_axes(rstsr_hip_ctx_t* ctx, rstsr_redop_t op, rstsr_dtype_t t, const void* a,
+                             const rstsr_layout_t* la, size_t naxes, const ptrdiff_t* axes, uint64_t* out,
+                             const rstsr_layout_t* lo);
+/* OpAllCloseAPI (operators/reduction.rs:73-94; isclose.rs:92-106): |a-b| <= atol + rtol*|b|, NaN never close. */
+int rstsr_hip_allclose(rstsr_hip_ctx_t* ctx, rstsr_dtype_t t, const void* a, const rstsr_layout_t* la,
+                       const void* b, const rstsr_layout_t* lb, double rtol, double atol, int equal_nan,
+                       int* host_out);
+
+/* ---- matmul (DeviceMatMulAPI, operators/matmul.rs:7-25; device_faer/matmul.rs:204-239) ---- */
+/* c = alpha * a % b + beta * c with rstsr's row-major matmul rules 1-7 (layout/matmul.rs:9-29).
+ * beta == 0 never reads c. alpha/beta point to host scalars of dtype t. mode selects the arithmetic
+ * for f64/c128; err_out (nullable) receives nothing and is reserved for per-call error reports. */
+int rstsr_hip_matmul(rstsr_hip_ctx_t* ctx, rstsr_dtype_t t, void* c, const rstsr_layout_t* lc, const void* a,
+                     const rstsr_layout_t* la, const void* b, const rstsr_layout_t* lb, const void* alpha,
+                     const void* beta, rstsr_gemm_mode_t mode);
+/* 2-D GEMM on strided matrices: C[m,n] = alpha*sum_k A[m,k]B[k,n] + beta*C, each operand given by
+ * (row stride, col stride) in elements (GEMMDriverAPI, rstsr-blas-traits/src/blas3/gemm.rs:4-20). */
+int rstsr_hip_gemm(rstsr_hip_ctx_t* ctx, rstsr_dtype_t t, rstsr_gemm_mode_t mode, size_t m, size_t n, size_t k,
+                   const void* alpha, const void* a, ptrdiff_t rsa, ptrdiff_t csa, const void* b, ptrdiff_t rsb,
+                   ptrdiff_t csb, const void* beta, void* c, ptrdiff_t rsc, ptrdiff_t csc);
+
+/* ---- timing on the context stream (no counterpart in the reference; used by benchmarks) ---- */
+int rstsr_hip_event_record(rstsr_hip_ctx_t* ctx, int slot);
+/* Milliseconds between two recorded slots (synchronises on the later one). */
+int rstsr_hip_event_elapsed(rstsr_hip_ctx_t* ctx, int slot_begin, int slot_end, float* ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RSTSR_HIP_H */

@@ -1,0 +1,401 @@
+"""rstsr_amd: MI355X (gfx950) device backend for RSTSR, driven from Python.
+
+The compute path is the C ABI library ``librstsr_hip.so`` (hand-written HIP
+kernels); ``_rstsr`` is the C++ front end over it (tensor layer mirrored from
+rstsr-core/src/tensor). This module only adds Python operator syntax and NumPy
+conversion. Importing it fails loudly if the native modules are not built.
+
+Names follow rstsr's API (``rt::asarray``, ``sum_axes``, ``argmax_axes``,
+``to_contig``, ``%`` for matmul) so that tests read like the reference's:
+
+    >>> import rstsr_amd as rt
+    >>> dev = rt.DeviceHIP(0)
+    >>> a = rt.arange(3240, device=dev).reshape([12, 15, 18]).swapaxes(-1, -2)
+    >>> a[2:-3, 1:-4:2, -1:3:-2].sum_all()
+    446586
+"""
+import numpy as np
+
+try:
+    from . import _rstsr
+except ImportError as e:  # pragma: no cover - exercised only on a broken build
+    raise ImportError(
+        "rstsr_amd native modules are not built; run `python -m rstsr_amd.build` "
+        "(needs hipcc for gfx950): " + str(e)
+    ) from e
+
+RSTSRError = _rstsr.RSTSRError
+DeviceHIP = _rstsr.DeviceHIP
+device_count = _rstsr.device_count
+
+# dtype codes of include/rstsr_hip.h
+F32, F64, I32, I64, C64, C128, U64 = 0, 1, 2, 3, 4, 5, 6
+_NP2CODE = {
+    np.dtype(np.float32): F32,
+    np.dtype(np.float64): F64,
+    np.dtype(np.int32): I32,
+    np.dtype(np.int64): I64,
+    np.dtype(np.complex64): C64,
+    np.dtype(np.complex128): C128,
+    np.dtype(np.uint64): U64,
+}
+_CODE2NP = {v: k for k, v in _NP2CODE.items()}
+
+OP_ADD, OP_SUB, OP_MUL, OP_DIV = 0, 1, 2, 3
+RED_SUM, RED_MAX, RED_MIN, RED_ARGMAX, RED_ARGMIN = 0, 1, 2, 3, 4
+GEMM_PLAIN, GEMM_OZAKI = 0, 1
+
+_default_device = None
+
+
+def default_device():
+    global _default_device
+    if _default_device is None:
+        _default_device = DeviceHIP(0)
+    return _default_device
+
+
+def _code(dtype):
+    dt = np.dtype(dtype)
+    if dt not in _NP2CODE:
+        raise RSTSRError("UnImplemented", f"dtype {dt} is not supported by the HIP device")
+    return _NP2CODE[dt]
+
+
+def _axes(axes, ndim):
+    if axes is None:
+        return list(range(ndim))
+    if isinstance(axes, (int, np.integer)):
+        return [int(axes)]
+    return [int(x) for x in axes]
+
+
+class Tensor:
+    """A strided view of device storage (rstsr-core/src/tensorbase.rs:5-24)."""
+
+    __slots__ = ("_t",)
+
+    def __init__(self, t):
+        self._t = t
+
+    # ---------------- properties ----------------
+    @property
+    def shape(self):
+        return tuple(self._t.shape)
+
+    @property
+    def stride(self):
+        return tuple(self._t.stride)
+
+    @property
+    def offset(self):
+        return self._t.offset
+
+    @property
+    def layout(self):
+        return self._t.layout
+
+    @property
+    def ndim(self):
+        return self._t.ndim
+
+    @property
+    def size(self):
+        return self._t.size
+
+    @property
+    def dtype(self):
+        return _CODE2NP[self._t.dtype_code]
+
+    @property
+    def device(self):
+        return self._t.device
+
+    def __repr__(self):
+        return f"Tensor(shape={self.shape}, stride={self.stride}, offset={self.offset}, dtype={self.dtype})"
+
+    # ---------------- host transfer ----------------
+    def to_numpy(self):
+        """Logical contents as a NumPy array (to_vec in logical order)."""
+        if self.size == 0:
+            return np.zeros(self.shape, dtype=self.dtype)
+        raw, off = self._t._raw_range()
+        buf = np.frombuffer(raw, dtype=self.dtype)
+        it = buf.itemsize
+        v = np.lib.stride_tricks.as_strided(
+            buf[off:], shape=self.shape, strides=[s * it for s in self.stride]
+        )
+        return np.array(v, copy=True)
+
+    def to_vec(self):
+        return self.to_numpy().reshape(-1).tolist()
+
+    def to_scalar(self):
+        if self.size != 1:
+            raise RSTSRError("InvalidLayout", "to_scalar needs exactly one element")
+        return self.to_numpy().reshape(-1)[0].item()
+
+    # ---------------- views (layout only) ----------------
+    def _view(self, layout):
+        return Tensor(self._t._with_layout(layout))
+
+    def transpose(self, axes=None):
+        nd = self.ndim
+        axes = list(range(nd))[::-1] if axes is None else [a % nd if a < 0 else a for a in axes]
+        if sorted(axes) != list(range(nd)):
+            raise RSTSRError("InvalidLayout", "transpose: invalid permutation")
+        sh, st, off = self.layout
+        return self._view(([sh[a] for a in axes], [st[a] for a in axes], off))
+
+    permute = transpose
+
+    def reverse_axes(self):
+        return self.transpose()
+
+    @property
+    def T(self):
+        return self.transpose()
+
+    def swapaxes(self, a, b):
+        nd = self.ndim
+        a, b = a % nd, b % nd
+        p = list(range(nd))
+        p[a], p[b] = p[b], p[a]
+        return self.transpose(p)
+
+    def flip(self, axis):
+        return self._view(_rstsr.dim_narrow(self.layout, axis, None, None, -1))
+
+    def slice(self, axis, start=None, stop=None, step=None):
+        return self._view(_rstsr.dim_narrow(self.layout, axis, start, stop, 1 if step is None else step))
+
+    def __getitem__(self, idx):
+        if not isinstance(idx, tuple):
+            idx = (idx,)
+        if any(i is Ellipsis for i in idx):
+            k = idx.index(Ellipsis)
+            fill = self.ndim - (len(idx) - 1)
+            idx = idx[:k] + (slice(None),) * fill + idx[k + 1:]
+        lay = self.layout
+        ax = 0
+        drop = []
+        for i in idx:
+            if isinstance(i, slice):
+                if not (i.start is None and i.stop is None and i.step in (None, 1)):
+                    lay = _rstsr.dim_narrow(lay, ax, i.start, i.stop, 1 if i.step is None else i.step)
+                ax += 1
+            elif isinstance(i, (int, np.integer)):
+                n = lay[0][ax]
+                j = int(i) + n if i < 0 else int(i)
+                if not 0 <= j < n:
+                    raise RSTSRError("IndexError", f"index {i} out of range for axis {ax} of size {n}")
+                lay = _rstsr.dim_narrow(lay, ax, j, j + 1, 1)
+                drop.append(ax)
+                ax += 1
+            else:
+                raise RSTSRError("UnImplemented", "only int / slice / ... indices are supported")
+        sh, st, off = lay
+        sh = [s for k, s in enumerate(sh) if k not in drop]
+        st = [s for k, s in enumerate(st) if k not in drop]
+        return self._view((sh, st, off))
+
+    def reshape(self, shape):
+        """into_shape: a view when the layout is c-contiguous, else a c-contiguous copy first."""
+        shape = [int(s) for s in shape]
+        if -1 in shape:
+            known = int(np.prod([s for s in shape if s != -1]))
+            shape[shape.index(-1)] = self.size // max(known, 1)
+        if int(np.prod(shape)) != self.size:
+            raise RSTSRError("InvalidLayout", "reshape: element counts differ")
+        src = self if _rstsr.c_contig(self.layout) else self.to_contig()
+        st = _rstsr.c_layout(shape)[1]
+        return Tensor(src._t._with_layout((shape, st, src.offset)))
+
+    into_shape = reshape
+
+    def broadcast_to(self, shape):
+        sh, st, off = self.layout
+        shape = list(shape)
+        n = len(shape)
+        if n < len(sh):
+            raise RSTSRError("InvalidLayout", "broadcast_to: rank too small")
+        nst = [0] * n
+        for i in range(len(sh)):
+            j = n - len(sh) + i
+            if sh[i] == shape[j]:
+                nst[j] = st[i]
+            elif sh[i] == 1:
+                nst[j] = 0
+            else:
+                raise RSTSRError("InvalidLayout", "Broadcasting failed.")
+        return self._view((shape, nst, off))
+
+    # ---------------- copies ----------------
+    def to_contig(self, order="C"):
+        return Tensor(self._t.to_contig(order))
+
+    def to_layout(self, layout):
+        return Tensor(self._t.to_layout(layout))
+
+    def astype(self, dtype):
+        return Tensor(self._t.astype(_code(dtype)))
+
+    def assign(self, other):
+        self._t.assign(other._t)
+
+    def fill(self, value):
+        self._t.fill(np.array(value, dtype=self.dtype).tobytes())
+
+    # ---------------- arithmetic ----------------
+    def _bin(self, op, other, left=False):
+        if isinstance(other, Tensor):
+            return Tensor(self._t.binary(op, other._t) if not left else other._t.binary(op, self._t))
+        s = np.array(other, dtype=self.dtype).tobytes()
+        return Tensor(self._t.binary_scalar(op, s, left))
+
+    def __add__(self, o):
+        return self._bin(OP_ADD, o)
+
+    def __radd__(self, o):
+        return self._bin(OP_ADD, o, left=True)
+
+    def __sub__(self, o):
+        return self._bin(OP_SUB, o)
+
+    def __rsub__(self, o):
+        return self._bin(OP_SUB, o, left=True)
+
+    def __mul__(self, o):
+        return self._bin(OP_MUL, o)
+
+    def __rmul__(self, o):
+        return self._bin(OP_MUL, o, left=True)
+
+    def __truediv__(self, o):
+        return self._bin(OP_DIV, o)
+
+    def __rtruediv__(self, o):
+        return self._bin(OP_DIV, o, left=True)
+
+    def _ibin(self, op, o):
+        if isinstance(o, Tensor):
+            self._t.binary_inplace(op, o._t)
+        else:
+            self._t.binary_inplace_scalar(op, np.array(o, dtype=self.dtype).tobytes())
+        return self
+
+    def __iadd__(self, o):
+        return self._ibin(OP_ADD, o)
+
+    def __isub__(self, o):
+        return self._ibin(OP_SUB, o)
+
+    def __imul__(self, o):
+        return self._ibin(OP_MUL, o)
+
+    def __itruediv__(self, o):
+        return self._ibin(OP_DIV, o)
+
+    def __mod__(self, o):
+        return matmul(self, o)
+
+    def __matmul__(self, o):
+        return matmul(self, o)
+
+    # ---------------- reductions ----------------
+    def _red_all(self, op):
+        raw = self._t.reduce_all(op)
+        if op in (RED_ARGMAX, RED_ARGMIN):
+            return int(np.frombuffer(raw, dtype=np.uint64)[0])
+        return np.frombuffer(raw, dtype=self.dtype)[0].item()
+
+    def _red_axes(self, op, axes):
+        return Tensor(self._t.reduce_axes(op, _axes(axes, self.ndim)))
+
+    def sum_all(self):
+        return self._red_all(RED_SUM)
+
+    def max_all(self):
+        return self._red_all(RED_MAX)
+
+    def min_all(self):
+        return self._red_all(RED_MIN)
+
+    def argmax_all(self):
+        return self._red_all(RED_ARGMAX)
+
+    def argmin_all(self):
+        return self._red_all(RED_ARGMIN)
+
+    def sum_axes(self, axes=None):
+        return self._red_axes(RED_SUM, axes)
+
+    def max_axes(self, axes=None):
+        return self._red_axes(RED_MAX, axes)
+
+    def min_axes(self, axes=None):
+        return self._red_axes(RED_MIN, axes)
+
+    def argmax_axes(self, axes=None):
+        return self._red_axes(RED_ARGMAX, axes)
+
+    def argmin_axes(self, axes=None):
+        return self._red_axes(RED_ARGMIN, axes)
+
+    sum = sum_axes
+
+
+def asarray(x, device=None, dtype=None):
+    """rt::asarray((vec, shape, &device)) — upload host data as a c-contiguous tensor."""
+    dev = device or default_device()
+    a = np.ascontiguousarray(np.asarray(x, dtype=dtype))
+    return Tensor(_rstsr.from_host(dev, a.tobytes(), list(a.shape), _code(a.dtype)))
+
+
+def empty(shape, dtype=np.float64, device=None):
+    dev = device or default_device()
+    return Tensor(_rstsr.empty(dev, _rstsr.c_layout(list(shape)), _code(dtype)))
+
+
+def full(shape, value, dtype=np.float64, device=None):
+    t = empty(shape, dtype, device)
+    t.fill(value)
+    return t
+
+
+def zeros(shape, dtype=np.float64, device=None):
+    return full(shape, 0, dtype, device)
+
+
+def ones(shape, dtype=np.float64, device=None):
+    return full(shape, 1, dtype, device)
+
+
+def arange(n, device=None, dtype=None):
+    """rt::arange(n): integers 0..n (i64 by default, like usize/isize), or floats."""
+    if dtype is None:
+        dtype = np.float64 if isinstance(n, float) else np.int64
+    return asarray(np.arange(n, dtype=dtype), device=device)
+
+
+def linspace(start, stop, num, device=None):
+    """rt::linspace((start, stop, n)) as NumPy computes it."""
+    return asarray(np.linspace(start, stop, num), device=device)
+
+
+def matmul(a, b, mode="plain"):
+    """`a % b` with rstsr's row-major matmul rules; mode 'plain' (FP64 MFMA) or 'ozaki' (INT8 split)."""
+    m = {"plain": GEMM_PLAIN, "ozaki": GEMM_OZAKI}[mode]
+    return Tensor(a._t.matmul(b._t, m))
+
+
+def allclose(a, b, rtol=1e-5, atol=1e-8, equal_nan=False):
+    return a._t.allclose(b._t, rtol, atol, equal_nan)
+
+
+def add(a, b):
+    return a + b
+
+
+def mul(a, b):
+    return a * b

@@ -1,0 +1,261 @@
+// Context, storage and transfer entry points of the C ABI (include/rstsr_hip.h).
+//
+// Storage follows rstsr's ownership model (rstsr-core/src/storage/data.rs:11-43):
+// the caller owns every buffer; the library frees only what rstsr_hip_free names.
+// Every allocation made here is registered with its extent so that each kernel
+// entry point can prove, on the host, that the layouts it is given stay inside
+// the buffers they index before anything is launched.
+#include <map>
+
+#include "common.hpp"
+#include "registry.hpp"
+
+namespace rstsr {
+namespace hip {
+
+static thread_local std::string g_last_error;
+void set_last_error(const std::string& msg) { g_last_error = msg; }
+
+/* ------------------------------ allocation registry ------------------------------ */
+
+static std::mutex g_reg_mu;
+static std::map<uintptr_t, size_t>& reg() {
+    static std::map<uintptr_t, size_t> m;
+    return m;
+}
+
+void registry_add(void* p, size_t bytes) {
+    std::lock_guard<std::mutex> g(g_reg_mu);
+    reg()[(uintptr_t)p] = bytes;
+}
+bool registry_remove(void* p) {
+    std::lock_guard<std::mutex> g(g_reg_mu);
+    return reg().erase((uintptr_t)p) > 0;
+}
+
+// The allocation containing address p: (base, bytes); bytes == 0 if none.
+std::pair<uintptr_t, size_t> registry_find(const void* p) {
+    std::lock_guard<std::mutex> g(g_reg_mu);
+    auto& m = reg();
+    auto it = m.upper_bound((uintptr_t)p);
+    if (it == m.begin()) return {0, 0};
+    --it;
+    if ((uintptr_t)p >= it->first && (uintptr_t)p < it->first + std::max<size_t>(it->second, 1)) return *it;
+    return {0, 0};
+}
+
+void check_extent(const void* ptr, const Layout& l, size_t elem, const char* name) {
+    if (l.size() == 0) return;
+    if (!ptr) RSTSR_RAISE(InvalidValue, std::string("null device pointer for operand ") + name);
+    auto b = l.bounds_index();
+    auto a = registry_find(ptr);
+    if (a.second == 0)
+        RSTSR_RAISE(InvalidValue, std::string("operand ") + name + " is not inside a buffer from rstsr_hip_malloc");
+    uintptr_t lo = (uintptr_t)ptr + b.first * elem;
+    uintptr_t hi = (uintptr_t)ptr + b.second * elem;
+    if (lo < a.first || hi > a.first + a.second)
+        RSTSR_RAISE(ValueOutOfRange, std::string("layout of operand ") + name + " reaches outside its buffer");
+}
+
+void check_extent_bytes(const void* ptr, size_t bytes, const char* name) {
+    if (bytes == 0) return;
+    auto a = registry_find(ptr);
+    if (a.second == 0)
+        RSTSR_RAISE(InvalidValue, std::string("operand ") + name + " is not inside a buffer from rstsr_hip_malloc");
+    if ((uintptr_t)ptr + bytes > a.first + a.second)
+        RSTSR_RAISE(ValueOutOfRange, std::string("copy of operand ") + name + " reaches outside its buffer");
+}
+
+void* workspace(rstsr_hip_ctx* ctx, size_t bytes) {
+    if (bytes <= ctx->ws.bytes) return ctx->ws.ptr;
+    size_t nb = std::max<size_t>(bytes, ctx->ws.bytes * 2);
+    nb = (nb + 4095) & ~(size_t)4095;
+    if (ctx->ws.ptr) {
+        HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        HIP_CHECK(hipFree(ctx->ws.ptr));
+        ctx->ws.ptr = nullptr;
+        ctx->ws.bytes = 0;
+    }
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, nb);
+    if (e != hipSuccess) throw Error(ErrorKind::MemoryError, "workspace hipMalloc failed: " + std::string(hipGetErrorString(e)));
+    ctx->ws.ptr = p;
+    ctx->ws.bytes = nb;
+    return p;
+}
+
+}  // namespace hip
+}  // namespace rstsr
+
+using namespace rstsr;
+using namespace rstsr::hip;
+
+extern "C" {
+
+const char* rstsr_hip_last_error(void) { return g_last_error.c_str(); }
+
+int rstsr_hip_device_count(int* count) {
+    return guarded([&] {
+        if (!count) RSTSR_RAISE(InvalidValue, "null output");
+        int n = 0;
+        hipError_t e = hipGetDeviceCount(&n);
+        *count = (e == hipSuccess) ? n : 0;
+    });
+}
+
+int rstsr_hip_ctx_create(int device_id, rstsr_hip_ctx_t** out) {
+    return guarded([&] {
+        if (!out) RSTSR_RAISE(InvalidValue, "null output");
+        *out = nullptr;
+        int n = 0;
+        HIP_CHECK(hipGetDeviceCount(&n));
+        if (device_id < 0 || device_id >= n)
+            RSTSR_RAISE(DeviceError, "device id " + std::to_string(device_id) + " not present (" + std::to_string(n) + " devices)");
+        auto* ctx = new rstsr_hip_ctx();
+        ctx->device = device_id;
+        try {
+            HIP_CHECK(hipSetDevice(device_id));
+            HIP_CHECK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+            hipDeviceProp_t prop;
+            HIP_CHECK(hipGetDeviceProperties(&prop, device_id));
+            ctx->num_cu = prop.multiProcessorCount;
+            ctx->l2_bytes = prop.l2CacheSize;
+            HIP_CHECK(hipHostMalloc(&ctx->host_pinned, 4096, hipHostMallocDefault));
+            for (auto& ev : ctx->events) HIP_CHECK(hipEventCreate(&ev));
+        } catch (...) {
+            if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+            if (ctx->host_pinned) (void)hipHostFree(ctx->host_pinned);
+            delete ctx;
+            throw;
+        }
+        *out = ctx;
+    });
+}
+
+int rstsr_hip_ctx_destroy(rstsr_hip_ctx_t* ctx) {
+    return guarded([&] {
+        if (!ctx) return;
+        bind(ctx);
+        HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        if (ctx->ws.ptr) HIP_CHECK(hipFree(ctx->ws.ptr));
+        for (auto& ev : ctx->events)
+            if (ev) HIP_CHECK(hipEventDestroy(ev));
+        if (ctx->host_pinned) HIP_CHECK(hipHostFree(ctx->host_pinned));
+        HIP_CHECK(hipStreamDestroy(ctx->stream));
+        delete ctx;
+    });
+}
+
+int rstsr_hip_same_device(const rstsr_hip_ctx_t* a, const rstsr_hip_ctx_t* b) {
+    return (a && b && a->device == b->device) ? 1 : 0;
+}
+
+int rstsr_hip_synchronize(rstsr_hip_ctx_t* ctx) {
+    return guarded([&] {
+        bind(ctx);
+        HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+int rstsr_hip_get_stream(rstsr_hip_ctx_t* ctx, void** stream) {
+    return guarded([&] {
+        if (!ctx || !stream) RSTSR_RAISE(InvalidValue, "null argument");
+        *stream = (void*)ctx->stream;
+    });
+}
+
+int rstsr_hip_malloc(rstsr_hip_ctx_t* ctx, size_t nbytes, void** ptr) {
+    return guarded([&] {
+        if (!ptr) RSTSR_RAISE(InvalidValue, "null output");
+        bind(ctx);
+        void* p = nullptr;
+        // round up so vectorised tails never leave the allocation
+        size_t nb = std::max<size_t>((nbytes + 255) & ~(size_t)255, 256);
+        hipError_t e = hipMalloc(&p, nb);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            throw Error(ErrorKind::MemoryError, "hipMalloc of " + std::to_string(nbytes) + " bytes failed: " + hipGetErrorString(e));
+        }
+        registry_add(p, nbytes);
+        *ptr = p;
+    });
+}
+
+int rstsr_hip_free(rstsr_hip_ctx_t* ctx, void* ptr) {
+    return guarded([&] {
+        if (!ptr) return;
+        bind(ctx);
+        if (!registry_remove(ptr)) RSTSR_RAISE(InvalidValue, "free of a pointer not returned by rstsr_hip_malloc");
+        // frees are stream ordered with respect to this context's queued work
+        HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        HIP_CHECK(hipFree(ptr));
+    });
+}
+
+int rstsr_hip_memcpy_h2d(rstsr_hip_ctx_t* ctx, void* dst, const void* src, size_t nbytes) {
+    return guarded([&] {
+        if (nbytes == 0) return;
+        if (!src) RSTSR_RAISE(InvalidValue, "null host source");
+        bind(ctx);
+        check_extent_bytes(dst, nbytes, "dst");
+        HIP_CHECK(hipMemcpyAsync(dst, src, nbytes, hipMemcpyHostToDevice, ctx->stream));
+        // pageable host memory: the copy must complete before the caller may reuse src
+        HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+int rstsr_hip_memcpy_d2h(rstsr_hip_ctx_t* ctx, void* dst, const void* src, size_t nbytes) {
+    return guarded([&] {
+        if (nbytes == 0) return;
+        if (!dst) RSTSR_RAISE(InvalidValue, "null host destination");
+        bind(ctx);
+        check_extent_bytes(src, nbytes, "src");
+        HIP_CHECK(hipMemcpyAsync(dst, src, nbytes, hipMemcpyDeviceToHost, ctx->stream));
+        HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+int rstsr_hip_memcpy_d2d(rstsr_hip_ctx_t* ctx, void* dst, const void* src, size_t nbytes) {
+    return guarded([&] {
+        if (nbytes == 0) return;
+        bind(ctx);
+        check_extent_bytes(dst, nbytes, "dst");
+        check_extent_bytes(src, nbytes, "src");
+        HIP_CHECK(hipMemcpyAsync(dst, src, nbytes, hipMemcpyDeviceToDevice, ctx->stream));
+    });
+}
+
+int rstsr_hip_event_record(rstsr_hip_ctx_t* ctx, int slot) {
+    return guarded([&] {
+        bind(ctx);
+        if (slot < 0 || slot >= 16) RSTSR_RAISE(InvalidValue, "event slot out of range [0,16)");
+        HIP_CHECK(hipEventRecord(ctx->events[slot], ctx->stream));
+    });
+}
+
+int rstsr_hip_event_elapsed(rstsr_hip_ctx_t* ctx, int s0, int s1, float* ms) {
+    return guarded([&] {
+        bind(ctx);
+        if (s0 < 0 || s0 >= 16 || s1 < 0 || s1 >= 16 || !ms) RSTSR_RAISE(InvalidValue, "bad event arguments");
+        HIP_CHECK(hipEventSynchronize(ctx->events[s1]));
+        HIP_CHECK(hipEventElapsedTime(ms, ctx->events[s0], ctx->events[s1]));
+    });
+}
+
+int rstsr_hip_reduce_out_layout(const rstsr_layout_t* la, size_t naxes, const ptrdiff_t* axes, size_t* ndim_out,
+                                size_t* shape_out, ptrdiff_t* stride_out) {
+    return guarded([&] {
+        if (!ndim_out || (la && la->ndim && (!shape_out || !stride_out))) RSTSR_RAISE(InvalidValue, "null output");
+        Layout l = from_abi(la);
+        std::vector<ptrdiff_t> ax(axes, axes + naxes);
+        auto sp = l.dim_split_axes(ax);
+        Layout lo = layout_for_array_copy(sp.second, IterOrder::K);
+        *ndim_out = lo.ndim();
+        for (size_t i = 0; i < lo.ndim(); ++i) {
+            shape_out[i] = lo.shape[i];
+            stride_out[i] = lo.stride[i];
+        }
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,155 @@
+// Shared definitions of the gfx950 backend: context, error plumbing, element
+// types, index helpers and launch utilities. Compiled only by hipcc.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/rstsr_hip.h"
+#include "layout.hpp"
+
+namespace rstsr {
+namespace hip {
+
+constexpr int MAXD = 8;  // dimensions a kernel sees after axis collapsing
+
+/* ------------------------------ errors ------------------------------ */
+
+void set_last_error(const std::string& msg);
+
+#define HIP_CHECK(expr)                                                                                 \
+    do {                                                                                                \
+        hipError_t _e = (expr);                                                                         \
+        if (_e != hipSuccess)                                                                           \
+            throw ::rstsr::Error(::rstsr::ErrorKind::DeviceError,                                       \
+                                 std::string(#expr) + " failed: " + hipGetErrorString(_e));             \
+    } while (0)
+
+// Run `body` and convert any exception into a status code + thread-local message.
+template <class F>
+inline int guarded(F&& body) {
+    try {
+        body();
+        return RSTSR_OK;
+    } catch (const ::rstsr::Error& e) {
+        set_last_error(std::string(error_kind_name(e.kind)) + ": " + e.what());
+        return (int)e.kind;
+    } catch (const std::bad_alloc&) {
+        set_last_error("MemoryError: host allocation failed");
+        return RSTSR_ERR_MEMORY;
+    } catch (const std::exception& e) {
+        set_last_error(std::string("RuntimeError: ") + e.what());
+        return RSTSR_ERR_RUNTIME;
+    }
+}
+
+/* ------------------------------ context ------------------------------ */
+
+// Device workspace that grows on demand and is reused across calls on one
+// context (partials of split reductions, scalar results, GEMM slices).
+struct Workspace {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+};
+
+}  // namespace hip
+}  // namespace rstsr
+
+struct rstsr_hip_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int num_cu = 0;
+    size_t l2_bytes = 0;
+    std::mutex mu;  // serialises workspace growth and host staging on one context
+    rstsr::hip::Workspace ws;
+    void* host_pinned = nullptr;  // 4 KiB pinned staging for scalar results
+    hipEvent_t events[16] = {};
+};
+
+namespace rstsr {
+namespace hip {
+
+// Make the context's device current on this thread.
+inline void bind(rstsr_hip_ctx* ctx) {
+    if (!ctx) RSTSR_RAISE(InvalidValue, "null context");
+    HIP_CHECK(hipSetDevice(ctx->device));
+}
+
+// Return a device buffer of at least `bytes` owned by the context. Growth is
+// ordered after all work already enqueued on the context stream.
+void* workspace(rstsr_hip_ctx* ctx, size_t bytes);
+
+/* ------------------------------ element types ------------------------------ */
+
+struct c64 {
+    float re, im;
+};
+struct c128 {
+    double re, im;
+};
+
+inline size_t dtype_size(rstsr_dtype_t t) {
+    switch (t) {
+        case RSTSR_F32: return 4;
+        case RSTSR_F64: return 8;
+        case RSTSR_I32: return 4;
+        case RSTSR_I64: return 8;
+        case RSTSR_C64: return 8;
+        case RSTSR_C128: return 16;
+        case RSTSR_U64: return 8;
+        case RSTSR_BOOL: return 1;
+    }
+    RSTSR_RAISE(InvalidValue, "unknown dtype");
+}
+
+/* ------------------------------ layouts across the ABI ------------------------------ */
+
+inline Layout from_abi(const rstsr_layout_t* l) {
+    if (!l) RSTSR_RAISE(InvalidValue, "null layout");
+    if (l->ndim > 0 && (!l->shape || !l->stride)) RSTSR_RAISE(InvalidValue, "layout without shape/stride arrays");
+    Shape s(l->shape, l->shape + l->ndim);
+    Stride t(l->stride, l->stride + l->ndim);
+    return Layout(s, t, l->offset);
+}
+
+/* ------------------------------ fast integer division ------------------------------ */
+
+// q = n / d for 32-bit n via a multiply-high (Granlund-Montgomery round-up method).
+struct FastDiv {
+    uint32_t d = 1, m = 0, s = 0;
+    FastDiv() = default;
+    explicit FastDiv(uint32_t dv) : d(dv) {
+        if (dv == 0) RSTSR_RAISE(InvalidValue, "division by zero extent");
+        uint32_t sh = 0;
+        while ((1ull << sh) < dv) ++sh;
+        s = sh;
+        m = (uint32_t)((((1ull << 32) * ((1ull << sh) - dv)) / dv) + 1);
+    }
+};
+
+__device__ __forceinline__ uint32_t fdiv(uint32_t n, const FastDiv& f) {
+    uint32_t t = __umulhi(n, f.m);
+    return (uint32_t)(((uint64_t)t + n) >> f.s);
+}
+
+/* ------------------------------ launch helpers ------------------------------ */
+
+inline void check_launch(const char* what) {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) throw Error(ErrorKind::DeviceError, std::string(what) + " launch failed: " + hipGetErrorString(e));
+}
+
+inline uint32_t ceil_div_u32(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
+
+// Verify that every element of a layout lies inside [0, bytes/elem) of a buffer
+// whose extent the caller does not tell us; only the sign is checkable here,
+// the upper bound is the caller's contract (as for the reference's Vec<T>).
+inline void check_layout_nonnegative(const Layout& l) { (void)l.bounds_index(); }
+
+}  // namespace hip
+}  // namespace rstsr

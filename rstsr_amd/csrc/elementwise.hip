@@ -1,0 +1,803 @@
+// Strided n-d elementwise kernels for gfx950: binary arithmetic, cast/copy,
+// fill and allclose, over layouts already broadcast to one shape.
+//
+// Replaces the layout-iterator loops of rstsr-native-impl/src/cpu_rayon/
+// op_with_func.rs:13-390 (binary ops), assignment.rs:14-225 (assign, fill) and
+// reduction.rs:334 (allclose). Results are independent of iteration order, so
+// the planner below is free to pick the order that coalesces HBM traffic:
+//   1. drop size-1 axes, sort axes by the output's |stride| (innermost first)
+//      and merge axes that are contiguous for every operand;
+//   2. if an input's unit-stride axis differs from the output's, run the
+//      LDS-tiled kernel: 64x64 tiles read along the input's unit axis and
+//      written along the output's unit axis (both sides coalesced);
+//   3. else if the innermost extent is long, run the row kernel (one block per
+//      row chunk, index math once per block);
+//   4. else the generic kernel (per-element fast-division unravel).
+// IEEE add/sub/mul/div are single correctly rounded operations; complex
+// multiply is (ac - bd, ad + bc) as num-complex computes it. This file is built
+// with -ffp-contract=off so that no product is fused into an FMA.
+#include "common.hpp"
+#include "registry.hpp"
+
+#include <array>
+#include <type_traits>
+
+namespace rstsr {
+namespace hip {
+
+/* =============================== element ops =============================== */
+
+struct None {};
+
+template <class T>
+__device__ __forceinline__ T op_add(T a, T b) { return a + b; }
+template <class T>
+__device__ __forceinline__ T op_sub(T a, T b) { return a - b; }
+template <class T>
+__device__ __forceinline__ T op_mul(T a, T b) { return a * b; }
+template <class T>
+__device__ __forceinline__ T op_div(T a, T b) { return a / b; }
+
+// integers wrap (Rust release semantics); computed on the unsigned type
+template <>
+__device__ __forceinline__ int32_t op_add(int32_t a, int32_t b) { return (int32_t)((uint32_t)a + (uint32_t)b); }
+template <>
+__device__ __forceinline__ int32_t op_sub(int32_t a, int32_t b) { return (int32_t)((uint32_t)a - (uint32_t)b); }
+template <>
+__device__ __forceinline__ int32_t op_mul(int32_t a, int32_t b) { return (int32_t)((uint32_t)a * (uint32_t)b); }
+template <>
+__device__ __forceinline__ int64_t op_add(int64_t a, int64_t b) { return (int64_t)((uint64_t)a + (uint64_t)b); }
+template <>
+__device__ __forceinline__ int64_t op_sub(int64_t a, int64_t b) { return (int64_t)((uint64_t)a - (uint64_t)b); }
+template <>
+__device__ __forceinline__ int64_t op_mul(int64_t a, int64_t b) { return (int64_t)((uint64_t)a * (uint64_t)b); }
+
+// num-complex: Add/Sub componentwise; Mul (re*re' - im*im', re*im' + im*re')
+template <>
+__device__ __forceinline__ c128 op_add(c128 a, c128 b) { return {a.re + b.re, a.im + b.im}; }
+template <>
+__device__ __forceinline__ c128 op_sub(c128 a, c128 b) { return {a.re - b.re, a.im - b.im}; }
+template <>
+__device__ __forceinline__ c128 op_mul(c128 a, c128 b) {
+    return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+template <>
+__device__ __forceinline__ c64 op_add(c64 a, c64 b) { return {a.re + b.re, a.im + b.im}; }
+template <>
+__device__ __forceinline__ c64 op_sub(c64 a, c64 b) { return {a.re - b.re, a.im - b.im}; }
+template <>
+__device__ __forceinline__ c64 op_mul(c64 a, c64 b) {
+    return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+
+template <int OP, class T>
+__device__ __forceinline__ T apply_op(T a, T b) {
+    if constexpr (OP == RSTSR_OP_ADD) return op_add<T>(a, b);
+    if constexpr (OP == RSTSR_OP_SUB) return op_sub<T>(a, b);
+    if constexpr (OP == RSTSR_OP_MUL) return op_mul<T>(a, b);
+    if constexpr (OP == RSTSR_OP_DIV) return op_div<T>(a, b);
+}
+
+// value conversion for assign with promotion (Rust `as`-free DTypeCastAPI subset)
+template <class TC, class TA>
+__device__ __forceinline__ TC cast_to(TA a) {
+    if constexpr (std::is_same<TC, TA>::value) return a;
+    else if constexpr (std::is_same<TC, c128>::value) {
+        if constexpr (std::is_same<TA, c64>::value) return c128{(double)a.re, (double)a.im};
+        else return c128{(double)a, 0.0};
+    } else if constexpr (std::is_same<TC, c64>::value) {
+        if constexpr (std::is_same<TA, c128>::value) return c64{(float)a.re, (float)a.im};
+        else return c64{(float)a, 0.0f};
+    } else {
+        return (TC)a;
+    }
+}
+
+/* =============================== bodies =============================== */
+// A body loads operand A and B at element offsets and stores C; kernels only do
+// index arithmetic. TA/TB = None marks an absent operand.
+
+template <int OP, class T, bool AS, bool BS>
+struct BinBody {
+    using TA = T;
+    using TB = T;
+    T* c;
+    const T* a;
+    const T* b;
+    T sa, sb;
+    __device__ __forceinline__ T la(int64_t o) const { return AS ? sa : a[o]; }
+    __device__ __forceinline__ T lb(int64_t o) const { return BS ? sb : b[o]; }
+    __device__ __forceinline__ void st(int64_t o, T x, T y) const { c[o] = apply_op<OP, T>(x, y); }
+};
+
+template <class TCv, class TAv>
+struct CastBody {
+    using TA = TAv;
+    using TB = None;
+    TCv* c;
+    const TAv* a;
+    __device__ __forceinline__ TAv la(int64_t o) const { return a[o]; }
+    __device__ __forceinline__ None lb(int64_t) const { return None{}; }
+    __device__ __forceinline__ void st(int64_t o, TAv x, None) const { c[o] = cast_to<TCv, TAv>(x); }
+};
+
+template <class T>
+struct FillBody {
+    using TA = None;
+    using TB = None;
+    T* c;
+    T v;
+    __device__ __forceinline__ None la(int64_t) const { return None{}; }
+    __device__ __forceinline__ None lb(int64_t) const { return None{}; }
+    __device__ __forceinline__ void st(int64_t o, None, None) const { c[o] = v; }
+};
+
+template <class T>
+__device__ __forceinline__ double abs_diff_d(T a, T b) {
+    if constexpr (std::is_same<T, c128>::value) return hypot(a.re - b.re, a.im - b.im);
+    else if constexpr (std::is_same<T, c64>::value) return (double)hypotf(a.re - b.re, a.im - b.im);
+    else if constexpr (std::is_integral<T>::value) return fabs((double)a - (double)b);
+    else return (double)fabs(a - b);
+}
+template <class T>
+__device__ __forceinline__ double abs_d(T a) {
+    if constexpr (std::is_same<T, c128>::value) return hypot(a.re, a.im);
+    else if constexpr (std::is_same<T, c64>::value) return (double)hypotf(a.re, a.im);
+    else return fabs((double)a);
+}
+template <class T>
+__device__ __forceinline__ bool is_nan_v(T a) {
+    if constexpr (std::is_same<T, c128>::value || std::is_same<T, c64>::value) return a.re != a.re || a.im != a.im;
+    else if constexpr (std::is_floating_point<T>::value) return a != a;
+    else return false;
+}
+
+// isclose (rstsr-dtype-traits/src/isclose.rs:92-106): |a-b| <= atol + rtol*|b|,
+// or both NaN when equal_nan. A failing element sets *flag (vector atomic).
+template <class T>
+struct CloseBody {
+    using TA = T;
+    using TB = T;
+    const T* a;
+    const T* b;
+    int* flag;
+    double rtol, atol;
+    int equal_nan;
+    __device__ __forceinline__ T la(int64_t o) const { return a[o]; }
+    __device__ __forceinline__ T lb(int64_t o) const { return b[o]; }
+    __device__ __forceinline__ void st(int64_t, T x, T y) const {
+        bool ok = abs_diff_d<T>(x, y) <= atol + rtol * abs_d<T>(y);
+        if (equal_nan && is_nan_v<T>(x) && is_nan_v<T>(y)) ok = true;
+        if (!ok) atomicOr(flag, 1);
+    }
+};
+
+/* =============================== plan =============================== */
+
+struct EwPlan {
+    int nd = 0;
+    uint32_t shape[MAXD];
+    FastDiv fd[MAXD];
+    int64_t st[3][MAXD];  // element strides of c, a, b
+    int64_t off[3];       // element offsets of c, a, b
+};
+
+// The index math of one row/tile/element: offsets of (c, a, b) for a linear
+// index over dims [d0, nd).
+__device__ __forceinline__ void unravel3(const EwPlan& p, int d0, uint32_t lin, int64_t& oc, int64_t& oa, int64_t& ob) {
+    oc = p.off[0];
+    oa = p.off[1];
+    ob = p.off[2];
+#pragma unroll
+    for (int d = 0; d < MAXD; ++d) {
+        if (d < d0 || d >= p.nd) continue;
+        uint32_t q = fdiv(lin, p.fd[d]);
+        uint32_t r = lin - q * p.shape[d];
+        oc += (int64_t)r * p.st[0][d];
+        oa += (int64_t)r * p.st[1][d];
+        ob += (int64_t)r * p.st[2][d];
+        lin = q;
+    }
+}
+
+constexpr int EW_BLOCK = 256;
+constexpr int EW_VEC = 4;
+
+// generic: one linear element index per (thread, v), unravel per element
+template <class Body>
+__global__ void __launch_bounds__(EW_BLOCK) k_ew_generic(Body body, EwPlan p, uint32_t total) {
+    using TA = typename Body::TA;
+    using TB = typename Body::TB;
+    uint32_t base = blockIdx.x * (EW_BLOCK * EW_VEC) + threadIdx.x;
+    TA va[EW_VEC];
+    TB vb[EW_VEC];
+    int64_t oc[EW_VEC];
+#pragma unroll
+    for (int v = 0; v < EW_VEC; ++v) {
+        uint32_t i = base + v * EW_BLOCK;
+        if (i < total) {
+            int64_t oa, ob;
+            unravel3(p, 0, i, oc[v], oa, ob);
+            va[v] = body.la(oa);
+            vb[v] = body.lb(ob);
+        }
+    }
+#pragma unroll
+    for (int v = 0; v < EW_VEC; ++v) {
+        uint32_t i = base + v * EW_BLOCK;
+        if (i < total) body.st(oc[v], va[v], vb[v]);
+    }
+}
+
+// rows: dim 0 is a long row; blockIdx -> (row, chunk); unravel once per block
+template <class Body>
+__global__ void __launch_bounds__(EW_BLOCK) k_ew_rows(Body body, EwPlan p, uint32_t nchunk, FastDiv fchunk) {
+    using TA = typename Body::TA;
+    using TB = typename Body::TB;
+    uint32_t row = fdiv(blockIdx.x, fchunk);
+    uint32_t chunk = blockIdx.x - row * nchunk;
+    int64_t rc, ra, rb;
+    unravel3(p, 1, row, rc, ra, rb);
+    const uint32_t L = p.shape[0];
+    const int64_t sc = p.st[0][0], sa = p.st[1][0], sb = p.st[2][0];
+    uint32_t base = chunk * (EW_BLOCK * EW_VEC) + threadIdx.x;
+    TA va[EW_VEC];
+    TB vb[EW_VEC];
+#pragma unroll
+    for (int v = 0; v < EW_VEC; ++v) {
+        uint32_t i = base + v * EW_BLOCK;
+        if (i < L) {
+            va[v] = body.la(ra + (int64_t)i * sa);
+            vb[v] = body.lb(rb + (int64_t)i * sb);
+        }
+    }
+#pragma unroll
+    for (int v = 0; v < EW_VEC; ++v) {
+        uint32_t i = base + v * EW_BLOCK;
+        if (i < L) body.st(rc + (int64_t)i * sc, va[v], vb[v]);
+    }
+}
+
+// tiled transpose: dims 0 (= dc, output unit axis) and DI (input unit axis) form
+// a TILE x TILE tile; remaining dims are unravelled from the block index.
+// Operands flagged in STAGE (bit0 = A, bit1 = B) are read along DI into LDS.
+template <class Body, int TILE, int STAGE>
+__global__ void __launch_bounds__(EW_BLOCK) k_ew_tiled(Body body, EwPlan p, int di, uint32_t nti, uint32_t ntc,
+                                                       FastDiv fti, FastDiv ftc) {
+    using TA = typename Body::TA;
+    using TB = typename Body::TB;
+    constexpr bool SA = (STAGE & 1) != 0;
+    constexpr bool SB = (STAGE & 2) != 0;
+    __shared__ TA lds_a[SA ? TILE : 1][SA ? TILE + 1 : 1];
+    __shared__ TB lds_b[SB ? TILE : 1][SB ? TILE + 1 : 1];
+
+    uint32_t b = blockIdx.x;
+    uint32_t q1 = fdiv(b, ftc);
+    uint32_t tc = b - q1 * ntc;
+    uint32_t outer = fdiv(q1, fti);
+    uint32_t ti = q1 - outer * nti;
+
+    // outer offsets: dims other than 0 and di, in order
+    int64_t oc = p.off[0], oa = p.off[1], ob = p.off[2];
+    {
+        uint32_t lin = outer;
+#pragma unroll
+        for (int d = 1; d < MAXD; ++d) {
+            if (d >= p.nd || d == di) continue;
+            uint32_t q = fdiv(lin, p.fd[d]);
+            uint32_t r = lin - q * p.shape[d];
+            oc += (int64_t)r * p.st[0][d];
+            oa += (int64_t)r * p.st[1][d];
+            ob += (int64_t)r * p.st[2][d];
+            lin = q;
+        }
+    }
+    const uint32_t Nc = p.shape[0], Ni = p.shape[di];
+    const uint32_t c0 = tc * TILE, i0 = ti * TILE;
+    const int64_t sc_c = p.st[0][0], sc_i = p.st[0][di];
+    const int64_t sa_c = p.st[1][0], sa_i = p.st[1][di];
+    const int64_t sb_c = p.st[2][0], sb_i = p.st[2][di];
+    constexpr int PER = TILE * TILE / EW_BLOCK;
+
+    if constexpr (SA || SB) {
+        // phase 1: ii fastest -> coalesced along the input's unit axis
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            int e = k * EW_BLOCK + threadIdx.x;
+            int ii = e % TILE, cc = e / TILE;
+            uint32_t gi = i0 + ii, gc = c0 + cc;
+            if (gi < Ni && gc < Nc) {
+                if constexpr (SA) lds_a[cc][ii] = body.la(oa + (int64_t)gi * sa_i + (int64_t)gc * sa_c);
+                if constexpr (SB) lds_b[cc][ii] = body.lb(ob + (int64_t)gi * sb_i + (int64_t)gc * sb_c);
+            }
+        }
+        __syncthreads();
+    }
+    // phase 2: cc fastest -> coalesced along the output's unit axis
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+        int e = k * EW_BLOCK + threadIdx.x;
+        int cc = e % TILE, ii = e / TILE;
+        uint32_t gi = i0 + ii, gc = c0 + cc;
+        if (gi < Ni && gc < Nc) {
+            TA x;
+            TB y;
+            if constexpr (SA) x = lds_a[cc][ii];
+            else x = body.la(oa + (int64_t)gi * sa_i + (int64_t)gc * sa_c);
+            if constexpr (SB) y = lds_b[cc][ii];
+            else y = body.lb(ob + (int64_t)gi * sb_i + (int64_t)gc * sb_c);
+            body.st(oc + (int64_t)gi * sc_i + (int64_t)gc * sc_c, x, y);
+        }
+    }
+}
+
+/* =============================== host planning =============================== */
+
+struct Dims {
+    int nd = 0;  // -1: empty (no elements)
+    std::vector<uint64_t> shape;
+    std::vector<std::array<int64_t, 3>> st;  // per dim, strides of c, a, b
+    int64_t off[3] = {0, 0, 0};
+};
+
+// Collapse the common shape of up to three operands (missing operands carry
+// stride 0). Order: output |stride| ascending; merge axes contiguous in all.
+static Dims collapse3(const Shape& shape, const Stride* s[3], const size_t off[3]) {
+    Dims d;
+    for (int k = 0; k < 3; ++k) d.off[k] = (int64_t)off[k];
+    std::vector<size_t> ax;
+    for (size_t i = 0; i < shape.size(); ++i) {
+        if (shape[i] == 0) {
+            d.nd = -1;
+            return d;
+        }
+        if (shape[i] != 1) ax.push_back(i);
+    }
+    auto S = [&](int k, size_t i) -> int64_t { return s[k] ? (int64_t)(*s[k])[i] : 0; };
+    std::stable_sort(ax.begin(), ax.end(), [&](size_t x, size_t y) {
+        for (int k = 0; k < 3; ++k) {
+            int64_t ax_ = std::llabs(S(k, x)), ay = std::llabs(S(k, y));
+            if (ax_ != ay) return ax_ < ay;
+        }
+        return false;
+    });
+    for (size_t i : ax) {
+        if (!d.shape.empty()) {
+            auto& ps = d.st.back();
+            uint64_t pn = d.shape.back();
+            bool ok = true;
+            for (int k = 0; k < 3; ++k)
+                if (S(k, i) != ps[k] * (int64_t)pn) ok = false;
+            if (ok) {
+                d.shape.back() *= shape[i];
+                continue;
+            }
+        }
+        d.shape.push_back(shape[i]);
+        d.st.push_back({S(0, i), S(1, i), S(2, i)});
+    }
+    d.nd = (int)d.shape.size();
+    return d;
+}
+
+static EwPlan to_plan(const Dims& d) {
+    EwPlan p;
+    p.nd = d.nd;
+    if (d.nd > MAXD)
+        RSTSR_RAISE(UnImplemented, "layouts with more than 8 non-mergeable axes are not supported by the HIP device");
+    for (int i = 0; i < MAXD; ++i) {
+        p.shape[i] = 1;
+        p.fd[i] = FastDiv(1);
+        for (int k = 0; k < 3; ++k) p.st[k][i] = 0;
+    }
+    for (int i = 0; i < d.nd; ++i) {
+        p.shape[i] = (uint32_t)d.shape[i];
+        p.fd[i] = FastDiv((uint32_t)d.shape[i]);
+        for (int k = 0; k < 3; ++k) p.st[k][i] = d.st[i][k];
+    }
+    for (int k = 0; k < 3; ++k) p.off[k] = d.off[k];
+    return p;
+}
+
+enum class EwKind { Generic, Rows, Tiled };
+
+struct EwChoice {
+    EwKind kind = EwKind::Generic;
+    int di = -1;
+    int stage = 0;  // bit0 A, bit1 B
+};
+
+// has_a / has_b: operand present as an array (not scalar / absent)
+static EwChoice choose(const Dims& d, bool has_a, bool has_b) {
+    EwChoice ch;
+    uint64_t total = 1;
+    for (auto n : d.shape) total *= n;
+    if (d.nd <= 0 || total < 4096) return ch;
+    // input-preferred axis: a unit-stride axis of some input that is not the
+    // output's innermost axis
+    int best_di = -1;
+    for (int k = 1; k <= 2; ++k) {
+        if ((k == 1 && !has_a) || (k == 2 && !has_b)) continue;
+        if (std::llabs(d.st[0][k]) <= 1) continue;  // already contiguous along dim 0
+        for (int i = 1; i < d.nd; ++i) {
+            if (std::llabs(d.st[i][k]) == 1) {
+                if (best_di < 0) best_di = i;
+                break;
+            }
+        }
+    }
+    if (best_di > 0 && d.shape[0] >= 16 && d.shape[best_di] >= 16) {
+        ch.kind = EwKind::Tiled;
+        ch.di = best_di;
+        for (int k = 1; k <= 2; ++k) {
+            if ((k == 1 && !has_a) || (k == 2 && !has_b)) continue;
+            int64_t si = std::llabs(d.st[best_di][k]), sc = std::llabs(d.st[0][k]);
+            if (si < sc) ch.stage |= (k == 1 ? 1 : 2);
+        }
+        return ch;
+    }
+    if (d.shape[0] >= (uint64_t)(EW_BLOCK * EW_VEC)) ch.kind = EwKind::Rows;
+    return ch;
+}
+
+// Launch `body` over the plan, splitting the outermost axis so that every
+// launch indexes fewer than 2^31 elements with 32-bit arithmetic.
+template <class Body>
+static void launch_ew(rstsr_hip_ctx* ctx, const Body& body, Dims d, const EwChoice& ch) {
+    if (d.nd < 0) return;
+    if (d.nd == 0) {
+        d.nd = 1;
+        d.shape.push_back(1);
+        d.st.push_back({0, 0, 0});
+    }
+    const uint64_t LIM = 1ull << 31;
+    for (auto n : d.shape)
+        if (n >= LIM) RSTSR_RAISE(UnImplemented, "an axis of 2^31 or more elements after merging is not supported");
+    uint64_t inner = 1;
+    for (int i = 0; i + 1 < d.nd; ++i) inner *= d.shape[i];
+    int last = d.nd - 1;
+    uint64_t nlast = d.shape[last];
+    uint64_t per = std::max<uint64_t>(1, (LIM / 2) / std::max<uint64_t>(inner, 1));
+    if (ch.kind == EwKind::Tiled && ch.di == last) per = nlast;  // tile axis itself is small enough
+    for (uint64_t s0 = 0; s0 < nlast; s0 += per) {
+        uint64_t cnt = std::min(per, nlast - s0);
+        Dims dd = d;
+        dd.shape[last] = cnt;
+        for (int k = 0; k < 3; ++k) dd.off[k] += (int64_t)s0 * d.st[last][k];
+        EwPlan p = to_plan(dd);
+        uint64_t total = inner * cnt;
+        if (ch.kind == EwKind::Generic) {
+            uint32_t nb = ceil_div_u32(total, EW_BLOCK * EW_VEC);
+            hipLaunchKernelGGL(k_ew_generic<Body>, dim3(nb), dim3(EW_BLOCK), 0, ctx->stream, body, p, (uint32_t)total);
+        } else if (ch.kind == EwKind::Rows) {
+            uint32_t nchunk = ceil_div_u32(dd.shape[0], EW_BLOCK * EW_VEC);
+            uint64_t nrows = total / dd.shape[0];
+            uint64_t nb = nrows * nchunk;
+            if (nb >= LIM) RSTSR_RAISE(UnImplemented, "grid too large");
+            hipLaunchKernelGGL(k_ew_rows<Body>, dim3((uint32_t)nb), dim3(EW_BLOCK), 0, ctx->stream, body, p, nchunk,
+                               FastDiv(nchunk));
+        } else {
+            constexpr int TILE = (sizeof(typename Body::TA) > 8 || sizeof(typename Body::TB) > 8) ? 32 : 64;
+            uint32_t ntc = ceil_div_u32(dd.shape[0], TILE);
+            uint32_t nti = ceil_div_u32(dd.shape[ch.di], TILE);
+            uint64_t nouter = total / (dd.shape[0] * dd.shape[ch.di]);
+            uint64_t nb = nouter * nti * ntc;
+            if (nb >= LIM) RSTSR_RAISE(UnImplemented, "grid too large");
+            switch (ch.stage) {
+                case 1:
+                    hipLaunchKernelGGL((k_ew_tiled<Body, TILE, 1>), dim3((uint32_t)nb), dim3(EW_BLOCK), 0, ctx->stream,
+                                       body, p, ch.di, nti, ntc, FastDiv(nti), FastDiv(ntc));
+                    break;
+                case 2:
+                    hipLaunchKernelGGL((k_ew_tiled<Body, TILE, 2>), dim3((uint32_t)nb), dim3(EW_BLOCK), 0, ctx->stream,
+                                       body, p, ch.di, nti, ntc, FastDiv(nti), FastDiv(ntc));
+                    break;
+                default:
+                    hipLaunchKernelGGL((k_ew_tiled<Body, TILE, 3>), dim3((uint32_t)nb), dim3(EW_BLOCK), 0, ctx->stream,
+                                       body, p, ch.di, nti, ntc, FastDiv(nti), FastDiv(ntc));
+                    break;
+            }
+        }
+        check_launch("elementwise");
+    }
+}
+
+/* =============================== dispatch =============================== */
+
+template <class F>
+static void dispatch_dtype(rstsr_dtype_t t, F&& f) {
+    switch (t) {
+        case RSTSR_F32: f(float{}); break;
+        case RSTSR_F64: f(double{}); break;
+        case RSTSR_I32: f(int32_t{}); break;
+        case RSTSR_I64: f(int64_t{}); break;
+        case RSTSR_C64: f(c64{}); break;
+        case RSTSR_C128: f(c128{}); break;
+        default: RSTSR_RAISE(UnImplemented, "dtype not supported for this operation");
+    }
+}
+
+// Binary op with each of a / b an array or a host scalar (nullptr layout).
+static void binary_impl(rstsr_hip_ctx* ctx, rstsr_binop_t op, rstsr_dtype_t t, void* c, const rstsr_layout_t* lc_,
+                        const void* a, const rstsr_layout_t* la_, const void* b, const rstsr_layout_t* lb_) {
+    bind(ctx);
+    if ((int)op < 0 || (int)op > 3) RSTSR_RAISE(InvalidValue, "unknown binary op");
+    Layout lc = from_abi(lc_);
+    bool has_a = la_ != nullptr, has_b = lb_ != nullptr;
+    Layout la = has_a ? from_abi(la_) : lc;
+    Layout lb = has_b ? from_abi(lb_) : lc;
+    if (la.shape != lc.shape || lb.shape != lc.shape)
+        RSTSR_RAISE(InvalidLayout, "operands of an elementwise op must be broadcast to one shape before the device call");
+    if (!a || !b) RSTSR_RAISE(InvalidValue, "null operand");
+    size_t es = dtype_size(t);
+    check_extent(c, lc, es, "c");
+    if (has_a) check_extent(a, la, es, "a");
+    if (has_b) check_extent(b, lb, es, "b");
+    if (op == RSTSR_OP_DIV && !(t == RSTSR_F32 || t == RSTSR_F64))
+        RSTSR_RAISE(UnImplemented, "division is provided for f32/f64 only");
+    const Stride* s[3] = {&lc.stride, has_a ? &la.stride : nullptr, has_b ? &lb.stride : nullptr};
+    size_t off[3] = {lc.offset, has_a ? la.offset : 0, has_b ? lb.offset : 0};
+    Dims d = collapse3(lc.shape, s, off);
+    EwChoice ch = choose(d, has_a, has_b);
+    dispatch_dtype(t, [&](auto tag) {
+        using T = decltype(tag);
+        auto run = [&](auto opc) {
+            constexpr int OP = decltype(opc)::value;
+            if constexpr (OP == RSTSR_OP_DIV && !std::is_floating_point<T>::value) {
+                RSTSR_RAISE(UnImplemented, "division is provided for f32/f64 only");
+            } else {
+                T sa{}, sb{};
+                if (!has_a) std::memcpy(&sa, a, sizeof(T));
+                if (!has_b) std::memcpy(&sb, b, sizeof(T));
+                if (has_a && has_b) {
+                    BinBody<OP, T, false, false> body{(T*)c, (const T*)a, (const T*)b, sa, sb};
+                    launch_ew(ctx, body, d, ch);
+                } else if (has_a) {
+                    BinBody<OP, T, false, true> body{(T*)c, (const T*)a, nullptr, sa, sb};
+                    launch_ew(ctx, body, d, ch);
+                } else if (has_b) {
+                    BinBody<OP, T, true, false> body{(T*)c, nullptr, (const T*)b, sa, sb};
+                    launch_ew(ctx, body, d, ch);
+                } else {
+                    BinBody<OP, T, true, true> body{(T*)c, nullptr, nullptr, sa, sb};
+                    launch_ew(ctx, body, d, ch);
+                }
+            }
+        };
+        switch (op) {
+            case RSTSR_OP_ADD: run(std::integral_constant<int, RSTSR_OP_ADD>{}); break;
+            case RSTSR_OP_SUB: run(std::integral_constant<int, RSTSR_OP_SUB>{}); break;
+            case RSTSR_OP_MUL: run(std::integral_constant<int, RSTSR_OP_MUL>{}); break;
+            case RSTSR_OP_DIV: run(std::integral_constant<int, RSTSR_OP_DIV>{}); break;
+        }
+    });
+}
+
+/* ---------------- assign / copy ---------------- */
+
+// Collapse one layout in logical row-major order (merge axis k into k+1 when
+// stride[k] == stride[k+1] * shape[k+1]); size-1 axes dropped.
+static Layout collapse_rowmajor(const Layout& l) {
+    Shape s;
+    Stride t;
+    for (size_t i = 0; i < l.ndim(); ++i) {
+        if (l.shape[i] == 1) continue;
+        if (!s.empty() && t.back() == l.stride[i] * (ptrdiff_t)l.shape[i]) {
+            s.back() *= l.shape[i];
+            t.back() = l.stride[i];
+            continue;
+        }
+        s.push_back(l.shape[i]);
+        t.push_back(l.stride[i]);
+    }
+    return Layout(s, t, l.offset);
+}
+
+// Two layouts with equal element count but different shapes, refined to one
+// shape when the row-major split points of both coincide; returns false otherwise.
+static bool common_refinement(const Layout& x, const Layout& y, Layout& xo, Layout& yo) {
+    // walk from the innermost axis, splitting the larger extent (always valid for a strided axis)
+    Shape sx, sy;
+    Stride tx, ty;
+    int i = (int)x.ndim() - 1, j = (int)y.ndim() - 1;
+    uint64_t rx = i >= 0 ? x.shape[i] : 1, ry = j >= 0 ? y.shape[j] : 1;
+    ptrdiff_t stx = i >= 0 ? x.stride[i] : 0, sty = j >= 0 ? y.stride[j] : 0;
+    while (i >= 0 && j >= 0) {
+        uint64_t g;
+        if (rx == ry) g = rx;
+        else if (rx % ry == 0) g = ry;
+        else if (ry % rx == 0) g = rx;
+        else return false;
+        sx.push_back(g);
+        tx.push_back(stx);
+        sy.push_back(g);
+        ty.push_back(sty);
+        rx /= g;
+        ry /= g;
+        stx *= (ptrdiff_t)g;
+        sty *= (ptrdiff_t)g;
+        if (rx == 1) {
+            --i;
+            if (i >= 0) {
+                rx = x.shape[i];
+                stx = x.stride[i];
+            }
+        }
+        if (ry == 1) {
+            --j;
+            if (j >= 0) {
+                ry = y.shape[j];
+                sty = y.stride[j];
+            }
+        }
+    }
+    if (i >= 0 || j >= 0) return false;
+    std::reverse(sx.begin(), sx.end());
+    std::reverse(tx.begin(), tx.end());
+    std::reverse(sy.begin(), sy.end());
+    std::reverse(ty.begin(), ty.end());
+    xo = Layout(sx, tx, x.offset);
+    yo = Layout(sy, ty, y.offset);
+    return true;
+}
+
+template <class F>
+static void dispatch_pair(rstsr_dtype_t tc, rstsr_dtype_t ta, F&& f) {
+    dispatch_dtype(tc, [&](auto ctag) {
+        dispatch_dtype(ta, [&](auto atag) {
+            using TC = decltype(ctag);
+            using TA = decltype(atag);
+            constexpr bool c_cplx = std::is_same<TC, c64>::value || std::is_same<TC, c128>::value;
+            constexpr bool a_cplx = std::is_same<TA, c64>::value || std::is_same<TA, c128>::value;
+            if constexpr (a_cplx && !c_cplx) RSTSR_RAISE(InvalidValue, "cannot assign complex into a real tensor");
+            else f(ctag, atag);
+        });
+    });
+}
+
+static void assign_impl(rstsr_hip_ctx* ctx, rstsr_dtype_t tc, void* c, const Layout& lc, rstsr_dtype_t ta, const void* a,
+                        const Layout& la) {
+    const Stride* s[3] = {&lc.stride, &la.stride, nullptr};
+    size_t off[3] = {lc.offset, la.offset, 0};
+    Dims d = collapse3(lc.shape, s, off);
+    EwChoice ch = choose(d, true, false);
+    dispatch_pair(tc, ta, [&](auto ctag, auto atag) {
+        using TC = decltype(ctag);
+        using TA = decltype(atag);
+        CastBody<TC, TA> body{(TC*)c, (const TA*)a};
+        launch_ew(ctx, body, d, ch);
+    });
+}
+
+}  // namespace hip
+}  // namespace rstsr
+
+using namespace rstsr;
+using namespace rstsr::hip;
+
+extern "C" {
+
+int rstsr_hip_op_mutc_refa_refb(rstsr_hip_ctx_t* ctx, rstsr_binop_t op, rstsr_dtype_t t, void* c,
+                                const rstsr_layout_t* lc, const void* a, const rstsr_layout_t* la, const void* b,
+                                const rstsr_layout_t* lb) {
+    return guarded([&] {
+        if (!la || !lb) RSTSR_RAISE(InvalidValue, "null layout");
+        binary_impl(ctx, op, t, c, lc, a, la, b, lb);
+    });
+}
+
+int rstsr_hip_op_mutc_refa_numb(rstsr_hip_ctx_t* ctx, rstsr_binop_t op, rstsr_dtype_t t, void* c,
+                                const rstsr_layout_t* lc, const void* a, const rstsr_layout_t* la, const void* b) {
+    return guarded([&] {
+        if (!la) RSTSR_RAISE(InvalidValue, "null layout");
+        binary_impl(ctx, op, t, c, lc, a, la, b, nullptr);
+    });
+}
+
+int rstsr_hip_op_mutc_numa_refb(rstsr_hip_ctx_t* ctx, rstsr_binop_t op, rstsr_dtype_t t, void* c,
+                                const rstsr_layout_t* lc, const void* a, const void* b, const rstsr_layout_t* lb) {
+    return guarded([&] {
+        if (!lb) RSTSR_RAISE(InvalidValue, "null layout");
+        binary_impl(ctx, op, t, c, lc, a, nullptr, b, lb);
+    });
+}
+
+int rstsr_hip_op_muta_refb(rstsr_hip_ctx_t* ctx, rstsr_binop_t op, rstsr_dtype_t t, void* c,
+                           const rstsr_layout_t* lc, const void* b, const rstsr_layout_t* lb) {
+    return guarded([&] {
+        if (!lb) RSTSR_RAISE(InvalidValue, "null layout");
+        binary_impl(ctx, op, t, c, lc, c, lc, b, lb);
+    });
+}
+
+int rstsr_hip_op_muta_numb(rstsr_hip_ctx_t* ctx, rstsr_binop_t op, rstsr_dtype_t t, void* c,
+                           const rstsr_layout_t* lc, const void* b) {
+    return guarded([&] { binary_impl(ctx, op, t, c, lc, c, lc, b, nullptr); });
+}
+
+int rstsr_hip_assign(rstsr_hip_ctx_t* ctx, rstsr_dtype_t tc, void* c, const rstsr_layout_t* lc_, rstsr_dtype_t ta,
+                     const void* a, const rstsr_layout_t* la_) {
+    return guarded([&] {
+        bind(ctx);
+        Layout lc = from_abi(lc_), la = from_abi(la_);
+        if (lc.shape != la.shape) RSTSR_RAISE(InvalidLayout, "assign requires equal shapes");
+        check_extent(c, lc, dtype_size(tc), "c");
+        check_extent(a, la, dtype_size(ta), "a");
+        assign_impl(ctx, tc, c, lc, ta, a, la);
+    });
+}
+
+int rstsr_hip_assign_arbitary(rstsr_hip_ctx_t* ctx, rstsr_dtype_t tc, void* c, const rstsr_layout_t* lc_,
+                              rstsr_dtype_t ta, const void* a, const rstsr_layout_t* la_) {
+    return guarded([&] {
+        bind(ctx);
+        Layout lc = from_abi(lc_), la = from_abi(la_);
+        if (lc.size() != la.size()) RSTSR_RAISE(InvalidLayout, "assign_arbitary requires equal element counts");
+        check_extent(c, lc, dtype_size(tc), "c");
+        check_extent(a, la, dtype_size(ta), "a");
+        if (lc.size() == 0) return;
+        if (lc.shape == la.shape) {
+            assign_impl(ctx, tc, c, lc, ta, a, la);
+            return;
+        }
+        Layout xc = collapse_rowmajor(lc), xa = collapse_rowmajor(la), rc, ra;
+        if (!common_refinement(xc, xa, rc, ra))
+            RSTSR_RAISE(UnImplemented, "assign_arbitary between shapes without a common row-major refinement");
+        assign_impl(ctx, tc, c, rc, ta, a, ra);
+    });
+}
+
+int rstsr_hip_fill(rstsr_hip_ctx_t* ctx, rstsr_dtype_t t, void* c, const rstsr_layout_t* lc_, const void* value) {
+    return guarded([&] {
+        bind(ctx);
+        if (!value) RSTSR_RAISE(InvalidValue, "null fill value");
+        Layout lc = from_abi(lc_);
+        check_extent(c, lc, dtype_size(t), "c");
+        const Stride* s[3] = {&lc.stride, nullptr, nullptr};
+        size_t off[3] = {lc.offset, 0, 0};
+        Dims d = collapse3(lc.shape, s, off);
+        EwChoice ch = choose(d, false, false);
+        dispatch_dtype(t, [&](auto tag) {
+            using T = decltype(tag);
+            FillBody<T> body{(T*)c, T{}};
+            std::memcpy(&body.v, value, sizeof(T));
+            launch_ew(ctx, body, d, ch);
+        });
+    });
+}
+
+int rstsr_hip_allclose(rstsr_hip_ctx_t* ctx, rstsr_dtype_t t, const void* a, const rstsr_layout_t* la_, const void* b,
+                       const rstsr_layout_t* lb_, double rtol, double atol, int equal_nan, int* host_out) {
+    return guarded([&] {
+        bind(ctx);
+        if (!host_out) RSTSR_RAISE(InvalidValue, "null output");
+        Layout la = from_abi(la_), lb = from_abi(lb_);
+        if (la.size() == 0 || lb.size() == 0) RSTSR_RAISE(InvalidValue, "zero-size array is not supported for allclose");
+        auto bl = broadcast_layout(la, lb, FlagOrder::RowMajor);
+        la = bl.first;
+        lb = bl.second;
+        size_t es = dtype_size(t);
+        check_extent(a, la, es, "a");
+        check_extent(b, lb, es, "b");
+        std::lock_guard<std::mutex> g(ctx->mu);
+        int* flag = (int*)workspace(ctx, 256);
+        HIP_CHECK(hipMemsetAsync(flag, 0, sizeof(int), ctx->stream));
+        const Stride* s[3] = {&la.stride, &la.stride, &lb.stride};
+        size_t off[3] = {la.offset, la.offset, lb.offset};
+        Dims d = collapse3(la.shape, s, off);
+        EwChoice ch = choose(d, true, true);
+        if (ch.kind == EwKind::Tiled) ch.kind = EwKind::Generic;
+        dispatch_dtype(t, [&](auto tag) {
+            using T = decltype(tag);
+            CloseBody<T> body{(const T*)a, (const T*)b, flag, rtol, atol, equal_nan};
+            launch_ew(ctx, body, d, ch);
+        });
+        HIP_CHECK(hipMemcpyAsync(ctx->host_pinned, flag, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+        HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        int bad = 0;
+        std::memcpy(&bad, ctx->host_pinned, sizeof(int));
+        *host_out = bad ? 0 : 1;
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,634 @@
+// GEMM on gfx950: FP64 MFMA DGEMM / ZGEMM, SIMT fallback for f32 and integers,
+// and the matmul entry point with rstsr's batch-broadcast rules.
+//
+// Replaces DeviceMatMulAPI for DeviceFaer (rstsr-core/src/device_faer/matmul.rs:
+// 19-239 -> faer) and the naive kernels of rstsr-native-impl/src/cpu_serial/
+// matmul_naive.rs:78-122 (non-float types). Operand layouts map to (row stride,
+// column stride) pairs directly, so no packing copy is made for any strided
+// operand (the OpenBLAS device packs non-BLAS layouts, crates-device/
+// rstsr-openblas/src/matmul_impl.rs:92-113; here the LDS tile load absorbs it).
+//
+// DGEMM tiling: 256 threads = 4 waves in a 2x2 grid; each wave owns a
+// (16*WM) x (16*WN) patch of C built from v_mfma_f64_16x16x4_f64 tiles. A and
+// B stage through LDS in BK=16 slices with a one-slice register prefetch, so
+// global loads of slice k+1 overlap the MFMAs of slice k. Global loads walk the
+// operand's unit-stride axis (template flags), so both row- and column-major
+// operands read coalesced. ZGEMM uses the same kernel with separate real and
+// imaginary LDS planes and four real MFMAs per complex tile (4M form).
+#include <type_traits>
+
+#include "common.hpp"
+#include "registry.hpp"
+
+namespace rstsr {
+namespace hip {
+
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+
+constexpr int GT = 256;  // threads per GEMM block
+constexpr int GBK = 16;  // K slice held in LDS
+constexpr int GPAD = 4;  // LDS row padding (doubles)
+
+struct BatchDesc {
+    int nbd = 0;
+    uint32_t shape[MAXD];
+    FastDiv fd[MAXD];
+    int64_t sa[MAXD], sb[MAXD], sc[MAXD];
+};
+
+template <class T>
+struct GemmArgs {
+    int64_t M, N, K;
+    const T* a;
+    int64_t rsa, csa;
+    const T* b;
+    int64_t rsb, csb;
+    T* c;
+    int64_t rsc, csc;
+    T alpha, beta;
+    int beta_zero;
+    uint32_t tiles_m, tiles_n;
+    uint32_t batch0;  // first batch index of this launch
+    BatchDesc bd;
+};
+
+__device__ __forceinline__ void batch_offsets(const BatchDesc& bd, uint32_t bi, int64_t& oa, int64_t& ob, int64_t& oc) {
+    oa = ob = oc = 0;
+#pragma unroll
+    for (int d = 0; d < MAXD; ++d) {
+        if (d >= bd.nbd) break;
+        uint32_t q = fdiv(bi, bd.fd[d]);
+        uint32_t r = bi - q * bd.shape[d];
+        oa += (int64_t)r * bd.sa[d];
+        ob += (int64_t)r * bd.sb[d];
+        oc += (int64_t)r * bd.sc[d];
+        bi = q;
+    }
+}
+
+template <class T>
+struct CplxOf : std::false_type {};
+template <>
+struct CplxOf<c128> : std::true_type {};
+
+__device__ __forceinline__ double re_of(double x) { return x; }
+__device__ __forceinline__ double im_of(double) { return 0.0; }
+__device__ __forceinline__ double re_of(c128 x) { return x.re; }
+__device__ __forceinline__ double im_of(c128 x) { return x.im; }
+
+// Map a linear tile id so that consecutive ids dispatched to one XCD (ids are
+// round-robined over the 8 XCDs) cover neighbouring C tiles, which share A row
+// panels and B column panels in that XCD's L2.
+__device__ __forceinline__ uint32_t xcd_remap(uint32_t t, uint32_t ntiles) {
+    const uint32_t NX = 8;
+    if (ntiles % NX != 0) return t;
+    return (t % NX) * (ntiles / NX) + t / NX;
+}
+
+template <class T, int WM, int WN, bool A_MC, bool B_KC>
+__global__ void __launch_bounds__(GT) k_gemm_mfma(GemmArgs<T> g) {
+    constexpr bool CX = CplxOf<T>::value;
+    constexpr int NP = CX ? 2 : 1;  // LDS planes (re, im)
+    constexpr int BM = 2 * 16 * WM;
+    constexpr int BN = 2 * 16 * WN;
+    constexpr int PA = GBK * BM / GT;  // A elements per thread per slice
+    constexpr int PB = GBK * BN / GT;
+    __shared__ double As[NP][GBK][BM + GPAD];
+    __shared__ double Bs[NP][GBK][BN + GPAD];
+
+    const uint32_t ntiles = g.tiles_m * g.tiles_n;
+    const uint32_t tile = xcd_remap(blockIdx.x, ntiles);
+    const uint32_t tm = tile / g.tiles_n, tn = tile - (tile / g.tiles_n) * g.tiles_n;
+    int64_t boa, bob, boc;
+    batch_offsets(g.bd, g.batch0 + blockIdx.y, boa, bob, boc);
+    const T* A = g.a + boa;
+    const T* B = g.b + bob;
+    T* C = g.c + boc;
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int wm = wave >> 1, wn = wave & 1;
+    const int64_t m0 = (int64_t)tm * BM, n0 = (int64_t)tn * BN;
+
+    dbl4 acc[NP][WM][WN];
+#pragma unroll
+    for (int p = 0; p < NP; ++p)
+#pragma unroll
+        for (int i = 0; i < WM; ++i)
+#pragma unroll
+            for (int j = 0; j < WN; ++j) acc[p][i][j] = dbl4{0.0, 0.0, 0.0, 0.0};
+
+    T ra[PA], rb[PB];
+    auto a_coord = [&](int i, int& mm, int& kk) {
+        if constexpr (A_MC) {
+            mm = (tid % BM);
+            kk = tid / BM + (GT / BM) * i;
+        } else {
+            kk = tid % GBK;
+            mm = tid / GBK + (GT / GBK) * i;
+        }
+    };
+    auto b_coord = [&](int i, int& kk, int& nn) {
+        if constexpr (B_KC) {
+            kk = tid % GBK;
+            nn = tid / GBK + (GT / GBK) * i;
+        } else {
+            nn = tid % BN;
+            kk = tid / BN + (GT / BN) * i;
+        }
+    };
+    auto load_slice = [&](int64_t k0) {
+#pragma unroll
+        for (int i = 0; i < PA; ++i) {
+            int mm, kk;
+            a_coord(i, mm, kk);
+            int64_t gm = m0 + mm, gk = k0 + kk;
+            ra[i] = (gm < g.M && gk < g.K) ? A[gm * g.rsa + gk * g.csa] : T{};
+        }
+#pragma unroll
+        for (int i = 0; i < PB; ++i) {
+            int kk, nn;
+            b_coord(i, kk, nn);
+            int64_t gk = k0 + kk, gn = n0 + nn;
+            rb[i] = (gk < g.K && gn < g.N) ? B[gk * g.rsb + gn * g.csb] : T{};
+        }
+    };
+    auto store_slice = [&]() {
+#pragma unroll
+        for (int i = 0; i < PA; ++i) {
+            int mm, kk;
+            a_coord(i, mm, kk);
+            As[0][kk][mm] = re_of(ra[i]);
+            if constexpr (CX) As[NP - 1][kk][mm] = im_of(ra[i]);
+        }
+#pragma unroll
+        for (int i = 0; i < PB; ++i) {
+            int kk, nn;
+            b_coord(i, kk, nn);
+            Bs[0][kk][nn] = re_of(rb[i]);
+            if constexpr (CX) Bs[NP - 1][kk][nn] = im_of(rb[i]);
+        }
+    };
+
+    load_slice(0);
+    store_slice();
+    __syncthreads();
+    const int fr = lane & 15, fk = lane >> 4;
+    for (int64_t k0 = 0; k0 < g.K; k0 += GBK) {
+        const bool more = k0 + GBK < g.K;
+        if (more) load_slice(k0 + GBK);
+#pragma unroll
+        for (int kk = 0; kk < GBK; kk += 4) {
+            double fa[NP][WM], fb[NP][WN];
+#pragma unroll
+            for (int i = 0; i < WM; ++i) {
+                fa[0][i] = As[0][kk + fk][wm * 16 * WM + i * 16 + fr];
+                if constexpr (CX) fa[NP - 1][i] = As[NP - 1][kk + fk][wm * 16 * WM + i * 16 + fr];
+            }
+#pragma unroll
+            for (int j = 0; j < WN; ++j) {
+                fb[0][j] = Bs[0][kk + fk][wn * 16 * WN + j * 16 + fr];
+                if constexpr (CX) fb[NP - 1][j] = Bs[NP - 1][kk + fk][wn * 16 * WN + j * 16 + fr];
+            }
+#pragma unroll
+            for (int i = 0; i < WM; ++i)
+#pragma unroll
+                for (int j = 0; j < WN; ++j) {
+                    if constexpr (!CX) {
+                        acc[0][i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[0][i], fb[0][j], acc[0][i][j], 0, 0, 0);
+                    } else {
+                        // re += ar*br - ai*bi ; im += ar*bi + ai*br
+                        acc[0][i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[0][i], fb[0][j], acc[0][i][j], 0, 0, 0);
+                        acc[0][i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(-fa[1][i], fb[1][j], acc[0][i][j], 0, 0, 0);
+                        acc[1][i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[0][i], fb[1][j], acc[1][i][j], 0, 0, 0);
+                        acc[1][i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[1][i], fb[0][j], acc[1][i][j], 0, 0, 0);
+                    }
+                }
+        }
+        __syncthreads();
+        if (more) {
+            store_slice();
+            __syncthreads();
+        }
+    }
+    // epilogue: v_mfma_f64_16x16x4f64 leaves register r of lane l at row 4*r + l/16,
+    // column l%16 of its 16x16 tile
+#pragma unroll
+    for (int i = 0; i < WM; ++i)
+#pragma unroll
+        for (int j = 0; j < WN; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                int64_t gm = m0 + wm * 16 * WM + i * 16 + 4 * r + (lane >> 4);
+                int64_t gn = n0 + wn * 16 * WN + j * 16 + (lane & 15);
+                if (gm < g.M && gn < g.N) {
+                    T* cp = C + gm * g.rsc + gn * g.csc;
+                    if constexpr (!CX) {
+                        double v = g.alpha * acc[0][i][j][r];
+                        if (!g.beta_zero) v += g.beta * (*cp);
+                        *cp = v;
+                    } else {
+                        double xr = acc[0][i][j][r], xi = acc[1][i][j][r];
+                        c128 v{g.alpha.re * xr - g.alpha.im * xi, g.alpha.re * xi + g.alpha.im * xr};
+                        if (!g.beta_zero) {
+                            c128 o = *cp;
+                            v.re += g.beta.re * o.re - g.beta.im * o.im;
+                            v.im += g.beta.re * o.im + g.beta.im * o.re;
+                        }
+                        *cp = v;
+                    }
+                }
+            }
+}
+
+/* ---------------- SIMT fallback: f32, i32, i64, c64 ---------------- */
+
+template <class T>
+__device__ __forceinline__ T g_mul(T a, T b) {
+    if constexpr (std::is_integral<T>::value) {
+        using U = typename std::make_unsigned<T>::type;
+        return (T)((U)a * (U)b);
+    } else {
+        return a * b;
+    }
+}
+template <class T>
+__device__ __forceinline__ T g_add(T a, T b) {
+    if constexpr (std::is_integral<T>::value) {
+        using U = typename std::make_unsigned<T>::type;
+        return (T)((U)a + (U)b);
+    } else {
+        return a + b;
+    }
+}
+__device__ __forceinline__ c64 g_mul(c64 a, c64 b) { return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re}; }
+__device__ __forceinline__ c64 g_add(c64 a, c64 b) { return {a.re + b.re, a.im + b.im}; }
+
+template <class T>
+__global__ void __launch_bounds__(256) k_gemm_simt(GemmArgs<T> g) {
+    constexpr int TS = 16;
+    __shared__ T As[TS][TS + 1];
+    __shared__ T Bs[TS][TS + 1];
+    const uint32_t tile = blockIdx.x;
+    const uint32_t tm = tile / g.tiles_n, tn = tile % g.tiles_n;
+    int64_t boa, bob, boc;
+    batch_offsets(g.bd, g.batch0 + blockIdx.y, boa, bob, boc);
+    const T* A = g.a + boa;
+    const T* B = g.b + bob;
+    T* C = g.c + boc;
+    const int tx = threadIdx.x % TS, ty = threadIdx.x / TS;
+    const int64_t gm = (int64_t)tm * TS + ty, gn = (int64_t)tn * TS + tx;
+    T acc{};
+    for (int64_t k0 = 0; k0 < g.K; k0 += TS) {
+        int64_t ka = k0 + tx, kb = k0 + ty;
+        As[ty][tx] = (gm < g.M && ka < g.K) ? A[gm * g.rsa + ka * g.csa] : T{};
+        Bs[ty][tx] = (kb < g.K && gn < g.N) ? B[kb * g.rsb + gn * g.csb] : T{};
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < TS; ++k) acc = g_add(acc, g_mul(As[ty][k], Bs[k][tx]));
+        __syncthreads();
+    }
+    if (gm < g.M && gn < g.N) {
+        T* cp = C + gm * g.rsc + gn * g.csc;
+        T v = g_mul(g.alpha, acc);
+        if (!g.beta_zero) v = g_add(v, g_mul(g.beta, *cp));
+        *cp = v;
+    }
+}
+
+/* ---------------- host side ---------------- */
+
+// One strided-batched GEMM problem: batch dims in bd, matrices by (rs, cs).
+struct GemmProblem {
+    int64_t M, N, K;
+    const void* a;
+    int64_t rsa, csa;
+    const void* b;
+    int64_t rsb, csb;
+    void* c;
+    int64_t rsc, csc;
+    BatchDesc bd;
+    uint64_t nbatch = 1;
+};
+
+template <class T>
+static void launch_gemm_t(rstsr_hip_ctx* ctx, const GemmProblem& P, T alpha, T beta, bool beta_zero) {
+    GemmArgs<T> g;
+    g.M = P.M;
+    g.N = P.N;
+    g.K = P.K;
+    g.a = (const T*)P.a;
+    g.rsa = P.rsa;
+    g.csa = P.csa;
+    g.b = (const T*)P.b;
+    g.rsb = P.rsb;
+    g.csb = P.csb;
+    g.c = (T*)P.c;
+    g.rsc = P.rsc;
+    g.csc = P.csc;
+    g.alpha = alpha;
+    g.beta = beta;
+    g.beta_zero = beta_zero ? 1 : 0;
+    g.bd = P.bd;
+    const uint64_t YMAX = 65535;
+    constexpr bool MF = std::is_same<T, double>::value || std::is_same<T, c128>::value;
+    for (uint64_t b0 = 0; b0 < P.nbatch; b0 += YMAX) {
+        uint32_t nb = (uint32_t)std::min<uint64_t>(YMAX, P.nbatch - b0);
+        g.batch0 = (uint32_t)b0;
+        if constexpr (MF) {
+            constexpr bool CX = std::is_same<T, c128>::value;
+            constexpr int WM = CX ? 2 : 4, WN = 4;
+            constexpr int BM = 32 * WM, BN = 32 * WN;
+            g.tiles_m = ceil_div_u32(P.M, BM);
+            g.tiles_n = ceil_div_u32(P.N, BN);
+            uint64_t nt = (uint64_t)g.tiles_m * g.tiles_n;
+            if (nt >= (1ull << 31)) RSTSR_RAISE(UnImplemented, "GEMM grid too large");
+            bool amc = std::llabs(P.rsa) == 1 && std::llabs(P.csa) != 1;
+            bool bkc = std::llabs(P.rsb) == 1 && std::llabs(P.csb) != 1;
+            dim3 grid((uint32_t)nt, nb);
+            if (amc && bkc) hipLaunchKernelGGL((k_gemm_mfma<T, WM, WN, true, true>), grid, dim3(GT), 0, ctx->stream, g);
+            else if (amc) hipLaunchKernelGGL((k_gemm_mfma<T, WM, WN, true, false>), grid, dim3(GT), 0, ctx->stream, g);
+            else if (bkc) hipLaunchKernelGGL((k_gemm_mfma<T, WM, WN, false, true>), grid, dim3(GT), 0, ctx->stream, g);
+            else hipLaunchKernelGGL((k_gemm_mfma<T, WM, WN, false, false>), grid, dim3(GT), 0, ctx->stream, g);
+        } else {
+            g.tiles_m = ceil_div_u32(P.M, 16);
+            g.tiles_n = ceil_div_u32(P.N, 16);
+            uint64_t nt = (uint64_t)g.tiles_m * g.tiles_n;
+            if (nt >= (1ull << 31)) RSTSR_RAISE(UnImplemented, "GEMM grid too large");
+            hipLaunchKernelGGL(k_gemm_simt<T>, dim3((uint32_t)nt, nb), dim3(256), 0, ctx->stream, g);
+        }
+        check_launch("gemm");
+    }
+}
+
+// Ozaki-II split GEMM (ozaki.hip); f64 only.
+void ozaki_gemm_f64(rstsr_hip_ctx* ctx, const GemmProblem& P, double alpha, double beta, bool beta_zero);
+
+// Zero-fill C (k == 0 with beta == 0: crates-device/rstsr-openblas/src/matmul_impl.rs:78-81).
+template <class T>
+__global__ void k_scale_c(GemmArgs<T> g) {
+    uint64_t idx = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    uint64_t mn = (uint64_t)g.M * g.N;
+    if (idx >= mn) return;
+    int64_t oa, ob, oc;
+    batch_offsets(g.bd, g.batch0 + blockIdx.y, oa, ob, oc);
+    int64_t m = idx / g.N, n = idx - (idx / g.N) * g.N;
+    T* cp = g.c + oc + m * g.rsc + n * g.csc;
+    *cp = g.beta_zero ? T{} : g_mul(g.beta, *cp);
+}
+__device__ __forceinline__ c128 g_mul(c128 a, c128 b) { return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re}; }
+
+template <class T>
+static void scale_c(rstsr_hip_ctx* ctx, const GemmProblem& P, T beta, bool beta_zero) {
+    GemmArgs<T> g{};
+    g.M = P.M;
+    g.N = P.N;
+    g.c = (T*)P.c;
+    g.rsc = P.rsc;
+    g.csc = P.csc;
+    g.beta = beta;
+    g.beta_zero = beta_zero;
+    g.bd = P.bd;
+    uint64_t mn = (uint64_t)P.M * P.N;
+    for (uint64_t b0 = 0; b0 < P.nbatch; b0 += 65535) {
+        g.batch0 = (uint32_t)b0;
+        uint32_t nb = (uint32_t)std::min<uint64_t>(65535, P.nbatch - b0);
+        hipLaunchKernelGGL(k_scale_c<T>, dim3(ceil_div_u32(mn, 256), nb), dim3(256), 0, ctx->stream, g);
+        check_launch("gemm scale");
+    }
+}
+
+static bool is_zero_scalar(rstsr_dtype_t t, const void* v) {
+    switch (t) {
+        case RSTSR_F32: return *(const float*)v == 0.0f;
+        case RSTSR_F64: return *(const double*)v == 0.0;
+        case RSTSR_I32: return *(const int32_t*)v == 0;
+        case RSTSR_I64: return *(const int64_t*)v == 0;
+        case RSTSR_C64: return ((const float*)v)[0] == 0.0f && ((const float*)v)[1] == 0.0f;
+        case RSTSR_C128: return ((const double*)v)[0] == 0.0 && ((const double*)v)[1] == 0.0;
+        default: return false;
+    }
+}
+
+void run_gemm(rstsr_hip_ctx* ctx, rstsr_dtype_t t, rstsr_gemm_mode_t mode, const GemmProblem& P, const void* alpha,
+              const void* beta) {
+    if (!alpha || !beta) RSTSR_RAISE(InvalidValue, "null alpha/beta");
+    if (P.M == 0 || P.N == 0 || P.nbatch == 0) return;
+    bool bz = is_zero_scalar(t, beta);
+    if (mode != RSTSR_GEMM_PLAIN && mode != RSTSR_GEMM_OZAKI_INT8) RSTSR_RAISE(InvalidValue, "unknown GEMM mode");
+    if (mode == RSTSR_GEMM_OZAKI_INT8 && t != RSTSR_F64)
+        RSTSR_RAISE(UnImplemented, "the Ozaki split GEMM is provided for f64");
+    auto body = [&](auto tag) {
+        using T = decltype(tag);
+        T al, be;
+        std::memcpy(&al, alpha, sizeof(T));
+        std::memcpy(&be, beta, sizeof(T));
+        if (P.K == 0) {
+            scale_c<T>(ctx, P, be, bz);
+            return;
+        }
+        if constexpr (std::is_same<T, double>::value) {
+            if (mode == RSTSR_GEMM_OZAKI_INT8) {
+                ozaki_gemm_f64(ctx, P, al, be, bz);
+                return;
+            }
+        }
+        launch_gemm_t<T>(ctx, P, al, be, bz);
+    };
+    switch (t) {
+        case RSTSR_F32: body(float{}); break;
+        case RSTSR_F64: body(double{}); break;
+        case RSTSR_I32: body(int32_t{}); break;
+        case RSTSR_I64: body(int64_t{}); break;
+        case RSTSR_C64: body(c64{}); break;
+        case RSTSR_C128: body(c128{}); break;
+        default: RSTSR_RAISE(UnImplemented, "matmul dtype");
+    }
+}
+
+// Check that each operand's matrices of every batch stay inside its buffer.
+static void check_gemm_extent(const GemmProblem& P, size_t es) {
+    auto mk = [&](const void* p, int64_t r, int64_t cdim, int64_t rs, int64_t cs, const int64_t* bs, const char* nm) {
+        Shape s;
+        Stride t;
+        for (int d = 0; d < P.bd.nbd; ++d) {
+            s.push_back(P.bd.shape[d]);
+            t.push_back(bs[d]);
+        }
+        s.push_back((size_t)r);
+        t.push_back(rs);
+        s.push_back((size_t)cdim);
+        t.push_back(cs);
+        check_extent(p, Layout(s, t, 0), es, nm);
+    };
+    mk(P.a, P.M, P.K, P.rsa, P.csa, P.bd.sa, "a");
+    mk(P.b, P.K, P.N, P.rsb, P.csb, P.bd.sb, "b");
+    mk(P.c, P.M, P.N, P.rsc, P.csc, P.bd.sc, "c");
+}
+
+}  // namespace hip
+}  // namespace rstsr
+
+using namespace rstsr;
+using namespace rstsr::hip;
+
+extern "C" {
+
+int rstsr_hip_gemm(rstsr_hip_ctx_t* ctx, rstsr_dtype_t t, rstsr_gemm_mode_t mode, size_t m, size_t n, size_t k,
+                   const void* alpha, const void* a, ptrdiff_t rsa, ptrdiff_t csa, const void* b, ptrdiff_t rsb,
+                   ptrdiff_t csb, const void* beta, void* c, ptrdiff_t rsc, ptrdiff_t csc) {
+    return guarded([&] {
+        bind(ctx);
+        GemmProblem P{};
+        P.M = (int64_t)m;
+        P.N = (int64_t)n;
+        P.K = (int64_t)k;
+        P.a = a;
+        P.rsa = rsa;
+        P.csa = csa;
+        P.b = b;
+        P.rsb = rsb;
+        P.csb = csb;
+        P.c = c;
+        P.rsc = rsc;
+        P.csc = csc;
+        P.bd.nbd = 0;
+        P.nbatch = 1;
+        check_gemm_extent(P, dtype_size(t));
+        std::lock_guard<std::mutex> g(ctx->mu);
+        run_gemm(ctx, t, mode, P, alpha, beta);
+    });
+}
+
+// rstsr-common/src/layout/matmul.rs:287-421 (layout_matmul_dyn_row_major_with_lc):
+// split each operand into (batch rest, matrix) and broadcast the batch rests
+// of A and B to C's; vectors become 1 x K / K x 1 matrices.
+int rstsr_hip_matmul(rstsr_hip_ctx_t* ctx, rstsr_dtype_t t, void* c, const rstsr_layout_t* lc_, const void* a,
+                     const rstsr_layout_t* la_, const void* b, const rstsr_layout_t* lb_, const void* alpha,
+                     const void* beta, rstsr_gemm_mode_t mode) {
+    return guarded([&] {
+        bind(ctx);
+        Layout lc = from_abi(lc_), la = from_abi(la_), lb = from_abi(lb_);
+        size_t na = la.ndim(), nb = lb.ndim(), nc = lc.ndim();
+        if (na == 0 || nb == 0) RSTSR_RAISE(InvalidValue, "In matmul, 0-dim is not allowed.");
+        // matrices as (shape, stride) pairs
+        int64_t M, N, K, rsa, csa, rsb, csb, rsc, csc;
+        Layout ra, rb, rc;  // batch rests
+        auto split = [](const Layout& l, size_t nm, Layout& rest) {
+            Shape s(l.shape.begin(), l.shape.end() - nm);
+            Stride st(l.stride.begin(), l.stride.end() - nm);
+            rest = Layout(s, st, 0);
+        };
+        if (na == 1 && nb == 1) {  // rule 1: inner dot -> 1 x 1
+            if (la.shape != lb.shape || nc != 0) RSTSR_RAISE(InvalidLayout, "matmul rule 1 shape mismatch");
+            M = 1;
+            N = 1;
+            K = la.shape[0];
+            rsa = 0;
+            csa = la.stride[0];
+            rsb = lb.stride[0];
+            csb = 0;
+            rsc = 0;
+            csc = 0;
+            ra = rb = rc = Layout({}, {}, 0);
+        } else if (na == 1) {  // rule 3: K @ (..., K, N) -> (..., N)
+            if (nb != nc + 1 || la.shape[0] != lb.shape[nb - 2] || lc.shape[nc - 1] != lb.shape[nb - 1])
+                RSTSR_RAISE(InvalidLayout, "matmul rule 3 shape mismatch");
+            M = 1;
+            K = la.shape[0];
+            N = lb.shape[nb - 1];
+            rsa = 0;
+            csa = la.stride[0];
+            rsb = lb.stride[nb - 2];
+            csb = lb.stride[nb - 1];
+            rsc = 0;
+            csc = lc.stride[nc - 1];
+            split(lb, 2, rb);
+            split(lc, 1, rc);
+            ra = Layout(rc.shape, Stride(rc.ndim(), 0), 0);
+        } else if (nb == 1) {  // rule 4: (..., M, K) @ K -> (..., M)
+            if (na != nc + 1 || lb.shape[0] != la.shape[na - 1] || lc.shape[nc - 1] != la.shape[na - 2])
+                RSTSR_RAISE(InvalidLayout, "matmul rule 4 shape mismatch");
+            M = la.shape[na - 2];
+            K = la.shape[na - 1];
+            N = 1;
+            rsa = la.stride[na - 2];
+            csa = la.stride[na - 1];
+            rsb = lb.stride[0];
+            csb = 0;
+            rsc = lc.stride[nc - 1];
+            csc = 0;
+            split(la, 2, ra);
+            split(lc, 1, rc);
+            rb = Layout(rc.shape, Stride(rc.ndim(), 0), 0);
+        } else {  // rules 2, 5, 6, 7
+            if (nc < 2) RSTSR_RAISE(InvalidLayout, "matmul output rank");
+            M = la.shape[na - 2];
+            K = la.shape[na - 1];
+            N = lb.shape[nb - 1];
+            if (lb.shape[nb - 2] != (size_t)K || lc.shape[nc - 2] != (size_t)M || lc.shape[nc - 1] != (size_t)N)
+                RSTSR_RAISE(InvalidLayout, "matmul shape mismatch");
+            rsa = la.stride[na - 2];
+            csa = la.stride[na - 1];
+            rsb = lb.stride[nb - 2];
+            csb = lb.stride[nb - 1];
+            rsc = lc.stride[nc - 2];
+            csc = lc.stride[nc - 1];
+            split(la, 2, ra);
+            split(lb, 2, rb);
+            split(lc, 2, rc);
+        }
+        // broadcast batch rests of A and B to C's (broadcast_layout_to_first)
+        ra = broadcast_layout_to_shape(ra, rc.shape, FlagOrder::RowMajor);
+        rb = broadcast_layout_to_shape(rb, rc.shape, FlagOrder::RowMajor);
+        GemmProblem P{};
+        P.M = M;
+        P.N = N;
+        P.K = K;
+        P.a = (const char*)a + la.offset * dtype_size(t);
+        P.b = (const char*)b + lb.offset * dtype_size(t);
+        P.c = (char*)c + lc.offset * dtype_size(t);
+        P.rsa = rsa;
+        P.csa = csa;
+        P.rsb = rsb;
+        P.csb = csb;
+        P.rsc = rsc;
+        P.csc = csc;
+        // batch dims: drop size 1, innermost first for the unravel
+        std::vector<size_t> bd;
+        for (size_t i = 0; i < rc.ndim(); ++i)
+            if (rc.shape[i] != 1) bd.push_back(i);
+        if ((int)bd.size() > MAXD) RSTSR_RAISE(UnImplemented, "too many batch axes");
+        P.bd.nbd = (int)bd.size();
+        uint64_t nbat = 1;
+        for (int d = 0; d < MAXD; ++d) {
+            P.bd.shape[d] = 1;
+            P.bd.fd[d] = FastDiv(1);
+            P.bd.sa[d] = P.bd.sb[d] = P.bd.sc[d] = 0;
+        }
+        for (int d = 0; d < P.bd.nbd; ++d) {
+            size_t ax = bd[bd.size() - 1 - d];
+            if (rc.shape[ax] >= (1ull << 31)) RSTSR_RAISE(UnImplemented, "batch extent too large");
+            P.bd.shape[d] = (uint32_t)rc.shape[ax];
+            P.bd.fd[d] = FastDiv((uint32_t)rc.shape[ax]);
+            P.bd.sa[d] = ra.stride[ax];
+            P.bd.sb[d] = rb.stride[ax];
+            P.bd.sc[d] = rc.stride[ax];
+            nbat *= rc.shape[ax];
+        }
+        if (nbat >= (1ull << 32)) RSTSR_RAISE(UnImplemented, "too many batches");
+        P.nbatch = nbat;
+        if (lc.size() == 0) return;
+        // extents (relative to the operand base pointers, offsets folded in)
+        {
+            size_t es = dtype_size(t);
+            check_extent(a, la, es, "a");
+            check_extent(b, lb, es, "b");
+            check_extent(c, lc, es, "c");
+        }
+        std::lock_guard<std::mutex> g(ctx->mu);
+        run_gemm(ctx, t, mode, P, alpha, beta);
+    });
+}
+
+}  // extern "C"

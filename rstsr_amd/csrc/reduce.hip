@@ -1,0 +1,746 @@
+// Axis and all-element reductions for gfx950: sum, max, min, argmax, argmin.
+//
+// Replaces rstsr-native-impl/src/cpu_rayon/reduction.rs:20-631 (reduce_all,
+// reduce_axes, the arg variants) behind the traits of
+// rstsr-core/src/operators/reduction.rs:3-95.
+//
+// Semantics kept from the reference:
+//  - sum: T::zero + x ... (auto_impl/reduction.rs:14-40). Floating sums are
+//    accumulated here in FP64 with TwoSum error compensation (f32 inputs too),
+//    so the result is the exact sum rounded once, up to O(n u^2) terms;
+//    integers wrap like Rust release builds.
+//  - max/min: start from T::MIN / T::MAX and use IEEE maxNum/minNum, so NaN is
+//    ignored (rstsr-dtype-traits/src/ext_real.rs:70-87, auto_impl/reduction.rs:94-128).
+//  - argmax/argmin: the first extreme in row-major order over the reduced axes
+//    in the order given; a NaN at logical position 0 wins, later NaNs are
+//    ignored (cpu_serial/reduction.rs:425-580). Indices are u64 raveled over the
+//    reduced shape. Every combine below uses (value, smaller logical index), a
+//    total order, so the result does not depend on how work is split.
+//
+// Kernels (host planning in reduce_plan):
+//  - row: one 64-lane wave per (output, split); lanes stride a unit-stride
+//    reduced axis (coalesced 512 B per load wave), wave shuffles combine;
+//  - col: one thread per (output, split) with outputs along the input's
+//    unit-stride kept axis (coalesced across lanes), sequential over the
+//    reduced range;
+//  - finalize: combine the split partials of each output in fixed order.
+// Built with -ffp-contract=off so that TwoSum is not fused or reassociated.
+#include <array>
+#include <cfloat>
+#include <climits>
+#include <type_traits>
+
+#include "common.hpp"
+#include "registry.hpp"
+
+namespace rstsr {
+namespace hip {
+
+/* =============================== accumulators =============================== */
+
+struct DD {
+    double s, c;
+};
+__device__ __forceinline__ void dd_add(DD& a, double x) {
+    double t = a.s + x;
+    double bp = t - a.s;
+    double e = (a.s - (t - bp)) + (x - bp);
+    a.s = t;
+    a.c += e;
+}
+__device__ __forceinline__ DD dd_combine(DD a, const DD& b) {
+    dd_add(a, b.s);
+    a.c += b.c;
+    return a;
+}
+__device__ __forceinline__ double shfl_d(double v, int m) { return __shfl_xor(v, m, 64); }
+__device__ __forceinline__ uint64_t shfl_u64(uint64_t v, int m) {
+    return (uint64_t)__shfl_xor((long long)v, m, 64);
+}
+
+template <class T>
+struct SumF {  // f32, f64
+    using Val = T;
+    using Out = T;
+    using S = DD;
+    __device__ static S init() { return {0.0, 0.0}; }
+    __device__ static void add(S& s, T x, uint64_t) { dd_add(s, (double)x); }
+    __device__ static S combine(S a, S b) { return dd_combine(a, b); }
+    __device__ static S shfl(S s, int m) { return {shfl_d(s.s, m), shfl_d(s.c, m)}; }
+    __device__ static Out fin(S s, T) { return (T)(s.s + s.c); }
+};
+
+template <class T, class R>  // complex: T = c64/c128, R = float/double
+struct SumC {
+    using Val = T;
+    using Out = T;
+    struct S {
+        DD re, im;
+    };
+    __device__ static S init() { return {{0.0, 0.0}, {0.0, 0.0}}; }
+    __device__ static void add(S& s, T x, uint64_t) {
+        dd_add(s.re, (double)x.re);
+        dd_add(s.im, (double)x.im);
+    }
+    __device__ static S combine(S a, S b) { return {dd_combine(a.re, b.re), dd_combine(a.im, b.im)}; }
+    __device__ static S shfl(S s, int m) {
+        return {{shfl_d(s.re.s, m), shfl_d(s.re.c, m)}, {shfl_d(s.im.s, m), shfl_d(s.im.c, m)}};
+    }
+    __device__ static Out fin(S s, T) { return T{(R)(s.re.s + s.re.c), (R)(s.im.s + s.im.c)}; }
+};
+
+template <class T>
+struct SumI {  // i32, i64: wrapping
+    using Val = T;
+    using Out = T;
+    using U = typename std::make_unsigned<T>::type;
+    using S = U;
+    __device__ static S init() { return 0; }
+    __device__ static void add(S& s, T x, uint64_t) { s = (U)(s + (U)x); }
+    __device__ static S combine(S a, S b) { return (U)(a + b); }
+    __device__ static S shfl(S s, int m) {
+        if constexpr (sizeof(T) == 8) return (U)shfl_u64((uint64_t)s, m);
+        else return (U)__shfl_xor((int)s, m, 64);
+    }
+    __device__ static Out fin(S s, T) { return (T)s; }
+};
+
+template <class T>
+__device__ __forceinline__ T shfl_val(T v, int m) {
+    if constexpr (sizeof(T) == 8) {
+        if constexpr (std::is_floating_point<T>::value) return __shfl_xor(v, m, 64);
+        else return (T)__shfl_xor((long long)v, m, 64);
+    } else {
+        if constexpr (std::is_floating_point<T>::value) return __shfl_xor(v, m, 64);
+        else return (T)__shfl_xor((int)v, m, 64);
+    }
+}
+
+template <class T>
+__device__ __forceinline__ T lowest_v() {
+    if constexpr (std::is_same<T, float>::value) return -FLT_MAX;
+    if constexpr (std::is_same<T, double>::value) return -DBL_MAX;
+    if constexpr (std::is_same<T, int32_t>::value) return INT_MIN;
+    if constexpr (std::is_same<T, int64_t>::value) return LLONG_MIN;
+}
+template <class T>
+__device__ __forceinline__ T highest_v() {
+    if constexpr (std::is_same<T, float>::value) return FLT_MAX;
+    if constexpr (std::is_same<T, double>::value) return DBL_MAX;
+    if constexpr (std::is_same<T, int32_t>::value) return INT_MAX;
+    if constexpr (std::is_same<T, int64_t>::value) return LLONG_MAX;
+}
+
+template <class T, bool MAX>
+struct Ext {  // max / min, NaN ignored
+    using Val = T;
+    using Out = T;
+    using S = T;
+    __device__ static S init() { return MAX ? lowest_v<T>() : highest_v<T>(); }
+    __device__ static void add(S& s, T x, uint64_t) {
+        if (MAX ? (x > s) : (x < s)) s = x;
+    }
+    __device__ static S combine(S a, S b) { return (MAX ? (b > a) : (b < a)) ? b : a; }
+    __device__ static S shfl(S s, int m) { return shfl_val<T>(s, m); }
+    __device__ static Out fin(S s, T) { return s; }
+};
+
+template <class T, bool MAX>
+struct Arg {  // argmax / argmin on (value, logical index)
+    using Val = T;
+    using Out = uint64_t;
+    struct S {
+        T v;
+        uint64_t i;  // UINT64_MAX: no candidate yet
+    };
+    __device__ static S init() { return {T{}, ~0ull}; }
+    __device__ static bool better(T x, uint64_t li, const S& s) {
+        if (s.i == ~0ull) return true;
+        if (MAX ? (x > s.v) : (x < s.v)) return true;
+        return x == s.v && li < s.i;
+    }
+    __device__ static void add(S& s, T x, uint64_t li) {
+        if constexpr (std::is_floating_point<T>::value)
+            if (x != x) return;
+        if (better(x, li, s)) {
+            s.v = x;
+            s.i = li;
+        }
+    }
+    __device__ static S combine(S a, S b) {
+        if (b.i == ~0ull) return a;
+        return better(b.v, b.i, a) ? b : a;
+    }
+    __device__ static S shfl(S s, int m) { return {shfl_val<T>(s.v, m), shfl_u64(s.i, m)}; }
+    // x0: the element at logical index 0 of this output's reduced space
+    __device__ static Out fin(S s, T x0) {
+        if constexpr (std::is_floating_point<T>::value)
+            if (x0 != x0) return 0;
+        return s.i == ~0ull ? 0 : s.i;
+    }
+};
+
+/* =============================== plan =============================== */
+
+struct RedPlan {
+    int nk = 0, nr = 0;
+    uint32_t kshape[MAXD];
+    FastDiv kfd[MAXD];
+    int64_t kin[MAXD], kout[MAXD];
+    uint64_t rshape[MAXD];
+    int64_t rin[MAXD], rlog[MAXD];
+    int64_t in_off = 0, out_off = 0;
+    uint32_t M = 1;   // outputs
+    uint64_t R = 1;   // reduced elements per output
+    uint32_t nsplit = 1;
+    FastDiv fsplit;
+    // row kernel segmentation of the reduced space: units = rows(rest) x nseg0
+    uint32_t seg0 = 1, nseg0 = 1;
+    uint64_t units = 1, upw = 1;  // units per split
+};
+
+__device__ __forceinline__ void kept_offsets(const RedPlan& p, uint32_t o, int64_t& oin, int64_t& oout) {
+    oin = p.in_off;
+    oout = p.out_off;
+#pragma unroll
+    for (int d = 0; d < MAXD; ++d) {
+        if (d >= p.nk) break;
+        uint32_t q = fdiv(o, p.kfd[d]);
+        uint32_t r = o - q * p.kshape[d];
+        oin += (int64_t)r * p.kin[d];
+        oout += (int64_t)r * p.kout[d];
+        o = q;
+    }
+}
+
+// offsets of reduced "row" u (dims 1..nr-1) : input offset and logical index
+__device__ __forceinline__ void row_offsets(const RedPlan& p, uint64_t u, int64_t& oin, uint64_t& li) {
+    oin = 0;
+    li = 0;
+#pragma unroll
+    for (int d = 1; d < MAXD; ++d) {
+        if (d >= p.nr) break;
+        uint64_t q = u / p.rshape[d];
+        uint64_t r = u - q * p.rshape[d];
+        oin += (int64_t)r * p.rin[d];
+        li += r * (uint64_t)p.rlog[d];
+        u = q;
+    }
+}
+
+constexpr int RB = 256;  // threads per block (4 waves)
+
+template <class A, class T>
+__global__ void __launch_bounds__(RB) k_red_row(const T* __restrict__ a, typename A::Out* out,
+                                                typename A::S* part, RedPlan p) {
+    using S = typename A::S;
+    const uint32_t wave = blockIdx.x * (RB / 64) + (threadIdx.x >> 6);
+    const uint32_t lane = threadIdx.x & 63;
+    const uint64_t nwork = (uint64_t)p.M * p.nsplit;
+    if (wave >= nwork) return;  // whole wave exits together; no block barrier below
+    const uint32_t o = fdiv(wave, p.fsplit);
+    const uint32_t sp = wave - o * p.nsplit;
+    int64_t oin, oout;
+    kept_offsets(p, o, oin, oout);
+    S acc = A::init();
+    const uint64_t u0 = (uint64_t)sp * p.upw;
+    const uint64_t u1 = min(p.units, u0 + p.upw);
+    const int64_t s0 = p.rin[0];
+    const uint64_t l0 = (uint64_t)p.rlog[0];
+    const uint32_t L0 = (uint32_t)p.rshape[0];
+    for (uint64_t u = u0; u < u1; ++u) {
+        uint64_t row = u / p.nseg0;
+        uint32_t seg = (uint32_t)(u - row * p.nseg0);
+        int64_t roff;
+        uint64_t rli;
+        row_offsets(p, row, roff, rli);
+        const T* base = a + oin + roff;
+        uint32_t e = seg * p.seg0 + lane;
+        uint32_t eend = min(L0, (seg + 1) * p.seg0);
+        for (; e + 192 < eend; e += 256) {
+            T x0 = base[(int64_t)e * s0];
+            T x1 = base[(int64_t)(e + 64) * s0];
+            T x2 = base[(int64_t)(e + 128) * s0];
+            T x3 = base[(int64_t)(e + 192) * s0];
+            A::add(acc, x0, rli + (uint64_t)e * l0);
+            A::add(acc, x1, rli + (uint64_t)(e + 64) * l0);
+            A::add(acc, x2, rli + (uint64_t)(e + 128) * l0);
+            A::add(acc, x3, rli + (uint64_t)(e + 192) * l0);
+        }
+        for (; e < eend; e += 64) A::add(acc, base[(int64_t)e * s0], rli + (uint64_t)e * l0);
+    }
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) {
+        S other = A::shfl(acc, m);
+        // lower lane's partial precedes the higher lane's
+        acc = (lane & m) ? A::combine(other, acc) : A::combine(acc, other);
+    }
+    if (lane == 0) {
+        if (p.nsplit == 1) out[oout] = A::fin(acc, a[oin]);
+        else part[(uint64_t)o * p.nsplit + sp] = acc;
+    }
+}
+
+template <class A, class T>
+__global__ void __launch_bounds__(RB) k_red_col(const T* __restrict__ a, typename A::Out* out,
+                                                typename A::S* part, RedPlan p) {
+    using S = typename A::S;
+    const uint64_t t = (uint64_t)blockIdx.x * RB + threadIdx.x;
+    if (t >= (uint64_t)p.M * p.nsplit) return;
+    const uint32_t sp = (uint32_t)(t / p.M);
+    const uint32_t o = (uint32_t)(t - (uint64_t)sp * p.M);
+    int64_t oin, oout;
+    kept_offsets(p, o, oin, oout);
+    S acc = A::init();
+    const uint64_t chunk = p.upw;  // reduced elements per split
+    uint64_t r0 = (uint64_t)sp * chunk;
+    uint64_t r1 = min(p.R, r0 + chunk);
+    if (r0 < r1) {
+        // unravel the start position over the reduced dims
+        uint64_t coord[MAXD];
+        int64_t off = 0;
+        uint64_t li = 0;
+        {
+            uint64_t u = r0;
+#pragma unroll
+            for (int d = 0; d < MAXD; ++d) {
+                coord[d] = 0;
+                if (d >= p.nr) continue;
+                uint64_t q = u / p.rshape[d];
+                coord[d] = u - q * p.rshape[d];
+                off += (int64_t)coord[d] * p.rin[d];
+                li += coord[d] * (uint64_t)p.rlog[d];
+                u = q;
+            }
+        }
+        const T* base = a + oin;
+        uint64_t remaining = r1 - r0;
+        const int64_t s0 = p.rin[0];
+        const uint64_t l0 = (uint64_t)p.rlog[0];
+        while (remaining > 0) {
+            uint64_t n = min(remaining, p.rshape[0] - coord[0]);
+            const T* rowp = base + off;
+            uint64_t j = 0;
+            for (; j + 4 <= n; j += 4) {
+                T x0 = rowp[(int64_t)j * s0];
+                T x1 = rowp[(int64_t)(j + 1) * s0];
+                T x2 = rowp[(int64_t)(j + 2) * s0];
+                T x3 = rowp[(int64_t)(j + 3) * s0];
+                A::add(acc, x0, li + j * l0);
+                A::add(acc, x1, li + (j + 1) * l0);
+                A::add(acc, x2, li + (j + 2) * l0);
+                A::add(acc, x3, li + (j + 3) * l0);
+            }
+            for (; j < n; ++j) A::add(acc, rowp[(int64_t)j * s0], li + j * l0);
+            remaining -= n;
+            if (remaining == 0) break;
+            // carry into dims >= 1
+            off -= (int64_t)coord[0] * s0;
+            li -= coord[0] * l0;
+            coord[0] = 0;
+#pragma unroll
+            for (int d = 1; d < MAXD; ++d) {
+                if (d >= p.nr) break;
+                coord[d] += 1;
+                off += p.rin[d];
+                li += (uint64_t)p.rlog[d];
+                if (coord[d] < p.rshape[d]) break;
+                off -= (int64_t)coord[d] * p.rin[d];
+                li -= coord[d] * (uint64_t)p.rlog[d];
+                coord[d] = 0;
+            }
+        }
+    }
+    if (p.nsplit == 1) out[oout] = A::fin(acc, a[oin]);
+    else part[(uint64_t)o * p.nsplit + sp] = acc;
+}
+
+template <class A, class T>
+__global__ void __launch_bounds__(RB) k_red_fin(const T* __restrict__ a, typename A::Out* out,
+                                                const typename A::S* part, RedPlan p) {
+    const uint32_t o = blockIdx.x * RB + threadIdx.x;
+    if (o >= p.M) return;
+    int64_t oin, oout;
+    kept_offsets(p, o, oin, oout);
+    auto acc = part[(uint64_t)o * p.nsplit];
+    for (uint32_t s = 1; s < p.nsplit; ++s) acc = A::combine(acc, part[(uint64_t)o * p.nsplit + s]);
+    out[oout] = A::fin(acc, a[oin]);
+}
+
+/* =============================== host planning =============================== */
+
+enum class RedKind { Row, Col };
+
+struct RedSetup {
+    RedPlan p;
+    RedKind kind = RedKind::Row;
+    bool empty_out = false;     // no outputs
+    bool empty_reduce = false;  // outputs exist, reduced extent 0
+};
+
+// la: input layout; axes: reduced axes (given order); lo: output layout (kept shape)
+static RedSetup reduce_plan(rstsr_hip_ctx* ctx, const Layout& la, const std::vector<ptrdiff_t>& axes, const Layout& lo) {
+    RedSetup s;
+    auto sp = la.dim_split_axes(axes);
+    const Layout& ls = sp.first;
+    const Layout& lm = sp.second;
+    if (lo.shape != lm.shape) RSTSR_RAISE(InvalidLayout, "output layout shape does not match the kept axes");
+    if (lm.size() == 0) {
+        s.empty_out = true;
+        return s;
+    }
+    if (ls.size() == 0) {
+        s.empty_reduce = true;
+        return s;
+    }
+    // logical (row-major over the reduced axes in the given order) strides
+    Layout lsc = c_layout(ls.shape);
+    // reduced dims: drop size 1, sort by |in stride|, merge when both strides chain
+    struct RD {
+        uint64_t n;
+        int64_t in, lg;
+    };
+    std::vector<RD> rd;
+    for (size_t i = 0; i < ls.ndim(); ++i)
+        if (ls.shape[i] != 1) rd.push_back({ls.shape[i], ls.stride[i], lsc.stride[i]});
+    std::stable_sort(rd.begin(), rd.end(), [](const RD& x, const RD& y) {
+        if (std::llabs(x.in) != std::llabs(y.in)) return std::llabs(x.in) < std::llabs(y.in);
+        return std::llabs(x.lg) < std::llabs(y.lg);
+    });
+    std::vector<RD> rm;
+    for (auto& d : rd) {
+        if (!rm.empty() && d.in == rm.back().in * (int64_t)rm.back().n && d.lg == rm.back().lg * (int64_t)rm.back().n) {
+            rm.back().n *= d.n;
+            continue;
+        }
+        rm.push_back(d);
+    }
+    if (rm.empty()) rm.push_back({1, 0, 0});
+    // kept dims: sort by |in stride| then |out stride|; merge when both chain
+    struct KD {
+        uint64_t n;
+        int64_t in, out;
+    };
+    std::vector<KD> kd;
+    for (size_t i = 0; i < lm.ndim(); ++i)
+        if (lm.shape[i] != 1) kd.push_back({lm.shape[i], lm.stride[i], lo.stride[i]});
+    std::stable_sort(kd.begin(), kd.end(), [](const KD& x, const KD& y) {
+        if (std::llabs(x.in) != std::llabs(y.in)) return std::llabs(x.in) < std::llabs(y.in);
+        return std::llabs(x.out) < std::llabs(y.out);
+    });
+    std::vector<KD> km;
+    for (auto& d : kd) {
+        if (!km.empty() && d.in == km.back().in * (int64_t)km.back().n && d.out == km.back().out * (int64_t)km.back().n) {
+            km.back().n *= d.n;
+            continue;
+        }
+        km.push_back(d);
+    }
+    if ((int)rm.size() > MAXD || (int)km.size() > MAXD)
+        RSTSR_RAISE(UnImplemented, "reductions over more than 8 non-mergeable axes are not supported by the HIP device");
+    RedPlan& p = s.p;
+    p.nr = (int)rm.size();
+    p.nk = (int)km.size();
+    for (int i = 0; i < MAXD; ++i) {
+        p.kshape[i] = 1;
+        p.kfd[i] = FastDiv(1);
+        p.kin[i] = p.kout[i] = 0;
+        p.rshape[i] = 1;
+        p.rin[i] = p.rlog[i] = 0;
+    }
+    uint64_t M = 1, R = 1;
+    for (int i = 0; i < p.nk; ++i) {
+        if (km[i].n >= (1ull << 31)) RSTSR_RAISE(UnImplemented, "kept extent too large");
+        p.kshape[i] = (uint32_t)km[i].n;
+        p.kfd[i] = FastDiv((uint32_t)km[i].n);
+        p.kin[i] = km[i].in;
+        p.kout[i] = km[i].out;
+        M *= km[i].n;
+    }
+    for (int i = 0; i < p.nr; ++i) {
+        p.rshape[i] = rm[i].n;
+        p.rin[i] = rm[i].in;
+        p.rlog[i] = rm[i].lg;
+        R *= rm[i].n;
+    }
+    if (M >= (1ull << 31)) RSTSR_RAISE(UnImplemented, "too many outputs for one reduction launch");
+    p.M = (uint32_t)M;
+    p.R = R;
+    p.in_off = (int64_t)la.offset;
+    p.out_off = (int64_t)lo.offset;
+
+    const uint64_t target_waves = (uint64_t)std::max(ctx->num_cu, 1) * 32;
+    bool row_unit = std::llabs(p.rin[0]) == 1 && p.rshape[0] >= 16;
+    bool col_unit = p.nk >= 1 && std::llabs(p.kin[0]) <= 1 && M >= 32;
+    if (row_unit || !(col_unit || M >= 4096)) {
+        s.kind = RedKind::Row;
+        if (p.rshape[0] >= (1ull << 31)) RSTSR_RAISE(UnImplemented, "reduced extent too large");
+        uint32_t L0 = (uint32_t)p.rshape[0];
+        p.seg0 = std::min<uint32_t>(L0, 8192);
+        p.nseg0 = (L0 + p.seg0 - 1) / p.seg0;
+        p.units = (R / L0) * p.nseg0;
+        uint64_t want = std::max<uint64_t>(1, target_waves / M);
+        uint64_t nsplit = std::min<uint64_t>(want, p.units);
+        nsplit = std::min<uint64_t>(nsplit, 65536);
+        p.upw = (p.units + nsplit - 1) / nsplit;
+        nsplit = (p.units + p.upw - 1) / p.upw;
+        p.nsplit = (uint32_t)nsplit;
+    } else {
+        s.kind = RedKind::Col;
+        uint64_t want = std::max<uint64_t>(1, (target_waves * 64) / M);
+        uint64_t nsplit = std::min<uint64_t>(want, std::max<uint64_t>(1, R / 64));
+        nsplit = std::min<uint64_t>(nsplit, 65536);
+        p.upw = (R + nsplit - 1) / nsplit;
+        nsplit = (R + p.upw - 1) / p.upw;
+        p.nsplit = (uint32_t)nsplit;
+    }
+    if ((uint64_t)p.M * p.nsplit >= (1ull << 32)) RSTSR_RAISE(UnImplemented, "reduction grid too large");
+    p.fsplit = FastDiv(p.nsplit);
+    return s;
+}
+
+template <class A, class T>
+static void run_reduce(rstsr_hip_ctx* ctx, const RedSetup& s, const T* a, typename A::Out* out) {
+    const RedPlan& p = s.p;
+    using S = typename A::S;
+    S* part = nullptr;
+    if (p.nsplit > 1) part = (S*)workspace(ctx, (size_t)p.M * p.nsplit * sizeof(S));
+    if (s.kind == RedKind::Row) {
+        uint64_t nwaves = (uint64_t)p.M * p.nsplit;
+        uint32_t nb = ceil_div_u32(nwaves, RB / 64);
+        hipLaunchKernelGGL((k_red_row<A, T>), dim3(nb), dim3(RB), 0, ctx->stream, a, out, part, p);
+    } else {
+        uint64_t nthr = (uint64_t)p.M * p.nsplit;
+        uint32_t nb = ceil_div_u32(nthr, RB);
+        hipLaunchKernelGGL((k_red_col<A, T>), dim3(nb), dim3(RB), 0, ctx->stream, a, out, part, p);
+    }
+    check_launch("reduce");
+    if (p.nsplit > 1) {
+        hipLaunchKernelGGL((k_red_fin<A, T>), dim3(ceil_div_u32(p.M, RB)), dim3(RB), 0, ctx->stream, a, out, part, p);
+        check_launch("reduce finalize");
+    }
+}
+
+template <class T>
+__global__ void k_fill_value(T* out, RedPlan p, T v) {
+    uint32_t o = blockIdx.x * RB + threadIdx.x;
+    if (o >= p.M) return;
+    int64_t oin, oout;
+    kept_offsets(p, o, oin, oout);
+    out[oout] = v;
+}
+
+// Dispatch (op, dtype) to an accumulator and run it.
+static void reduce_dispatch(rstsr_hip_ctx* ctx, rstsr_redop_t op, rstsr_dtype_t t, const void* a, const RedSetup& s,
+                            void* out) {
+    auto go = [&](auto acc_tag, auto val_tag) {
+        using A = decltype(acc_tag);
+        using T = decltype(val_tag);
+        run_reduce<A, T>(ctx, s, (const T*)a, (typename A::Out*)out);
+    };
+    switch (op) {
+        case RSTSR_RED_SUM:
+            switch (t) {
+                case RSTSR_F32: go(SumF<float>{}, float{}); return;
+                case RSTSR_F64: go(SumF<double>{}, double{}); return;
+                case RSTSR_I32: go(SumI<int32_t>{}, int32_t{}); return;
+                case RSTSR_I64: go(SumI<int64_t>{}, int64_t{}); return;
+                case RSTSR_C64: go(SumC<c64, float>{}, c64{}); return;
+                case RSTSR_C128: go(SumC<c128, double>{}, c128{}); return;
+                default: break;
+            }
+            break;
+        case RSTSR_RED_MAX:
+        case RSTSR_RED_MIN: {
+            bool mx = op == RSTSR_RED_MAX;
+            switch (t) {
+                case RSTSR_F32: mx ? go(Ext<float, true>{}, float{}) : go(Ext<float, false>{}, float{}); return;
+                case RSTSR_F64: mx ? go(Ext<double, true>{}, double{}) : go(Ext<double, false>{}, double{}); return;
+                case RSTSR_I32: mx ? go(Ext<int32_t, true>{}, int32_t{}) : go(Ext<int32_t, false>{}, int32_t{}); return;
+                case RSTSR_I64: mx ? go(Ext<int64_t, true>{}, int64_t{}) : go(Ext<int64_t, false>{}, int64_t{}); return;
+                default: break;
+            }
+            break;
+        }
+        case RSTSR_RED_ARGMAX:
+        case RSTSR_RED_ARGMIN: {
+            bool mx = op == RSTSR_RED_ARGMAX;
+            switch (t) {
+                case RSTSR_F32: mx ? go(Arg<float, true>{}, float{}) : go(Arg<float, false>{}, float{}); return;
+                case RSTSR_F64: mx ? go(Arg<double, true>{}, double{}) : go(Arg<double, false>{}, double{}); return;
+                case RSTSR_I32: mx ? go(Arg<int32_t, true>{}, int32_t{}) : go(Arg<int32_t, false>{}, int32_t{}); return;
+                case RSTSR_I64: mx ? go(Arg<int64_t, true>{}, int64_t{}) : go(Arg<int64_t, false>{}, int64_t{}); return;
+                default: break;
+            }
+            break;
+        }
+    }
+    RSTSR_RAISE(UnImplemented, "reduction not provided for this dtype");
+}
+
+static size_t out_size_of(rstsr_redop_t op, rstsr_dtype_t t) {
+    if (op == RSTSR_RED_ARGMAX || op == RSTSR_RED_ARGMIN) return 8;
+    return dtype_size(t);
+}
+
+static void check_op(rstsr_redop_t op, bool arg) {
+    if ((int)op < 0 || (int)op > 4) RSTSR_RAISE(InvalidValue, "unknown reduction op");
+    bool is_arg = op == RSTSR_RED_ARGMAX || op == RSTSR_RED_ARGMIN;
+    if (is_arg != arg) RSTSR_RAISE(InvalidValue, arg ? "expected argmax/argmin" : "argmax/argmin need the arg entry point");
+}
+
+// Common path of *_axes entry points; la, lo already validated.
+static void reduce_axes_impl(rstsr_hip_ctx* ctx, rstsr_redop_t op, rstsr_dtype_t t, const void* a, const Layout& la,
+                             const std::vector<ptrdiff_t>& axes, void* out, const Layout& lo) {
+    if (op != RSTSR_RED_SUM && la.size() == 0) {
+        if (op == RSTSR_RED_MAX || op == RSTSR_RED_MIN)
+            RSTSR_RAISE(InvalidValue, "zero-size array is not supported for max/min");
+        RSTSR_RAISE(InvalidLayout, "empty sequence is not allowed for reduce_arg.");
+    }
+    RedSetup s = reduce_plan(ctx, la, axes, lo);
+    if (s.empty_out) return;
+    if (s.empty_reduce) {
+        // only sums reach here (la.size() == 0 but outputs exist): T::zero
+        if (op != RSTSR_RED_SUM) RSTSR_RAISE(InvalidValue, "zero-size reduction");
+        // build a plan over the kept axes only to address outputs
+        Layout lm = la.dim_split_axes(axes).second;
+        std::vector<ptrdiff_t> none;
+        Layout la_k(lm.shape, lm.stride, la.offset);
+        // outputs via a trivial reduced space of extent 1 at a harmless address
+        RedPlan p;
+        p.nk = 0;
+        std::vector<std::pair<uint64_t, int64_t>> kd;
+        for (size_t i = 0; i < lo.ndim(); ++i)
+            if (lo.shape[i] != 1) kd.push_back({lo.shape[i], lo.stride[i]});
+        if ((int)kd.size() > MAXD) RSTSR_RAISE(UnImplemented, "too many output axes");
+        p.nk = (int)kd.size();
+        uint64_t M = 1;
+        for (int i = 0; i < MAXD; ++i) {
+            p.kshape[i] = 1;
+            p.kfd[i] = FastDiv(1);
+            p.kin[i] = p.kout[i] = 0;
+        }
+        for (int i = 0; i < p.nk; ++i) {
+            p.kshape[i] = (uint32_t)kd[i].first;
+            p.kfd[i] = FastDiv((uint32_t)kd[i].first);
+            p.kout[i] = kd[i].second;
+            M *= kd[i].first;
+        }
+        p.M = (uint32_t)M;
+        p.out_off = (int64_t)lo.offset;
+        auto z = [&](auto tag) {
+            using T = decltype(tag);
+            hipLaunchKernelGGL((k_fill_value<T>), dim3(ceil_div_u32(M, RB)), dim3(RB), 0, ctx->stream, (T*)out, p, T{});
+            check_launch("fill");
+        };
+        switch (t) {
+            case RSTSR_F32: z(float{}); break;
+            case RSTSR_F64: z(double{}); break;
+            case RSTSR_I32: z(int32_t{}); break;
+            case RSTSR_I64: z(int64_t{}); break;
+            case RSTSR_C64: z(c64{}); break;
+            case RSTSR_C128: z(c128{}); break;
+            default: RSTSR_RAISE(UnImplemented, "dtype");
+        }
+        return;
+    }
+    reduce_dispatch(ctx, op, t, a, s, out);
+}
+
+}  // namespace hip
+}  // namespace rstsr
+
+using namespace rstsr;
+using namespace rstsr::hip;
+
+static std::vector<ptrdiff_t> all_axes(size_t nd) {
+    std::vector<ptrdiff_t> v(nd);
+    for (size_t i = 0; i < nd; ++i) v[i] = (ptrdiff_t)i;
+    return v;
+}
+
+extern "C" {
+
+int rstsr_hip_reduce_axes(rstsr_hip_ctx_t* ctx, rstsr_redop_t op, rstsr_dtype_t t, const void* a,
+                          const rstsr_layout_t* la_, size_t naxes, const ptrdiff_t* axes, void* out,
+                          const rstsr_layout_t* lo_) {
+    return guarded([&] {
+        bind(ctx);
+        check_op(op, false);
+        Layout la = from_abi(la_), lo = from_abi(lo_);
+        if (naxes && !axes) RSTSR_RAISE(InvalidValue, "null axes");
+        std::vector<ptrdiff_t> ax(axes, axes + naxes);
+        check_extent(a, la, dtype_size(t), "a");
+        check_extent(out, lo, out_size_of(op, t), "out");
+        std::lock_guard<std::mutex> g(ctx->mu);
+        reduce_axes_impl(ctx, op, t, a, la, ax, out, lo);
+    });
+}
+
+int rstsr_hip_argreduce_axes(rstsr_hip_ctx_t* ctx, rstsr_redop_t op, rstsr_dtype_t t, const void* a,
+                             const rstsr_layout_t* la_, size_t naxes, const ptrdiff_t* axes, uint64_t* out,
+                             const rstsr_layout_t* lo_) {
+    return guarded([&] {
+        bind(ctx);
+        check_op(op, true);
+        Layout la = from_abi(la_), lo = from_abi(lo_);
+        if (naxes && !axes) RSTSR_RAISE(InvalidValue, "null axes");
+        std::vector<ptrdiff_t> ax(axes, axes + naxes);
+        check_extent(a, la, dtype_size(t), "a");
+        check_extent(out, lo, 8, "out");
+        std::lock_guard<std::mutex> g(ctx->mu);
+        reduce_axes_impl(ctx, op, t, a, la, ax, out, lo);
+    });
+}
+
+static void reduce_all_common(rstsr_hip_ctx_t* ctx, rstsr_redop_t op, rstsr_dtype_t t, const void* a,
+                              const rstsr_layout_t* la_, void* host_out, size_t out_bytes) {
+    bind(ctx);
+    if (!host_out) RSTSR_RAISE(InvalidValue, "null output");
+    Layout la = from_abi(la_);
+    check_extent(a, la, dtype_size(t), "a");
+    std::lock_guard<std::mutex> g(ctx->mu);
+    // the scalar result lives in the last 256 bytes of the workspace; partials
+    // come first, so reserve room for both in one allocation.
+    RedSetup s;
+    bool trivial_sum = false;
+    if (la.size() == 0) {
+        if (op == RSTSR_RED_MAX || op == RSTSR_RED_MIN)
+            RSTSR_RAISE(InvalidValue, "zero-size array is not supported for max/min");
+        if (op != RSTSR_RED_SUM) RSTSR_RAISE(InvalidLayout, "empty sequence is not allowed for reduce_arg.");
+        trivial_sum = true;
+    }
+    if (trivial_sum) {
+        std::memset(host_out, 0, out_bytes);
+        return;
+    }
+    Layout lo({}, {}, 0);
+    s = reduce_plan(ctx, la, all_axes(la.ndim()), lo);
+    size_t part_bytes = 0;
+    if (s.p.nsplit > 1) part_bytes = (size_t)s.p.M * s.p.nsplit * 64;  // upper bound of sizeof(S)
+    char* ws = (char*)workspace(ctx, part_bytes + 256);
+    void* dout = ws + part_bytes;
+    // run_reduce asks for the same workspace size again, which is a no-op
+    reduce_dispatch(ctx, op, t, a, s, dout);
+    HIP_CHECK(hipMemcpyAsync(ctx->host_pinned, dout, out_bytes, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    std::memcpy(host_out, ctx->host_pinned, out_bytes);
+}
+
+int rstsr_hip_reduce_all(rstsr_hip_ctx_t* ctx, rstsr_redop_t op, rstsr_dtype_t t, const void* a,
+                         const rstsr_layout_t* la, void* host_out) {
+    return guarded([&] {
+        check_op(op, false);
+        reduce_all_common(ctx, op, t, a, la, host_out, dtype_size(t));
+    });
+}
+
+int rstsr_hip_argreduce_all(rstsr_hip_ctx_t* ctx, rstsr_redop_t op, rstsr_dtype_t t, const void* a,
+                            const rstsr_layout_t* la, uint64_t* host_out) {
+    return guarded([&] {
+        check_op(op, true);
+        reduce_all_common(ctx, op, t, a, la, host_out, 8);
+    });
+}
+
+}  // extern "C"

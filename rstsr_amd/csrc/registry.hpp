@@ -1,0 +1,22 @@
+// Extent checks of device operands against the allocations this library made.
+#pragma once
+
+#include <cstddef>
+#include <cstdint>
+#include <utility>
+
+#include "layout.hpp"
+
+namespace rstsr {
+namespace hip {
+
+void registry_add(void* p, size_t bytes);
+bool registry_remove(void* p);
+std::pair<uintptr_t, size_t> registry_find(const void* p);
+// Raise unless every element `l` addresses (elements of `elem` bytes from ptr)
+// lies inside one buffer returned by rstsr_hip_malloc.
+void check_extent(const void* ptr, const Layout& l, size_t elem, const char* name);
+void check_extent_bytes(const void* ptr, size_t bytes, const char* name);
+
+}  // namespace hip
+}  // namespace rstsr

@@ -1,0 +1,162 @@
+"""HIP GEMM / matmul vs the oracle.
+
+f64 / c128 (FP64 MFMA, fused multiply-add accumulation): componentwise bound
+|C - C*| <= K * u * (|A| |B|) with u = 2^-53 against the Dot2 / extended-precision oracle
+(the classical gamma_K bound of any summation order; stated tolerance).
+f32: the same bound with u = 2^-24. Integers: bit-exact (wrapping), vs the naive oracle.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+import pytest
+
+import rstsr_amd as rt
+from gpu_util import from_np
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def comp_bound_ok(got, A, B, ref, u):
+    K = A.shape[-1]
+    scale = np.abs(A) @ np.abs(B)
+    err = np.abs(got - ref)
+    return np.all(err <= K * u * scale + 1e-300), float(np.max(err / np.maximum(K * u * scale, 1e-300)))
+
+
+def wide_exponent(rng, shape):
+    return rng.normal(size=shape) * np.exp2(rng.integers(-8, 9, size=shape))
+
+
+@pytest.mark.parametrize("m,n,k", [(1, 1, 1), (17, 33, 65), (128, 128, 128), (300, 257, 129), (64, 512, 1000),
+                                   (512, 512, 512)])
+@pytest.mark.parametrize("trans", ["nn", "tn", "nt", "tt"])
+def test_dgemm_plain(dev, oracle, m, n, k, trans):
+    rng = np.random.default_rng(m * 7 + n * 3 + k)
+    A = wide_exponent(rng, (m, k) if trans[0] == "n" else (k, m))
+    B = wide_exponent(rng, (k, n) if trans[1] == "n" else (n, k))
+    A = A if trans[0] == "n" else A.T
+    B = B if trans[1] == "n" else B.T
+    ta, _ = from_np(dev, A)
+    tb, _ = from_np(dev, B)
+    got = (ta % tb).to_numpy()
+    ref = oracle.gemm_dot2(A, B)
+    ok, worst = comp_bound_ok(got, A, B, ref, 2.0 ** -53)
+    assert ok, worst
+
+
+def test_dgemm_strided_views(dev, oracle):
+    rng = np.random.default_rng(0)
+    big = rng.normal(size=(90, 140))
+    A = big[::3, 5:105:2]       # 30 x 50, both strides non-unit
+    B = rng.normal(size=(100, 70))[::2, ::-1]  # 50 x 70, negative column stride
+    ta, _ = from_np(dev, A)
+    tb, _ = from_np(dev, B)
+    got = (ta % tb).to_numpy()
+    ref = oracle.gemm_dot2(np.ascontiguousarray(A), np.ascontiguousarray(B))
+    ok, worst = comp_bound_ok(got, A, B, ref, 2.0 ** -53)
+    assert ok, worst
+
+
+def test_zgemm(dev):
+    rng = np.random.default_rng(1)
+    A = rng.normal(size=(70, 90)) + 1j * rng.normal(size=(70, 90))
+    B = rng.normal(size=(90, 130)) + 1j * rng.normal(size=(90, 130))
+    got = (rt.asarray(A, device=dev) % rt.asarray(B, device=dev)).to_numpy()
+    ref = A.astype(np.clongdouble) @ B.astype(np.clongdouble)
+    scale = np.abs(A) @ np.abs(B)
+    assert np.all(np.abs(got - ref) <= 2 * 90 * 2.0 ** -53 * scale)
+
+
+@pytest.mark.parametrize("dtype", [np.int32, np.int64])
+def test_integer_matmul_exact(dev, oracle, dtype):
+    rng = np.random.default_rng(2)
+    A = rng.integers(-(2 ** 20), 2 ** 20, size=(37, 53)).astype(dtype)
+    B = rng.integers(-(2 ** 20), 2 ** 20, size=(53, 29)).astype(dtype).T.copy().T
+    got = (rt.asarray(A, device=dev) % from_np(dev, B)[0]).to_numpy()
+    assert np.array_equal(got, oracle.gemm_naive(A, np.ascontiguousarray(B)))
+
+
+def test_f32_matmul(dev):
+    rng = np.random.default_rng(3)
+    A = rng.normal(size=(45, 77)).astype(np.float32)
+    B = rng.normal(size=(77, 33)).astype(np.float32)
+    got = (rt.asarray(A, device=dev) % rt.asarray(B, device=dev)).to_numpy().astype(np.float64)
+    ref = A.astype(np.float64) @ B.astype(np.float64)
+    scale = np.abs(A.astype(np.float64)) @ np.abs(B.astype(np.float64))
+    assert np.all(np.abs(got - ref) <= 77 * 2.0 ** -24 * scale)
+
+
+@pytest.mark.parametrize("sa,sb", [((5,), (5,)), ((4,), (3, 4, 6)), ((3, 5, 4), (4,)), ((6, 4), (3, 4, 5)),
+                                   ((3, 6, 4), (4, 5)), ((2, 1, 6, 4), (3, 4, 5)), ((64, 32, 48), (48, 40))])
+def test_matmul_rules(dev, sa, sb):
+    rng = np.random.default_rng(len(sa) * 10 + len(sb))
+    A = rng.integers(-50, 50, size=sa).astype(np.int64)
+    B = rng.integers(-50, 50, size=sb).astype(np.int64)
+    got = rt.asarray(A, device=dev) % rt.asarray(B, device=dev)
+    ref = np.matmul(A, B)
+    assert got.shape == ref.shape
+    assert np.array_equal(got.to_numpy(), ref)
+    # the same through the FP64 MFMA path (exact for these small integers)
+    gotf = rt.asarray(A.astype(np.float64), device=dev) % rt.asarray(B.astype(np.float64), device=dev)
+    assert np.array_equal(gotf.to_numpy(), ref.astype(np.float64))
+
+
+def test_reference_matmul_cases(dev):
+    m1 = rt.asarray(np.array([[1, 2], [3, 4]], dtype=np.int32), device=dev)
+    m2 = rt.asarray(np.array([[1, 0], [1, 1]], dtype=np.int32), device=dev)
+    assert (m1 % m2).to_numpy().tolist() == [[3, 2], [7, 4]]
+    assert (m2 % m1).to_numpy().tolist() == [[1, 2], [4, 6]]
+    v = rt.asarray(np.array([1, 2], dtype=np.int32), device=dev)
+    assert (v % m1).to_vec() == [7, 10]
+    assert (m1 % v).to_vec() == [5, 11]
+    m12 = rt.asarray(np.array([[[1, 2], [3, 4]], [[1, 0], [1, 1]]], dtype=np.int32), device=dev)
+    assert (m12 % m1).to_numpy().tolist() == [[[7, 10], [15, 22]], [[1, 2], [4, 6]]]
+    a = rt.asarray(np.array([0 + 1j, 1 + 2j, 2 + 3j, 3 + 4j, 4 + 5j, 5 + 6j], dtype=np.complex64).reshape(3, 2), device=dev)
+    b = rt.asarray(np.array([0 + 1j, 2 + 3j, 4 + 5j, 6 + 7j], dtype=np.complex64).reshape(2, 2), device=dev)
+    s = (a % b).to_numpy().sum()
+    assert abs(s.real + 78.0) < 1e-5 and abs(s.imag - 270.0) < 1e-5
+
+
+def test_reference_large_matmul_stats(dev):
+    a = rt.linspace(0.0, 1.0, 1048576, device=dev).reshape([16, 256, 256])
+    b = rt.linspace(1.0, 2.0, 1048576, device=dev).reshape([16, 256, 256])
+    c = (a % b).to_numpy()
+    assert abs(c.mean() - 213.2503660477036) < 1e-6
+    assert abs(c.std() - 148.88523481701804) < 1e-6
+    assert abs(c.std(axis=(0, 1))[0] - 148.8763226818815) < 1e-6
+    assert abs(c.std(axis=(0, 1))[255] - 148.8941462322758) < 1e-6
+    assert abs(c.std(axis=(1, 2))[0] - 4.763105902995575) < 1e-6
+    assert abs(c.std(axis=(1, 2))[15] - 9.093224903569157) < 1e-6
+
+
+def test_gemm_alpha_beta_c_abi(dev, oracle):
+    """rstsr_hip_gemm with alpha/beta on strided C; beta == 0 must not read C (NaN-filled)."""
+    lib = C.CDLL(os.path.join(ROOT, "rstsr_amd", "librstsr_hip.so"))
+    rng = np.random.default_rng(4)
+    m, n, k = 40, 50, 60
+    A = rng.normal(size=(m, k))
+    B = rng.normal(size=(k, n))
+    C0 = rng.normal(size=(n, m)).T  # column-major C
+    ta = rt.asarray(A, device=dev)
+    tb = rt.asarray(B, device=dev)
+    tc, _ = from_np(dev, C0)
+    ctx = C.c_void_p(dev._ctx_ptr())
+    al = C.c_double(0.5)
+    be = C.c_double(-2.0)
+    ptr = lambda t: C.c_void_p(t._t._data_ptr() + t.offset * 8)
+    lib.rstsr_hip_gemm.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_size_t, C.c_size_t, C.c_size_t, C.c_void_p,
+                                   C.c_void_p, C.c_ssize_t, C.c_ssize_t, C.c_void_p, C.c_ssize_t, C.c_ssize_t,
+                                   C.c_void_p, C.c_void_p, C.c_ssize_t, C.c_ssize_t]
+    r = lib.rstsr_hip_gemm(ctx, 1, 0, m, n, k, C.byref(al), ptr(ta), k, 1, ptr(tb), n, 1, C.byref(be), ptr(tc),
+                           tc.stride[0], tc.stride[1])
+    assert r == 0
+    ref = 0.5 * oracle.gemm_dot2(A, B) - 2.0 * C0
+    assert np.max(np.abs(tc.to_numpy() - ref)) < 1e-12 * np.max(np.abs(ref))
+    # beta = 0 over a NaN-filled C
+    tn = rt.full((m, n), np.nan, device=dev)
+    z = C.c_double(0.0)
+    r = lib.rstsr_hip_gemm(ctx, 1, 0, m, n, k, C.byref(al), ptr(ta), k, 1, ptr(tb), n, 1, C.byref(z), ptr(tn), n, 1)
+    assert r == 0
+    assert np.all(np.isfinite(tn.to_numpy()))

@@ -1,0 +1,141 @@
+"""HIP reductions vs the oracle.
+
+max/min/argmax/argmin and integer sums: bit-exact. Floating sums: the device accumulates
+in FP64 with TwoSum compensation, the oracle in double-double; both are the exact sum
+rounded once up to O(n u^2) terms, so the test bound is 2 ulp of the result plus
+1e-30 * sum|x| (stated tolerance; the reference's own sums are only ~n*u accurate).
+"""
+import zlib
+
+import numpy as np
+import pytest
+
+import rstsr_amd as rt
+from gpu_util import bits_equal, from_np, oracle_reduce, rand
+from oracle_ffi import View
+
+pytestmark = pytest.mark.gpu
+
+SUM, MAX, MIN, ARGMAX, ARGMIN = 0, 1, 2, 3, 4
+METH = {SUM: "sum_axes", MAX: "max_axes", MIN: "min_axes", ARGMAX: "argmax_axes", ARGMIN: "argmin_axes"}
+ALL = {SUM: "sum_all", MAX: "max_all", MIN: "min_all", ARGMAX: "argmax_all", ARGMIN: "argmin_all"}
+
+
+def check_sum(got, ref, absx_sum, dtype):
+    got = np.asarray(got, dtype=np.float64 if np.dtype(dtype).kind != "c" else np.complex128)
+    ref = np.asarray(ref, dtype=got.dtype)
+    eps = np.finfo(np.dtype(dtype).type(0).real.dtype if np.dtype(dtype).kind == "c" else dtype).eps
+    tol = 2 * eps * np.abs(ref) + 1e-30 * absx_sum
+    assert np.all(np.abs(got - ref) <= tol), float(np.max(np.abs(got - ref) / np.maximum(tol, 1e-300)))
+
+
+CASES = [
+    # (X, perm, step, axes)  view = base.reshape(X).transpose(perm)[..., ::step]
+    ((16, 32, 64), (2, 0, 1), 2, [0]),          # unit-stride reduced axis -> row kernel
+    ((16, 32, 64), (2, 0, 1), 2, [1, 2]),       # unit-stride kept axis -> column kernel + split
+    ((16, 32, 64), (2, 0, 1), 2, [2, 1]),
+    ((64, 128, 256), (0, 1, 2), 1, [1]),        # the small config's sum over axis 1
+    ((64, 128, 256), (0, 1, 2), 1, [2]),        # ... and argmax over axis 2
+    ((9, 300, 7), (1, 0, 2), 1, [0, 2]),
+    ((4, 6, 15, 9), (2, 0, 3, 1), 1, [0, -2]),
+    ((5, 6, 7), (0, 1, 2), 1, [0, 1, 2]),
+    ((2000, 3), (0, 1), 1, [0]),                # long strided reduced axis, few outputs
+    ((3, 100000), (0, 1), 1, [1]),              # long unit reduced axis, few outputs (split)
+]
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32, np.int32, np.int64])
+@pytest.mark.parametrize("op", [SUM, MAX, MIN, ARGMAX, ARGMIN])
+@pytest.mark.parametrize("ci", range(len(CASES)))
+def test_reduce_axes(dev, oracle, dtype, op, ci):
+    X, perm, step, axes = CASES[ci]
+    rng = np.random.default_rng(zlib.crc32(repr((ci, op, str(dtype))).encode()))
+    n = int(np.prod(X))
+    base = rand(rng, n, dtype)
+    if op in (ARGMAX, ARGMIN) and np.dtype(dtype).kind == "i":
+        base = rng.integers(-8, 8, size=n).astype(dtype)  # many ties
+    x = base.reshape(X).transpose(perm)[..., ::step]
+    tx, vx = from_np(dev, x)
+    out = getattr(tx, METH[op])(axes)
+    ref = oracle_reduce(oracle, op, vx, axes, dtype)
+    got = out.to_numpy()
+    assert got.shape == ref.shape
+    if op == SUM and np.dtype(dtype).kind == "f":
+        check_sum(got, ref, np.abs(x).sum(), dtype)
+    else:
+        assert bits_equal(got, ref)
+    # output layout: layout_for_array_copy(rest, K)
+    ls, lm = rt._rstsr.dim_split_axes(tx.layout, axes)
+    assert out.layout == rt._rstsr.layout_for_array_copy(lm, "K")
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32, np.int32, np.int64])
+@pytest.mark.parametrize("op", [SUM, MAX, MIN, ARGMAX, ARGMIN])
+@pytest.mark.parametrize("n", [1, 63, 4097, 1 << 20])
+def test_reduce_all(dev, oracle, dtype, op, n):
+    rng = np.random.default_rng(n + op)
+    x = rand(rng, 2 * n, dtype)[::2]  # strided
+    tx, vx = from_np(dev, x)
+    got = getattr(tx, ALL[op])()
+    ref = oracle.reduce_all(op, vx)
+    if op == SUM and np.dtype(dtype).kind == "f":
+        check_sum(got, ref, np.abs(x).sum(), dtype)
+    else:
+        assert got == ref
+
+
+def test_complex_sum(dev, oracle):
+    rng = np.random.default_rng(2)
+    x = (rng.normal(size=(64, 96)) + 1j * rng.normal(size=(64, 96))).T
+    tx, vx = from_np(dev, x)
+    ref = oracle_reduce(oracle, SUM, vx, [1], np.complex128)
+    check_sum(tx.sum_axes([1]).to_numpy(), ref, np.abs(x).sum(), np.complex128)
+    check_sum(tx.sum_all(), oracle.reduce_all(SUM, vx), np.abs(x).sum(), np.complex128)
+
+
+def test_nan_rules(dev):
+    t = rt.asarray(np.array([np.nan, 1.0, 2.0]), device=dev)
+    assert t.max_all() == 2.0 and t.argmax_all() == 0
+    t = rt.asarray(np.array([1.0, np.nan, 5.0, 5.0]), device=dev)
+    assert t.argmax_all() == 2 and t.max_all() == 5.0
+    t = rt.asarray(np.array([-np.inf, -np.inf]), device=dev)
+    assert t.max_all() == -np.finfo(np.float64).max
+    # NaN at logical index 0 of each reduced row wins; later NaNs ignored (large row -> split path)
+    x = np.random.default_rng(0).normal(size=(3, 50000))
+    x[0, 0] = np.nan
+    x[1, 17] = np.nan
+    x[2, 49999] = 100.0
+    got = rt.asarray(x, device=dev).argmax_axes(1).to_vec()
+    assert got[0] == 0 and got[1] == int(np.nanargmax(x[1])) and got[2] == 49999
+
+
+def test_known_answers(dev):
+    a = rt.arange(3240, device=dev).reshape([12, 15, 18]).swapaxes(-1, -2)
+    assert a[2:-3, 1:-4:2, -1:3:-2].sum_all() == 446586
+    s = rt.arange(3240, device=dev).reshape([4, 6, 15, 9]).transpose([2, 0, 3, 1]).sum_axes([0, -2]).to_numpy()
+    assert s[0, 1] == 27270 and s[1, 2] == 154845 and s[3, 5] == 428220
+    v = rt.asarray(np.array([8, 4, 2, 9, 7, 1, 2, 1, 8, 6, 10, 5], dtype=np.int32).reshape(4, 3), device=dev)
+    assert v.argmin_all() == 5
+    assert v.argmin_axes(-1).to_vec() == [2, 2, 1, 2]
+    assert v.argmin_axes(0).to_vec() == [2, 2, 1]
+    b = rt.asarray(np.array([[3, 6, 9], [4, 10, 5], [8, 3, 2]], dtype=np.int32), device=dev)
+    assert b.argmax_axes(0).to_vec() == [2, 1, 0] and b.argmax_axes(1).to_vec() == [2, 1, 0]
+    m = rt.arange(24.0, device=dev).reshape([2, 3, 4])
+    assert (m.sum_axes((0, 2)).to_numpy() / 8).tolist() == [7.5, 11.5, 15.5]
+    assert (m[::-1, :, ::-2].sum_axes((-1, 1)).to_numpy() / 6).tolist() == [18.0, 6.0]
+    assert m.sum_all() / 24 == 11.5
+
+
+def test_reduce_errors(dev):
+    e = rt.zeros((0,), device=dev)
+    assert e.sum_all() == 0.0
+    with pytest.raises(rt.RSTSRError) as x:
+        e.max_all()
+    assert x.value.args[0] == "InvalidValue"
+    with pytest.raises(rt.RSTSRError) as x:
+        e.argmax_all()
+    assert x.value.args[0] == "InvalidLayout"
+    z = rt.zeros((3, 0), device=dev).sum_axes([1]).to_numpy()
+    assert z.tolist() == [0.0, 0.0, 0.0]
+    with pytest.raises(rt.RSTSRError):
+        rt.zeros((3, 4), device=dev).sum_axes([0, 0])
