@@ -1,0 +1,385 @@
+"""Benchmark of the strided elementwise / reduction / copy path on one MI355X.
+
+Workload (the bandwidth-bound strided configuration): X f64/f32 [256,512,1024] ~ N(0,1),
+b [1,512,1] ~ U(0.5,2), Xi i32 [256,512,1024] ~ U{-2^20..2^20}, bi [1,512,1] likewise.
+V = X.permute(2,0,1)[:, :, ::2] ([1024,256,256], strides (1, 524288, 2048)), bV the same
+view of b. One step runs the whole suite once on device-resident inputs:
+
+    V*bV (f64), V*bV (f32), Vi+bVi (i32),
+    sum / max / argmax of V over axis 0 and over axes (1,2),
+    argmax of Vi over axis 0 and over axes (1,2),
+    X.permute(2,0,1).to_contig()                            (f64, 2.15 GB moved)
+
+value = algorithmic bytes of one step / measured step time (GB/s). The FP64 GEMM leg
+(8192^3, entries N(0,1)*2^u, u in [-8,8]) is reported beside it with its normwise error
+against the Dot2 oracle on sampled rows.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-gemm] [--no-cpu] [--no-check]
+                    [--only OPNAME]
+"""
+import argparse
+import ctypes as C
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "HBM GB/s (elemwise/reduce); f64 GEMM TFLOP/s plain vs split at ≤1e-14"
+HBM_PEAK = 8000.0  # GB/s, MI355X
+FP64_PEAK = 78.6  # TFLOP/s dense FP64 MFMA
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=20)
+    p.add_argument("--warmup", type=int, default=3)
+    p.add_argument("--no-gemm", action="store_true")
+    p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--no-check", action="store_true")
+    p.add_argument("--only", default=None, help="run only this op (for profiling)")
+    p.add_argument("--gemm-n", type=int, default=8192)
+    p.add_argument("--gemm-modes", default="plain,ozaki")
+    return p.parse_args()
+
+
+class Op:
+    def __init__(self, name, fn, nbytes, in_step=True):
+        self.name, self.fn, self.nbytes, self.in_step = name, fn, nbytes, in_step
+
+
+def main():
+    args = parse()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(collectives and sharding are design-only). Run with --gpus 1.", flush=True)
+        sys.exit(2)
+
+    import rstsr_amd as rt
+    from rstsr_amd import abi
+    from rstsr_amd import _rstsr as RL
+
+    L = abi.lib()
+    dev = rt.DeviceHIP(0)
+    ctx = C.c_void_p(dev._ctx_ptr())
+    ev = abi.Events(dev._ctx_ptr())
+
+    # ---------------- inputs (seed 20261202 + config index 1) ----------------
+    t0 = time.time()
+    rng = np.random.default_rng(20261203)
+    X = rng.standard_normal((256, 512, 1024))
+    b = rng.uniform(0.5, 2.0, size=(1, 512, 1))
+    Xi = rng.integers(-(2 ** 20), 2 ** 20, size=(256, 512, 1024), dtype=np.int32, endpoint=True)
+    bi = rng.integers(-(2 ** 20), 2 ** 20, size=(1, 512, 1), dtype=np.int32, endpoint=True)
+    X32 = X.astype(np.float32)
+    b32 = b.astype(np.float32)
+    log(f"[bench] inputs generated in {time.time() - t0:.1f}s")
+    tX, tX32, tXi = rt.asarray(X, device=dev), rt.asarray(X32, device=dev), rt.asarray(Xi, device=dev)
+    tb, tb32, tbi = rt.asarray(b, device=dev), rt.asarray(b32, device=dev), rt.asarray(bi, device=dev)
+    log(f"[bench] inputs resident in HBM after {time.time() - t0:.1f}s")
+
+    def strided(t):
+        return t.transpose([2, 0, 1])[:, :, ::2]
+
+    V, V32, Vi = strided(tX), strided(tX32), strided(tXi)
+    bV, bV32, bVi = strided(tb), strided(tb32), strided(tbi)
+    shapeV = list(V.shape)  # [1024, 256, 256]
+
+    def lay(t):
+        sh, st, off = t.layout
+        return abi.make_layout(sh, st, off)
+
+    def ptr(t):
+        return C.c_void_p(t._t._data_ptr())
+
+    # broadcast views of b to V's shape
+    def bcast(t):
+        sh, st, off = RL.broadcast_layout(V.layout, t.layout, "C")[1]
+        return abi.make_layout(sh, st, off)
+
+    outc = rt._rstsr.c_layout(shapeV)
+    Y64 = rt.empty(shapeV, np.float64, device=dev)
+    Y32 = rt.empty(shapeV, np.float32, device=dev)
+    Yi = rt.empty(shapeV, np.int32, device=dev)
+    lY = abi.make_layout(*outc)
+
+    def red_out(t, axes, dtype):
+        _, lm = RL.dim_split_axes(t.layout, axes)
+        lo = RL.layout_for_array_copy(lm, "K")
+        o = rt.Tensor(RL.empty(dev, lo, rt._code(dtype)))
+        return o, abi.make_layout(*lo)
+
+    red = {}
+    for nm, t, axes, dt in [("f64_ax0", V, [0], np.float64), ("f64_ax12", V, [1, 2], np.float64),
+                            ("u64_ax0", V, [0], np.uint64), ("u64_ax12", V, [1, 2], np.uint64),
+                            ("max_ax0", V, [0], np.float64), ("max_ax12", V, [1, 2], np.float64),
+                            ("i_ax0", Vi, [0], np.uint64), ("i_ax12", Vi, [1, 2], np.uint64)]:
+        red[nm] = red_out(t, axes, dt)
+    P = tX.transpose([2, 0, 1])
+    Pout = rt.empty(list(P.shape), np.float64, device=dev)
+    Cflat = rt.empty([256, 512, 1024], np.float64, device=dev)
+
+    lV, lV32, lVi = lay(V), lay(V32), lay(Vi)
+    lbV, lbV32, lbVi = bcast(bV), bcast(bV32), bcast(bVi)
+    lP, lPout = lay(P), abi.make_layout(*rt._rstsr.c_layout(list(P.shape)))
+    lX, lC = lay(tX), abi.make_layout(*rt._rstsr.c_layout([256, 512, 1024]))
+    ax0 = (C.c_ssize_t * 1)(0)
+    ax12 = (C.c_ssize_t * 2)(1, 2)
+
+    F32, F64, I32 = 0, 1, 2
+    ADD, MUL = 0, 2
+    SUM, MAX, ARGMAX = 0, 1, 3
+    nV = 1024 * 256 * 256
+
+    def mk_bin(op, t, c, lc, a, la, bb, lb):
+        return lambda: abi.check(L.rstsr_hip_op_mutc_refa_refb(ctx, op, t, ptr(c), C.byref(lc), ptr(a), C.byref(la),
+                                                               ptr(bb), C.byref(lb)))
+
+    def mk_red(op, t, a, la, axes, na, out):
+        o, lo = out
+        f = L.rstsr_hip_argreduce_axes if op == ARGMAX else L.rstsr_hip_reduce_axes
+        return lambda: abi.check(f(ctx, op, t, ptr(a), C.byref(la), na, axes, ptr(o), C.byref(lo)))
+
+    ops = [
+        Op("mul_f64", mk_bin(MUL, F64, Y64, lY, tX, lV, tb, lbV), nV * 16 + 512 * 8),
+        Op("mul_f32", mk_bin(MUL, F32, Y32, lY, tX32, lV32, tb32, lbV32), nV * 8 + 512 * 4),
+        Op("add_i32", mk_bin(ADD, I32, Yi, lY, tXi, lVi, tbi, lbVi), nV * 8 + 512 * 4),
+        Op("sum_ax0_f64", mk_red(SUM, F64, tX, lV, ax0, 1, red["f64_ax0"]), nV * 8 + 65536 * 8),
+        Op("sum_ax12_f64", mk_red(SUM, F64, tX, lV, ax12, 2, red["f64_ax12"]), nV * 8 + 1024 * 8),
+        Op("max_ax0_f64", mk_red(MAX, F64, tX, lV, ax0, 1, red["max_ax0"]), nV * 8 + 65536 * 8),
+        Op("max_ax12_f64", mk_red(MAX, F64, tX, lV, ax12, 2, red["max_ax12"]), nV * 8 + 1024 * 8),
+        Op("argmax_ax0_f64", mk_red(ARGMAX, F64, tX, lV, ax0, 1, red["u64_ax0"]), nV * 8 + 65536 * 8),
+        Op("argmax_ax12_f64", mk_red(ARGMAX, F64, tX, lV, ax12, 2, red["u64_ax12"]), nV * 8 + 1024 * 8),
+        Op("argmax_ax0_i32", mk_red(ARGMAX, I32, tXi, lVi, ax0, 1, red["i_ax0"]), nV * 4 + 65536 * 8),
+        Op("argmax_ax12_i32", mk_red(ARGMAX, I32, tXi, lVi, ax12, 2, red["i_ax12"]), nV * 4 + 1024 * 8),
+        Op("to_contig_f64", lambda: abi.check(L.rstsr_hip_assign(ctx, F64, ptr(Pout), C.byref(lPout), F64, ptr(tX),
+                                                                 C.byref(lP))), 2 * 8 * 256 * 512 * 1024),
+        # calibration: contiguous copy of the same 1.07 GB (not part of the step)
+        Op("contig_copy_f64", lambda: abi.check(L.rstsr_hip_assign(ctx, F64, ptr(Cflat), C.byref(lC), F64, ptr(tX),
+                                                                   C.byref(lX))), 2 * 8 * 256 * 512 * 1024,
+           in_step=False),
+    ]
+    if args.only:
+        ops = [o for o in ops if o.name == args.only]
+        if not ops:
+            sys.exit(f"unknown op {args.only}")
+
+    step_ops = [o for o in ops if o.in_step]
+    step_bytes = sum(o.nbytes for o in step_ops)
+
+    # ---------------- timed region ----------------
+    for _ in range(args.warmup):
+        for o in ops:
+            o.fn()
+    dev.synchronize()
+    log("[bench] warmup done")
+    K = args.steps
+    ev.record(0)
+    for _ in range(K):
+        for o in step_ops:
+            o.fn()
+    ev.record(1)
+    dev.synchronize()
+    step_ms = ev.elapsed_ms(0, 1) / max(K, 1)
+    log(f"[bench] {K} steps, {step_ms:.3f} ms/step")
+
+    per_op = {}
+    for o in ops:
+        ev.record(2)
+        for _ in range(K):
+            o.fn()
+        ev.record(3)
+        ms = ev.elapsed_ms(2, 3) / max(K, 1)
+        per_op[o.name] = {"ms": round(ms, 4), "GB_per_launch": round(o.nbytes / 1e9, 4),
+                          "GB/s": round(o.nbytes / 1e9 / (ms / 1e3), 1),
+                          "frac_hbm": round(o.nbytes / 1e9 / (ms / 1e3) / HBM_PEAK, 3)}
+    dev.synchronize()
+    log("[bench] per-op: " + json.dumps(per_op))
+
+    # ---------------- parity at full size ----------------
+    parity = {}
+    if not args.no_check and not args.only:
+        t1 = time.time()
+        Vn = X.transpose(2, 0, 1)[:, :, ::2]
+        bVn = b.transpose(2, 0, 1)[:, :, ::2]
+        parity["mul_f64_bitexact"] = bool(np.array_equal(Y64.to_numpy(), Vn * bVn))
+        parity["mul_f32_bitexact"] = bool(np.array_equal(Y32.to_numpy(), X32.transpose(2, 0, 1)[:, :, ::2] * b32.transpose(2, 0, 1)[:, :, ::2]))
+        parity["add_i32_bitexact"] = bool(np.array_equal(Yi.to_numpy(), Xi.transpose(2, 0, 1)[:, :, ::2] + bi.transpose(2, 0, 1)[:, :, ::2]))
+        parity["to_contig_bitexact"] = bool(np.array_equal(Pout.to_numpy(), np.ascontiguousarray(X.transpose(2, 0, 1))))
+        parity["argmax_ax0_bitexact"] = bool(np.array_equal(red["u64_ax0"][0].to_numpy(), np.argmax(Vn, axis=0)))
+        parity["argmax_ax12_bitexact"] = bool(np.array_equal(red["u64_ax12"][0].to_numpy(), np.argmax(Vn.reshape(1024, -1), axis=1)))
+        Vin = Xi.transpose(2, 0, 1)[:, :, ::2]
+        parity["argmax_i32_ax0_bitexact"] = bool(np.array_equal(red["i_ax0"][0].to_numpy(), np.argmax(Vin, axis=0)))
+        parity["argmax_i32_ax12_bitexact"] = bool(np.array_equal(red["i_ax12"][0].to_numpy(), np.argmax(Vin.reshape(1024, -1), axis=1)))
+        parity["max_ax0_bitexact"] = bool(np.array_equal(red["max_ax0"][0].to_numpy(), Vn.max(axis=0)))
+        parity["max_ax12_bitexact"] = bool(np.array_equal(red["max_ax12"][0].to_numpy(), Vn.reshape(1024, -1).max(axis=1)))
+        s0 = np.sum(Vn.astype(np.longdouble), axis=0)
+        parity["sum_ax0_normwise_err"] = float(np.max(np.abs(red["f64_ax0"][0].to_numpy() - s0)) / np.max(np.abs(s0)))
+        s12 = np.sum(Vn.reshape(1024, -1).astype(np.longdouble), axis=1)
+        parity["sum_ax12_normwise_err"] = float(np.max(np.abs(red["f64_ax12"][0].to_numpy() - s12)) / np.max(np.abs(s12)))
+        parity["sum_tolerance"] = 1e-13
+        parity["all_ok"] = all(v for k, v in parity.items() if k.endswith("bitexact")) and \
+            parity["sum_ax0_normwise_err"] <= 1e-13 and parity["sum_ax12_normwise_err"] <= 1e-13
+        log(f"[bench] parity checked in {time.time() - t1:.1f}s: {json.dumps(parity)}")
+
+    # ---------------- roofline of the dominant kernel ----------------
+    dom = max(step_ops, key=lambda o: per_op[o.name]["ms"]) if step_ops else ops[0]
+    dms = per_op[dom.name]["ms"]
+    achieved = dom.nbytes / 1e9 / (dms / 1e3)
+    roofline = {"bound": "hbm", "kernel": dom.name, "achieved": round(achieved, 1), "peak": HBM_PEAK,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 4), "traffic": None,
+                "algorithmic_bytes_per_launch": dom.nbytes, "avg_launch_ms": dms}
+
+    # ---------------- FP64 GEMM leg ----------------
+    gemm = {}
+    if not args.no_gemm and not args.only:
+        gemm = gemm_leg(args, dev, rt)
+
+    # ---------------- CPU baseline (oracle port of the cpu_rayon control flow) ----------------
+    cpu = None
+    if not args.no_cpu and not args.only:
+        cpu = cpu_leg(X, b, Xi, bi)
+
+    value = step_bytes / 1e9 / (step_ms / 1e3)
+    out = {
+        "metric": METRIC,
+        "value": round(value, 1),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": args.warmup,
+        "ms_per_step": round(step_ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "strided broadcast elementwise + axis reductions + permuted copy: "
+                               "X[256,512,1024] viewed permute(2,0,1)[:,:,::2], b[1,512,1]; f64/f32/i32",
+                   "step_bytes": step_bytes, "ops": [o.name for o in step_ops]},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "ops": per_op,
+        "parity": parity,
+        "gemm": gemm,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def gemm_leg(args, dev, rt):
+    """Plain FP64 MFMA DGEMM (and the split mode when available) on n^3 with wide exponents."""
+    n = args.gemm_n
+    rng = np.random.default_rng(20261204)
+    t0 = time.time()
+    A = rng.standard_normal((n, n)) * np.exp2(rng.integers(-8, 9, size=(n, n)))
+    B = rng.standard_normal((n, n)) * np.exp2(rng.integers(-8, 9, size=(n, n)))
+    tA, tB = rt.asarray(A, device=dev), rt.asarray(B, device=dev)
+    log(f"[bench] gemm inputs {n}^3 ready in {time.time() - t0:.1f}s")
+    from rstsr_amd import abi
+
+    ev = abi.Events(dev._ctx_ptr())
+    res = {"n": n, "flop": 2.0 * n ** 3, "peak_fp64_tflops": FP64_PEAK}
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from oracle_ffi import Oracle
+
+    orc = Oracle()
+    rows = np.array([0, n // 3, (2 * n) // 3, n - 1])
+    ref = orc.gemm_dot2(A[rows], B, nthreads=0)
+    for mode in args.gemm_modes.split(","):
+        try:
+            Cm = rt.matmul(tA, tB, mode=mode)  # warmup + correctness
+        except rt.RSTSRError as e:
+            res[mode] = {"error": str(e.args)}
+            continue
+        dev.synchronize()
+        reps = 3
+        ev.record(4)
+        for _ in range(reps):
+            Cm = rt.matmul(tA, tB, mode=mode)
+        ev.record(5)
+        ms = ev.elapsed_ms(4, 5) / reps
+        got = Cm.to_numpy()[rows]
+        err = float(np.max(np.abs(got - ref)) / np.max(np.abs(ref)))
+        tf = 2.0 * n ** 3 / (ms / 1e3) / 1e12
+        res[mode] = {"ms": round(ms, 3), "TFLOP/s": round(tf, 2), "frac_fp64_peak": round(tf / FP64_PEAK, 3),
+                     "normwise_err_sampled_rows": err, "target_err": 1e-14}
+        log(f"[bench] gemm {mode}: {ms:.2f} ms, {tf:.2f} TFLOP/s, err {err:.2e}")
+        del Cm
+    return res
+
+
+def cpu_leg(X, b, Xi, bi):
+    """Time the oracle's OpenMP port of the cpu_rayon control flow on a bounded sample."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from oracle_ffi import Oracle, View, c_view
+    import ctypes as C
+
+    orc = Oracle()
+    cores = os.cpu_count() or 1
+    # sample: the same ops on X[:, :, :128] (1/8 of the data)
+    Xs, Xis = X[:, :, :128], Xi[:, :, :128]
+    Vs = Xs.transpose(2, 0, 1)[:, :, ::2]
+    bVs = np.broadcast_to(b.transpose(2, 0, 1)[:, :, ::2], Vs.shape)
+    Vis = Xis.transpose(2, 0, 1)[:, :, ::2]
+    bVis = np.broadcast_to(bi.transpose(2, 0, 1)[:, :, ::2], Vis.shape)
+    Ps = Xs.transpose(2, 0, 1)
+    n = Vs.size
+    vV, vb, vVi, vbi, vP = View.of(Vs), View.of(bVs), View.of(Vis), View.of(bVis), View.of(Ps)
+    vY = c_view(Vs.shape, np.float64)
+    vYi = c_view(Vis.shape, np.int32)
+    vPo = c_view(Ps.shape, np.float64)
+    lib = orc.lib
+
+    def red(op, v, axes, dt):
+        nd = 3
+        kept = [v.shape[i] for i in range(nd) if i not in axes]
+        vo = c_view(kept, dt)
+        ax = (C.c_ssize_t * len(axes))(*axes)
+        return lambda: lib.b_reduce_axes(op, 1 if v.base.dtype == np.float64 else 2, v.ptr, C.byref(v.c), len(axes),
+                                         ax, vo.ptr, C.byref(vo.c), cores)
+
+    tasks = [
+        ("mul_f64", lambda: lib.b_binary(2, 1, vY.ptr, C.byref(vY.c), vV.ptr, C.byref(vV.c), vb.ptr, C.byref(vb.c), cores), n * 16),
+        ("add_i32", lambda: lib.b_binary(0, 2, vYi.ptr, C.byref(vYi.c), vVi.ptr, C.byref(vVi.c), vbi.ptr, C.byref(vbi.c), cores), n * 8),
+        ("sum_ax0_f64", red(0, vV, [0], np.float64), n * 8),
+        ("sum_ax12_f64", red(0, vV, [1, 2], np.float64), n * 8),
+        ("max_ax0_f64", red(1, vV, [0], np.float64), n * 8),
+        ("argmax_ax0_f64", red(3, vV, [0], np.uint64), n * 8),
+        ("argmax_ax12_f64", red(3, vV, [1, 2], np.uint64), n * 8),
+        ("to_contig_f64", lambda: lib.b_assign(1, vPo.ptr, C.byref(vPo.c), vP.ptr, C.byref(vP.c), cores), Ps.size * 16),
+    ]
+    tot_bytes, tot_s, reps = 0, 0.0, 0
+    per = {}
+    t_start = time.time()
+    while True:
+        for nm, fn, nb in tasks:
+            t0 = time.perf_counter()
+            fn()
+            dt = time.perf_counter() - t0
+            per.setdefault(nm, []).append(nb / 1e9 / dt)
+            tot_bytes += nb
+            tot_s += dt
+        reps += 1
+        if time.time() - t_start > 10.0 or reps >= 20:
+            break
+    value = tot_bytes / 1e9 / tot_s
+    log(f"[bench] cpu baseline {value:.2f} GB/s over {reps} reps, {cores} threads")
+    return {"value": round(value, 3), "unit": "GB/s", "cores": cores, "kind": "port",
+            "sample": "oracle OpenMP port of rstsr cpu_rayon control flow (greedy order, contiguous inner run, "
+                      "parallel outer loop) on X[:, :, :128] of the same workload (1/8 of the data), "
+                      f"{len(tasks)} ops x {reps} reps, f64/i32",
+            "per_op_GBps": {k: round(float(np.median(v)), 3) for k, v in per.items()}}
+
+
+if __name__ == "__main__":
+    main()

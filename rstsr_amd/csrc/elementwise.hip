@@ -431,8 +431,10 @@ static EwChoice choose(const Dims& d, bool has_a, bool has_b) {
         ch.di = best_di;
         for (int k = 1; k <= 2; ++k) {
             if ((k == 1 && !has_a) || (k == 2 && !has_b)) continue;
+            // stage only operands that walk the input axis (a broadcast along it reads one
+            // element per tile column, which phase 2 fetches directly)
             int64_t si = std::llabs(d.st[best_di][k]), sc = std::llabs(d.st[0][k]);
-            if (si < sc) ch.stage |= (k == 1 ? 1 : 2);
+            if (si > 0 && si < sc) ch.stage |= (k == 1 ? 1 : 2);
         }
         return ch;
     }

@@ -19,6 +19,7 @@
 
 #include "common.hpp"
 #include "registry.hpp"
+#include "gemm.hpp"
 
 namespace rstsr {
 namespace hip {
@@ -28,13 +29,6 @@ typedef double dbl4 __attribute__((ext_vector_type(4)));
 constexpr int GT = 256;  // threads per GEMM block
 constexpr int GBK = 16;  // K slice held in LDS
 constexpr int GPAD = 4;  // LDS row padding (doubles)
-
-struct BatchDesc {
-    int nbd = 0;
-    uint32_t shape[MAXD];
-    FastDiv fd[MAXD];
-    int64_t sa[MAXD], sb[MAXD], sc[MAXD];
-};
 
 template <class T>
 struct GemmArgs {
@@ -298,19 +292,6 @@ __global__ void __launch_bounds__(256) k_gemm_simt(GemmArgs<T> g) {
 
 /* ---------------- host side ---------------- */
 
-// One strided-batched GEMM problem: batch dims in bd, matrices by (rs, cs).
-struct GemmProblem {
-    int64_t M, N, K;
-    const void* a;
-    int64_t rsa, csa;
-    const void* b;
-    int64_t rsb, csb;
-    void* c;
-    int64_t rsc, csc;
-    BatchDesc bd;
-    uint64_t nbatch = 1;
-};
-
 template <class T>
 static void launch_gemm_t(rstsr_hip_ctx* ctx, const GemmProblem& P, T alpha, T beta, bool beta_zero) {
     GemmArgs<T> g;
@@ -361,8 +342,9 @@ static void launch_gemm_t(rstsr_hip_ctx* ctx, const GemmProblem& P, T alpha, T b
     }
 }
 
-// Ozaki-II split GEMM (ozaki.hip); f64 only.
-void ozaki_gemm_f64(rstsr_hip_ctx* ctx, const GemmProblem& P, double alpha, double beta, bool beta_zero);
+void plain_dgemm(rstsr_hip_ctx* ctx, const GemmProblem& P, double alpha, double beta, bool beta_zero) {
+    launch_gemm_t<double>(ctx, P, alpha, beta, beta_zero);
+}
 
 // Zero-fill C (k == 0 with beta == 0: crates-device/rstsr-openblas/src/matmul_impl.rs:78-81).
 template <class T>

@@ -1,0 +1,50 @@
+// Shared GEMM problem description (gemm.hip, ozaki.hip).
+#pragma once
+
+#include "common.hpp"
+
+namespace rstsr {
+namespace hip {
+
+// Batch axes of a strided-batched GEMM, innermost first.
+struct BatchDesc {
+    int nbd = 0;
+    uint32_t shape[MAXD];
+    FastDiv fd[MAXD];
+    int64_t sa[MAXD], sb[MAXD], sc[MAXD];
+};
+
+// One strided-batched GEMM: C[m,n] = alpha * sum_k A[m,k] B[k,n] + beta * C[m,n]
+// per batch; matrices by (row stride, column stride) in elements.
+struct GemmProblem {
+    int64_t M, N, K;
+    const void* a;
+    int64_t rsa, csa;
+    const void* b;
+    int64_t rsb, csb;
+    void* c;
+    int64_t rsc, csc;
+    BatchDesc bd;
+    uint64_t nbatch = 1;
+};
+
+// Host offsets of batch `bi` (outermost-last unravel of bd).
+inline void batch_offsets_host(const BatchDesc& bd, uint64_t bi, int64_t& oa, int64_t& ob, int64_t& oc) {
+    oa = ob = oc = 0;
+    for (int d = 0; d < bd.nbd; ++d) {
+        uint64_t r = bi % bd.shape[d];
+        bi /= bd.shape[d];
+        oa += (int64_t)r * bd.sa[d];
+        ob += (int64_t)r * bd.sb[d];
+        oc += (int64_t)r * bd.sc[d];
+    }
+}
+
+// Ozaki-II split GEMM for f64 (ozaki.hip). Falls back to the plain kernel for
+// non-finite inputs.
+void ozaki_gemm_f64(rstsr_hip_ctx* ctx, const GemmProblem& P, double alpha, double beta, bool beta_zero);
+// Plain FP64 MFMA DGEMM of one problem (gemm.hip).
+void plain_dgemm(rstsr_hip_ctx* ctx, const GemmProblem& P, double alpha, double beta, bool beta_zero);
+
+}  // namespace hip
+}  // namespace rstsr

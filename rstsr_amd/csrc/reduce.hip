@@ -355,16 +355,28 @@ __global__ void __launch_bounds__(RB) k_red_col(const T* __restrict__ a, typenam
     else part[(uint64_t)o * p.nsplit + sp] = acc;
 }
 
+// One wave per output: lane l combines partials l, l+64, ... in order, then a
+// fixed shuffle tree combines the lanes (deterministic for a given split).
 template <class A, class T>
 __global__ void __launch_bounds__(RB) k_red_fin(const T* __restrict__ a, typename A::Out* out,
                                                 const typename A::S* part, RedPlan p) {
-    const uint32_t o = blockIdx.x * RB + threadIdx.x;
-    if (o >= p.M) return;
-    int64_t oin, oout;
-    kept_offsets(p, o, oin, oout);
-    auto acc = part[(uint64_t)o * p.nsplit];
-    for (uint32_t s = 1; s < p.nsplit; ++s) acc = A::combine(acc, part[(uint64_t)o * p.nsplit + s]);
-    out[oout] = A::fin(acc, a[oin]);
+    using S = typename A::S;
+    const uint32_t o = blockIdx.x * (RB / 64) + (threadIdx.x >> 6);
+    const uint32_t lane = threadIdx.x & 63;
+    if (o >= p.M) return;  // whole wave exits together
+    const S* pp = part + (uint64_t)o * p.nsplit;
+    S acc = A::init();
+    for (uint32_t s = lane; s < p.nsplit; s += 64) acc = A::combine(acc, pp[s]);
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) {
+        S other = A::shfl(acc, m);
+        acc = (lane & m) ? A::combine(other, acc) : A::combine(acc, other);
+    }
+    if (lane == 0) {
+        int64_t oin, oout;
+        kept_offsets(p, o, oin, oout);
+        out[oout] = A::fin(acc, a[oin]);
+    }
 }
 
 /* =============================== host planning =============================== */
@@ -516,7 +528,8 @@ static void run_reduce(rstsr_hip_ctx* ctx, const RedSetup& s, const T* a, typena
     }
     check_launch("reduce");
     if (p.nsplit > 1) {
-        hipLaunchKernelGGL((k_red_fin<A, T>), dim3(ceil_div_u32(p.M, RB)), dim3(RB), 0, ctx->stream, a, out, part, p);
+        hipLaunchKernelGGL((k_red_fin<A, T>), dim3(ceil_div_u32(p.M, RB / 64)), dim3(RB), 0, ctx->stream, a, out, part,
+                           p);
         check_launch("reduce finalize");
     }
 }

@@ -160,3 +160,54 @@ def test_gemm_alpha_beta_c_abi(dev, oracle):
     r = lib.rstsr_hip_gemm(ctx, 1, 0, m, n, k, C.byref(al), ptr(ta), k, 1, ptr(tb), n, 1, C.byref(z), ptr(tn), n, 1)
     assert r == 0
     assert np.all(np.isfinite(tn.to_numpy()))
+
+
+# ---------------- Ozaki-II split GEMM (INT8 MFMA + CRT) ----------------
+# Stated tolerance: normwise max|C - C*| / max|C*| <= 1e-14 against the Dot2 oracle (the
+# project's target), on entries N(0,1) * 2^u, u uniform in [-8, 8].
+
+@pytest.mark.parametrize("m,n,k", [(1, 1, 1), (17, 33, 65), (128, 128, 128), (300, 257, 129), (200, 130, 1500),
+                                   (512, 512, 512)])
+@pytest.mark.parametrize("trans", ["nn", "tt"])
+def test_ozaki_normwise(dev, oracle, m, n, k, trans):
+    rng = np.random.default_rng(m + 3 * n + 7 * k)
+    A = wide_exponent(rng, (m, k) if trans[0] == "n" else (k, m))
+    B = wide_exponent(rng, (k, n) if trans[1] == "n" else (n, k))
+    A = A if trans[0] == "n" else A.T
+    B = B if trans[1] == "n" else B.T
+    ta, _ = from_np(dev, A)
+    tb, _ = from_np(dev, B)
+    got = rt.matmul(ta, tb, mode="ozaki").to_numpy()
+    ref = oracle.gemm_dot2(np.ascontiguousarray(A), np.ascontiguousarray(B))
+    err = np.max(np.abs(got - ref)) / np.max(np.abs(ref))
+    assert err <= 1e-14, err
+
+
+def test_ozaki_exact_on_integers(dev):
+    """Small integer matrices are exact after scaling, so the split product is exact."""
+    rng = np.random.default_rng(6)
+    A = rng.integers(-1000, 1000, size=(70, 90)).astype(np.float64)
+    B = rng.integers(-1000, 1000, size=(90, 50)).astype(np.float64)
+    got = rt.matmul(rt.asarray(A, device=dev), rt.asarray(B, device=dev), mode="ozaki").to_numpy()
+    assert np.array_equal(got, (A.astype(np.int64) @ B.astype(np.int64)).astype(np.float64))
+
+
+def test_ozaki_batched_and_zero_rows(dev, oracle):
+    rng = np.random.default_rng(7)
+    A = wide_exponent(rng, (3, 40, 60))
+    A[1, 5, :] = 0.0
+    B = wide_exponent(rng, (60, 30))
+    B[:, 7] = 0.0
+    got = rt.matmul(rt.asarray(A, device=dev), rt.asarray(B, device=dev), mode="ozaki").to_numpy()
+    for b in range(3):
+        ref = oracle.gemm_dot2(A[b], B)
+        assert np.max(np.abs(got[b] - ref)) <= 1e-14 * np.max(np.abs(ref))
+    assert np.all(got[1, 5, :] == 0.0) and np.all(got[:, :, 7] == 0.0)
+
+
+def test_ozaki_nonfinite_falls_back(dev):
+    A = np.ones((20, 30))
+    A[3, 4] = np.inf
+    B = np.ones((30, 10))
+    got = rt.matmul(rt.asarray(A, device=dev), rt.asarray(B, device=dev), mode="ozaki").to_numpy()
+    assert np.all(np.isinf(got[3])) and np.all(got[:3] == 30.0)
