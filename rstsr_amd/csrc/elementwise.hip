@@ -261,6 +261,9 @@ __global__ void __launch_bounds__(EW_BLOCK) k_ew_rows(Body body, EwPlan p, uint3
 // tiled transpose: dims 0 (= dc, output unit axis) and DI (input unit axis) form
 // a TILE x TILE tile; remaining dims are unravelled from the block index.
 // Operands flagged in STAGE (bit0 = A, bit1 = B) are read along DI into LDS.
+// Each thread keeps one tile column (phase 1) / one tile row position (phase 2)
+// fixed, so per-element addressing is a constant stride added to a per-thread
+// base; interior tiles skip the bounds tests.
 template <class Body, int TILE, int STAGE>
 __global__ void __launch_bounds__(EW_BLOCK) k_ew_tiled(Body body, EwPlan p, int di, uint32_t nti, uint32_t ntc,
                                                        FastDiv fti, FastDiv ftc) {
@@ -268,6 +271,8 @@ __global__ void __launch_bounds__(EW_BLOCK) k_ew_tiled(Body body, EwPlan p, int 
     using TB = typename Body::TB;
     constexpr bool SA = (STAGE & 1) != 0;
     constexpr bool SB = (STAGE & 2) != 0;
+    constexpr int ROWS = EW_BLOCK / TILE;  // tile rows covered per pass
+    constexpr int PER = TILE / ROWS;       // passes
     __shared__ TA lds_a[SA ? TILE : 1][SA ? TILE + 1 : 1];
     __shared__ TB lds_b[SB ? TILE : 1][SB ? TILE + 1 : 1];
 
@@ -297,37 +302,52 @@ __global__ void __launch_bounds__(EW_BLOCK) k_ew_tiled(Body body, EwPlan p, int 
     const int64_t sc_c = p.st[0][0], sc_i = p.st[0][di];
     const int64_t sa_c = p.st[1][0], sa_i = p.st[1][di];
     const int64_t sb_c = p.st[2][0], sb_i = p.st[2][di];
-    constexpr int PER = TILE * TILE / EW_BLOCK;
+    const bool full = (i0 + TILE <= Ni) && (c0 + TILE <= Nc);
+    const int tid = threadIdx.x;
 
     if constexpr (SA || SB) {
-        // phase 1: ii fastest -> coalesced along the input's unit axis
+        // phase 1: ii = tid % TILE fixed, cc = k*ROWS + tid/TILE -> coalesced along the input axis
+        const int ii = tid % TILE, cb = tid / TILE;
+        const int64_t pa = oa + (int64_t)(i0 + ii) * sa_i + (int64_t)(c0 + cb) * sa_c, da = (int64_t)ROWS * sa_c;
+        const int64_t pb = ob + (int64_t)(i0 + ii) * sb_i + (int64_t)(c0 + cb) * sb_c, db = (int64_t)ROWS * sb_c;
+        if (full) {
 #pragma unroll
-        for (int k = 0; k < PER; ++k) {
-            int e = k * EW_BLOCK + threadIdx.x;
-            int ii = e % TILE, cc = e / TILE;
-            uint32_t gi = i0 + ii, gc = c0 + cc;
-            if (gi < Ni && gc < Nc) {
-                if constexpr (SA) lds_a[cc][ii] = body.la(oa + (int64_t)gi * sa_i + (int64_t)gc * sa_c);
-                if constexpr (SB) lds_b[cc][ii] = body.lb(ob + (int64_t)gi * sb_i + (int64_t)gc * sb_c);
+            for (int k = 0; k < PER; ++k) {
+                if constexpr (SA) lds_a[k * ROWS + cb][ii] = body.la(pa + k * da);
+                if constexpr (SB) lds_b[k * ROWS + cb][ii] = body.lb(pb + k * db);
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < PER; ++k) {
+                if (i0 + ii < Ni && c0 + k * ROWS + cb < Nc) {
+                    if constexpr (SA) lds_a[k * ROWS + cb][ii] = body.la(pa + k * da);
+                    if constexpr (SB) lds_b[k * ROWS + cb][ii] = body.lb(pb + k * db);
+                }
             }
         }
         __syncthreads();
     }
-    // phase 2: cc fastest -> coalesced along the output's unit axis
+    // phase 2: cc = tid % TILE fixed, ii = k*ROWS + tid/TILE -> coalesced along the output axis
+    const int cc = tid % TILE, ib = tid / TILE;
+    const int64_t pc = oc + (int64_t)(c0 + cc) * sc_c + (int64_t)(i0 + ib) * sc_i, dc2 = (int64_t)ROWS * sc_i;
+    const int64_t qa = oa + (int64_t)(c0 + cc) * sa_c + (int64_t)(i0 + ib) * sa_i, da2 = (int64_t)ROWS * sa_i;
+    const int64_t qb = ob + (int64_t)(c0 + cc) * sb_c + (int64_t)(i0 + ib) * sb_i, db2 = (int64_t)ROWS * sb_i;
+    auto one = [&](int k) {
+        TA x;
+        TB y;
+        if constexpr (SA) x = lds_a[cc][k * ROWS + ib];
+        else x = body.la(qa + k * da2);
+        if constexpr (SB) y = lds_b[cc][k * ROWS + ib];
+        else y = body.lb(qb + k * db2);
+        body.st(pc + k * dc2, x, y);
+    };
+    if (full) {
 #pragma unroll
-    for (int k = 0; k < PER; ++k) {
-        int e = k * EW_BLOCK + threadIdx.x;
-        int cc = e % TILE, ii = e / TILE;
-        uint32_t gi = i0 + ii, gc = c0 + cc;
-        if (gi < Ni && gc < Nc) {
-            TA x;
-            TB y;
-            if constexpr (SA) x = lds_a[cc][ii];
-            else x = body.la(oa + (int64_t)gi * sa_i + (int64_t)gc * sa_c);
-            if constexpr (SB) y = lds_b[cc][ii];
-            else y = body.lb(ob + (int64_t)gi * sb_i + (int64_t)gc * sb_c);
-            body.st(oc + (int64_t)gi * sc_i + (int64_t)gc * sc_c, x, y);
-        }
+        for (int k = 0; k < PER; ++k) one(k);
+    } else {
+#pragma unroll
+        for (int k = 0; k < PER; ++k)
+            if (c0 + cc < Nc && i0 + k * ROWS + ib < Ni) one(k);
     }
 }
 
