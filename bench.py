@@ -276,45 +276,215 @@ def main():
     print(json.dumps(out), flush=True)
 
 
+def _timed(ev, fn, reps):
+    fn()  # warmup (also settles the autotuner)
+    ev.record(4)
+    for _ in range(reps):
+        fn()
+    ev.record(5)
+    return ev.elapsed_ms(4, 5) / reps
+
+
 def gemm_leg(args, dev, rt):
-    """Plain FP64 MFMA DGEMM (and the split mode when available) on n^3 with wide exponents."""
-    n = args.gemm_n
-    rng = np.random.default_rng(20261204)
-    t0 = time.time()
-    A = rng.standard_normal((n, n)) * np.exp2(rng.integers(-8, 9, size=(n, n)))
-    B = rng.standard_normal((n, n)) * np.exp2(rng.integers(-8, 9, size=(n, n)))
-    tA, tB = rt.asarray(A, device=dev), rt.asarray(B, device=dev)
-    log(f"[bench] gemm inputs {n}^3 ready in {time.time() - t0:.1f}s")
+    """Dense FP64 GEMM configurations: plain FP64 MFMA vs Ozaki-II INT8 split, error vs Dot2."""
     from rstsr_amd import abi
 
-    ev = abi.Events(dev._ctx_ptr())
-    res = {"n": n, "flop": 2.0 * n ** 3, "peak_fp64_tflops": FP64_PEAK}
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from oracle_ffi import Oracle
 
     orc = Oracle()
+    ev = abi.Events(dev._ctx_ptr())
+    modes = args.gemm_modes.split(",")
+    out = {}
+
+    def wide(rng, shape):
+        return rng.standard_normal(shape) * np.exp2(rng.integers(-8, 9, size=shape))
+
+    # ---- square n^3 with wide exponents ----
+    n = args.gemm_n
+    rng = np.random.default_rng(20261204)
+    t0 = time.time()
+    A = wide(rng, (n, n))
+    B = wide(rng, (n, n))
+    tA, tB = rt.asarray(A, device=dev), rt.asarray(B, device=dev)
+    log(f"[bench] gemm inputs {n}^3 ready in {time.time() - t0:.1f}s")
     rows = np.array([0, n // 3, (2 * n) // 3, n - 1])
     ref = orc.gemm_dot2(A[rows], B, nthreads=0)
-    for mode in args.gemm_modes.split(","):
-        try:
-            Cm = rt.matmul(tA, tB, mode=mode)  # warmup + correctness
-        except rt.RSTSRError as e:
-            res[mode] = {"error": str(e.args)}
-            continue
-        dev.synchronize()
-        reps = 3
-        ev.record(4)
-        for _ in range(reps):
-            Cm = rt.matmul(tA, tB, mode=mode)
-        ev.record(5)
-        ms = ev.elapsed_ms(4, 5) / reps
-        got = Cm.to_numpy()[rows]
-        err = float(np.max(np.abs(got - ref)) / np.max(np.abs(ref)))
+    res = {"shape": [n, n, n], "flop": 2.0 * n ** 3}
+    for mode in modes:
+        holder = {}
+
+        def run():
+            holder["c"] = rt.matmul(tA, tB, mode=mode)
+
+        ms = _timed(ev, run, 3)
+        err = float(np.max(np.abs(holder["c"].to_numpy()[rows] - ref)) / np.max(np.abs(ref)))
         tf = 2.0 * n ** 3 / (ms / 1e3) / 1e12
         res[mode] = {"ms": round(ms, 3), "TFLOP/s": round(tf, 2), "frac_fp64_peak": round(tf / FP64_PEAK, 3),
-                     "normwise_err_sampled_rows": err, "target_err": 1e-14}
-        log(f"[bench] gemm {mode}: {ms:.2f} ms, {tf:.2f} TFLOP/s, err {err:.2e}")
-        del Cm
+                     "normwise_err_sampled_rows": err}
+        log(f"[bench] dense {n}^3 {mode}: {ms:.2f} ms {tf:.2f} TFLOP/s err {err:.2e}")
+        holder.clear()
+    out["dense_square"] = res
+    del tA, tB, A, B
+
+    # ---- broadcast batch [64,1024,1024] % [1024,1024] (rule 6) ----
+    rng = np.random.default_rng(20261205)
+    A3 = wide(rng, (64, 1024, 1024))
+    B2 = wide(rng, (1024, 1024))
+    tA3, tB2 = rt.asarray(A3, device=dev), rt.asarray(B2, device=dev)
+    ref = orc.gemm_dot2(A3[7, :4], B2)
+    res = {"shape": [64, 1024, 1024, 1024], "flop": 64 * 2.0 * 1024 ** 3}
+    for mode in modes:
+        holder = {}
+
+        def run():
+            holder["c"] = rt.matmul(tA3, tB2, mode=mode)
+
+        ms = _timed(ev, run, 3)
+        err = float(np.max(np.abs(holder["c"].to_numpy()[7, :4] - ref)) / np.max(np.abs(ref)))
+        tf = res["flop"] / (ms / 1e3) / 1e12
+        res[mode] = {"ms": round(ms, 3), "TFLOP/s": round(tf, 2), "normwise_err_sampled_rows": err}
+        log(f"[bench] batch64 {mode}: {ms:.2f} ms {tf:.2f} TFLOP/s err {err:.2e}")
+        holder.clear()
+    out["batched_broadcast"] = res
+    del tA3, tB2, A3
+
+    # ---- RI contraction (ia|jb) = sum_P B[P,i,a] B[P,j,b], B [1600,60,400] ~ N(0,0.1) ----
+    rng = np.random.default_rng(20261206)
+    Bri = rng.normal(0.0, 0.1, size=(1600, 60, 400))
+    tB = rt.asarray(Bri, device=dev)
+    lhs = tB.permute([1, 2, 0])            # [60, 400, 1600]  (i, a, P)
+    rhs = tB.reshape([1600, 24000])        # [1600, 24000]    (P, (j, b))
+    Ai = np.ascontiguousarray(Bri[:, 5, :].T[:4])  # rows a=0..3 of C_5
+    ref = orc.gemm_dot2(Ai, Bri.reshape(1600, 24000))
+    res = {"shape": [60, 400, 24000, 1600], "flop": 60 * 2.0 * 400 * 24000 * 1600}
+    for mode in modes:
+        holder = {}
+
+        def run():
+            holder["c"] = rt.matmul(lhs, rhs, mode=mode)
+
+        ms = _timed(ev, run, 2)
+        got = holder["c"][5, 0:4].to_numpy()
+        err = float(np.max(np.abs(got - ref)) / np.max(np.abs(ref)))
+        tf = res["flop"] / (ms / 1e3) / 1e12
+        res[mode] = {"ms": round(ms, 3), "TFLOP/s": round(tf, 2), "normwise_err_sampled_rows": err}
+        log(f"[bench] RI {mode}: {ms:.2f} ms {tf:.2f} TFLOP/s err {err:.2e}")
+        holder.clear()
+    out["ri_contraction"] = res
+    del tB, lhs, rhs, Bri
+
+    # ---- 512 uneven GEMMs, m,n,k ~ U{32..1024 step 8}, one grouped launch ----
+    rng = np.random.default_rng(20261207)
+    dims = rng.integers(4, 129, size=(512, 3)) * 8
+    pool = rng.standard_normal(1 << 24)  # 128 MiB operand pool; operands are views into it
+    tpool = rt.asarray(pool, device=dev)
+    offs = rng.integers(0, (1 << 24) - 1024 * 1024, size=(512, 2))
+    tot_c = int(sum(int(m) * int(n) for m, n, _ in dims))
+    tC = rt.empty([tot_c], np.float64, device=dev)
+    probs, coff = [], 0
+    for g, (m, n, k) in enumerate(dims):
+        m, n, k = int(m), int(n), int(k)
+        ta = tpool._view(([m, k], [k, 1], int(offs[g, 0])))
+        tb = tpool._view(([k, n], [n, 1], int(offs[g, 1])))
+        tc = tC._view(([m, n], [n, 1], coff))
+        coff += m * n
+        probs.append((ta, tb, tc))
+    flop = float(sum(2.0 * int(m) * int(n) * int(k) for m, n, k in dims))
+    res = {"count": 512, "flop": flop}
+    check = [0, 100, 511]
+    refs = {}
+    for g in check:
+        m, n, k = (int(x) for x in dims[g])
+        Ah = pool[offs[g, 0]:offs[g, 0] + m * k].reshape(m, k)
+        Bh = pool[offs[g, 1]:offs[g, 1] + k * n].reshape(k, n)
+        refs[g] = orc.gemm_dot2(Ah, Bh)
+    for mode in modes:
+        ms = _timed(ev, lambda: rt.gemm_grouped(probs, mode=mode), 3)
+        err = max(float(np.max(np.abs(probs[g][2].to_numpy() - refs[g])) / np.max(np.abs(refs[g]))) for g in check)
+        tf = flop / (ms / 1e3) / 1e12
+        res[mode] = {"ms": round(ms, 3), "TFLOP/s": round(tf, 2), "normwise_err_sampled": err}
+        log(f"[bench] grouped512 {mode}: {ms:.2f} ms {tf:.2f} TFLOP/s err {err:.2e}")
+    out["uneven_grouped"] = res
+    del probs, tpool, tC
+
+    # ---- hipGraph-captured c128 chain, 20 steps ----
+    out["graph_chain_c128"] = chain_leg(dev, rt, orc, modes)
+    return out
+
+
+def chain_leg(dev, rt, orc, modes, n=4096, steps=20):
+    """Z1 += v; Z3 = Z1 * Z2; Z4 = Z3 % Z2; s = Z4.sum_axes(1): uncaptured vs one captured graph."""
+    from rstsr_amd import abi
+
+    ev = abi.Events(dev._ctx_ptr())
+    rng = np.random.default_rng(20261208)
+    z1h = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    z2h = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    vh = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    Z2, V = rt.asarray(z2h, device=dev), rt.asarray(vh, device=dev)
+    res = {"shape": [n, n], "steps": steps, "flop_per_step": 8.0 * n ** 3}
+    # oracle for the last step, sampled rows of s
+    z = z1h.copy()
+    for _ in range(steps):
+        z = z + vh
+    z3 = z * z2h
+    rows = [0, n // 2, n - 1]
+    r_re = orc.gemm_dot2(np.ascontiguousarray(z3.real[rows]), z2h.real) - orc.gemm_dot2(
+        np.ascontiguousarray(z3.imag[rows]), z2h.imag)
+    r_im = orc.gemm_dot2(np.ascontiguousarray(z3.real[rows]), z2h.imag) + orc.gemm_dot2(
+        np.ascontiguousarray(z3.imag[rows]), z2h.real)
+    s_ref = (r_re + 1j * r_im).sum(axis=1)
+    for mode in modes:
+        Z1 = rt.asarray(z1h, device=dev)
+        holder = {}
+
+        def step():
+            nonlocal Z1
+            Z1 += V
+            Z3 = Z1 * Z2
+            Z4 = rt.matmul(Z3, Z2, mode=mode)
+            holder["s"] = Z4.sum_axes([1])
+            holder["keep"] = (Z3, Z4)
+
+        step()  # settles autotuner / workspaces
+        Z1.assign(rt.asarray(z1h, device=dev))
+        dev.synchronize()
+        t0 = time.perf_counter()
+        ev.record(6)
+        for _ in range(steps):
+            step()
+        ev.record(7)
+        eager_ms = ev.elapsed_ms(6, 7) / steps
+        wall_eager = (time.perf_counter() - t0) * 1e3 / steps
+        err_eager = float(np.max(np.abs(holder["s"].to_numpy()[rows] - s_ref)) / np.max(np.abs(s_ref)))
+        # captured: the whole 20-step chain as one graph
+        Z1.assign(rt.asarray(z1h, device=dev))
+        keep = []
+        with rt.capture(dev) as g:
+            for _ in range(steps):
+                Z1 += V
+                Z3 = Z1 * Z2
+                Z4 = rt.matmul(Z3, Z2, mode=mode)
+                s = Z4.sum_axes([1])
+                keep.append((Z3, Z4, s))
+        Z1.assign(rt.asarray(z1h, device=dev))
+        dev.synchronize()
+        t0 = time.perf_counter()
+        ev.record(6)
+        g.launch()
+        ev.record(7)
+        graph_ms = ev.elapsed_ms(6, 7) / steps
+        wall_graph = (time.perf_counter() - t0) * 1e3 / steps
+        err_graph = float(np.max(np.abs(keep[-1][2].to_numpy()[rows] - s_ref)) / np.max(np.abs(s_ref)))
+        res[mode] = {"eager_ms_per_step": round(eager_ms, 3), "graph_ms_per_step": round(graph_ms, 3),
+                     "eager_wall_ms_per_step": round(wall_eager, 3), "graph_wall_ms_per_step": round(wall_graph, 3),
+                     "launch_overhead_saved_ms_per_step": round(eager_ms - graph_ms, 4),
+                     "err_last_step_eager": err_eager, "err_last_step_graph": err_graph, "target_err": 1e-13,
+                     "zgemm_TFLOP/s_graph": round(8.0 * n ** 3 / (graph_ms / 1e3) / 1e12, 2)}
+        log(f"[bench] chain {mode}: eager {eager_ms:.3f} ms/step, graph {graph_ms:.3f} ms/step, "
+            f"err {err_eager:.2e}/{err_graph:.2e}")
+        del keep, g, holder
     return res
 
 

@@ -61,7 +61,8 @@ typedef enum {
     RSTSR_RED_ARGMIN = 4
 } rstsr_redop_t;
 
-/* GEMM arithmetic: plain FP64 MFMA, or Ozaki-II split over INT8 MFMA with CRT. */
+/* GEMM arithmetic: plain FP64 MFMA, or Ozaki-II split over INT8 MFMA with CRT (f64; c128 as
+ * four real products with real alpha/beta). */
 typedef enum { RSTSR_GEMM_PLAIN = 0, RSTSR_GEMM_OZAKI_INT8 = 1 } rstsr_gemm_mode_t;
 
 /* Layout<IxD>: rstsr-common/src/layout/layoutbase.rs:15-23 (shape: usize, stride: isize, offset: usize). */
@@ -165,6 +166,32 @@ int rstsr_hip_matmul(rstsr_hip_ctx_t* ctx, rstsr_dtype_t t, void* c, const rstsr
 int rstsr_hip_gemm(rstsr_hip_ctx_t* ctx, rstsr_dtype_t t, rstsr_gemm_mode_t mode, size_t m, size_t n, size_t k,
                    const void* alpha, const void* a, ptrdiff_t rsa, ptrdiff_t csa, const void* b, ptrdiff_t rsb,
                    ptrdiff_t csb, const void* beta, void* c, ptrdiff_t rsc, ptrdiff_t csc);
+
+/* One problem of a grouped GEMM (uneven shapes, each operand by row/column strides). */
+typedef struct {
+    size_t m, n, k;
+    const void* a;
+    ptrdiff_t rsa, csa;
+    const void* b;
+    ptrdiff_t rsb, csb;
+    void* c;
+    ptrdiff_t rsc, csc;
+} rstsr_gemm_desc_t;
+/* Grouped GEMM: C_g = alpha A_g B_g + beta C_g for count uneven problems. Plain f64/c128 run as
+ * ONE persistent launch whose tile shape the autotuner picks; the split mode and other dtypes
+ * run one call per problem. No trait in the reference takes a list of problems; the closest is
+ * DeviceFaer's batched loop over matmul tasks (device_faer/matmul.rs:151-199). */
+int rstsr_hip_gemm_grouped(rstsr_hip_ctx_t* ctx, rstsr_dtype_t t, rstsr_gemm_mode_t mode, size_t count,
+                           const rstsr_gemm_desc_t* descs, const void* alpha, const void* beta);
+
+/* ---- hipGraph capture of the context stream (no counterpart in the reference) ----
+ * Everything enqueued between begin and end is recorded, not run; entry points that hand
+ * values to the host (*_all, memcpy_d2h, allclose) cannot be captured and fail. */
+int rstsr_hip_graph_begin(rstsr_hip_ctx_t* ctx);
+/* Ends capture and returns an executable graph (hipGraphExec_t) in *graph_exec. */
+int rstsr_hip_graph_end(rstsr_hip_ctx_t* ctx, void** graph_exec);
+int rstsr_hip_graph_launch(rstsr_hip_ctx_t* ctx, void* graph_exec);
+int rstsr_hip_graph_destroy(void* graph_exec);
 
 /* ---- timing on the context stream (no counterpart in the reference; used by benchmarks) ---- */
 int rstsr_hip_event_record(rstsr_hip_ctx_t* ctx, int slot);

@@ -393,6 +393,88 @@ def allclose(a, b, rtol=1e-5, atol=1e-8, equal_nan=False):
     return a._t.allclose(b._t, rtol, atol, equal_nan)
 
 
+def _elem_ptr(t):
+    import ctypes as C
+
+    return C.c_void_p(t._t._data_ptr() + t.offset * t.dtype.itemsize)
+
+
+def gemm_grouped(problems, mode="plain", alpha=1.0, beta=0.0):
+    """C_g = alpha * A_g @ B_g + beta * C_g for a list of 2-D (A, B, C) tensors of uneven shapes.
+
+    Plain f64 / c128 run as one persistent launch with autotuned tiles."""
+    import ctypes as C
+    from . import abi
+
+    if not problems:
+        return
+    dt = problems[0][0].dtype
+    descs = (abi.GemmDesc * len(problems))()
+    for i, (a, b, c) in enumerate(problems):
+        if a.ndim != 2 or b.ndim != 2 or c.ndim != 2:
+            raise RSTSRError("InvalidLayout", "gemm_grouped takes 2-D operands")
+        m, k = a.shape
+        k2, n = b.shape
+        if k != k2 or c.shape != (m, n):
+            raise RSTSRError("InvalidLayout", "gemm_grouped shape mismatch")
+        descs[i] = abi.GemmDesc(m, n, k, _elem_ptr(a), a.stride[0], a.stride[1], _elem_ptr(b), b.stride[0],
+                                b.stride[1], _elem_ptr(c), c.stride[0], c.stride[1])
+    al = np.array(alpha, dtype=dt)
+    be = np.array(beta, dtype=dt)
+    ctx = C.c_void_p(problems[0][0].device._ctx_ptr())
+    m = {"plain": GEMM_PLAIN, "ozaki": GEMM_OZAKI}[mode]
+    abi.check(abi.lib().rstsr_hip_gemm_grouped(ctx, _code(dt), m, len(problems), descs,
+                                               al.ctypes.data_as(C.c_void_p), be.ctypes.data_as(C.c_void_p)))
+
+
+class capture:
+    """Record device work on a DeviceHIP's stream into a hipGraph.
+
+        with rt.capture(dev) as g:
+            z1 += v; z3 = z1 * z2; s = (z3 % z2).sum_axes([1])
+        g.launch(); dev.synchronize()
+
+    Tensors created inside the block must stay alive while the graph is used."""
+
+    def __init__(self, device):
+        import ctypes as C
+
+        self.device = device
+        self.ctx = C.c_void_p(device._ctx_ptr())
+        self.exec = None
+
+    def __enter__(self):
+        from . import abi
+
+        abi.check(abi.lib().rstsr_hip_graph_begin(self.ctx))
+        return self
+
+    def __exit__(self, et, ev, tb):
+        import ctypes as C
+        from . import abi
+
+        g = C.c_void_p()
+        rc = abi.lib().rstsr_hip_graph_end(self.ctx, C.byref(g))
+        if et is None:
+            abi.check(rc)
+            self.exec = g
+        return False
+
+    def launch(self):
+        from . import abi
+
+        abi.check(abi.lib().rstsr_hip_graph_launch(self.ctx, self.exec))
+
+    def __del__(self):
+        try:
+            if self.exec:
+                from . import abi
+
+                abi.lib().rstsr_hip_graph_destroy(self.exec)
+        except Exception:
+            pass
+
+
 def add(a, b):
     return a + b
 

@@ -17,6 +17,15 @@ class Layout(C.Structure):
     ]
 
 
+class GemmDesc(C.Structure):
+    _fields_ = [
+        ("m", C.c_size_t), ("n", C.c_size_t), ("k", C.c_size_t),
+        ("a", C.c_void_p), ("rsa", C.c_ssize_t), ("csa", C.c_ssize_t),
+        ("b", C.c_void_p), ("rsb", C.c_ssize_t), ("csb", C.c_ssize_t),
+        ("c", C.c_void_p), ("rsc", C.c_ssize_t), ("csc", C.c_ssize_t),
+    ]
+
+
 def make_layout(shape, stride, offset=0):
     """rstsr_layout_t owning its arrays (keep the returned object alive across the call)."""
     n = len(shape)
@@ -51,6 +60,11 @@ def lib():
         "rstsr_hip_event_record": [vp, C.c_int],
         "rstsr_hip_event_elapsed": [vp, C.c_int, C.c_int, C.POINTER(C.c_float)],
         "rstsr_hip_synchronize": [vp],
+        "rstsr_hip_gemm_grouped": [vp, C.c_int, C.c_int, C.c_size_t, C.POINTER(GemmDesc), vp, vp],
+        "rstsr_hip_graph_begin": [vp],
+        "rstsr_hip_graph_end": [vp, C.POINTER(C.c_void_p)],
+        "rstsr_hip_graph_launch": [vp, vp],
+        "rstsr_hip_graph_destroy": [vp],
     }
     for name, args in sig.items():
         f = getattr(L, name)

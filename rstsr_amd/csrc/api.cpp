@@ -66,23 +66,35 @@ void check_extent_bytes(const void* ptr, size_t bytes, const char* name) {
         RSTSR_RAISE(ValueOutOfRange, std::string("copy of operand ") + name + " reaches outside its buffer");
 }
 
-void* workspace(rstsr_hip_ctx* ctx, size_t bytes) {
-    if (bytes <= ctx->ws.bytes) return ctx->ws.ptr;
-    size_t nb = std::max<size_t>(bytes, ctx->ws.bytes * 2);
+static void* grow(rstsr_hip_ctx* ctx, Workspace& w, size_t bytes) {
+    if (bytes <= w.bytes) return w.ptr;
+    size_t nb = std::max<size_t>(bytes, w.bytes * 2);
     nb = (nb + 4095) & ~(size_t)4095;
-    if (ctx->ws.ptr) {
-        HIP_CHECK(hipStreamSynchronize(ctx->stream));
-        HIP_CHECK(hipFree(ctx->ws.ptr));
-        ctx->ws.ptr = nullptr;
-        ctx->ws.bytes = 0;
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    HIP_CHECK(hipStreamIsCapturing(ctx->stream, &cs));
+    if (w.ptr) {
+        if (cs != hipStreamCaptureStatusNone) {
+            ctx->retired.push_back(w.ptr);
+        } else {
+            HIP_CHECK(hipStreamSynchronize(ctx->stream));
+            HIP_CHECK(hipFree(w.ptr));
+        }
+        w.ptr = nullptr;
+        w.bytes = 0;
     }
     void* p = nullptr;
     hipError_t e = hipMalloc(&p, nb);
-    if (e != hipSuccess) throw Error(ErrorKind::MemoryError, "workspace hipMalloc failed: " + std::string(hipGetErrorString(e)));
-    ctx->ws.ptr = p;
-    ctx->ws.bytes = nb;
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        throw Error(ErrorKind::MemoryError, "workspace hipMalloc failed: " + std::string(hipGetErrorString(e)));
+    }
+    w.ptr = p;
+    w.bytes = nb;
     return p;
 }
+
+void* workspace(rstsr_hip_ctx* ctx, size_t bytes) { return grow(ctx, ctx->ws, bytes); }
+void* aux_buffer(rstsr_hip_ctx* ctx, size_t bytes) { return grow(ctx, ctx->aux, bytes); }
 
 }  // namespace hip
 }  // namespace rstsr
@@ -138,6 +150,8 @@ int rstsr_hip_ctx_destroy(rstsr_hip_ctx_t* ctx) {
         bind(ctx);
         HIP_CHECK(hipStreamSynchronize(ctx->stream));
         if (ctx->ws.ptr) HIP_CHECK(hipFree(ctx->ws.ptr));
+        if (ctx->aux.ptr) HIP_CHECK(hipFree(ctx->aux.ptr));
+        for (void* p : ctx->retired) HIP_CHECK(hipFree(p));
         for (auto& ev : ctx->events)
             if (ev) HIP_CHECK(hipEventDestroy(ev));
         if (ctx->host_pinned) HIP_CHECK(hipHostFree(ctx->host_pinned));
@@ -239,6 +253,41 @@ int rstsr_hip_event_elapsed(rstsr_hip_ctx_t* ctx, int s0, int s1, float* ms) {
         if (s0 < 0 || s0 >= 16 || s1 < 0 || s1 >= 16 || !ms) RSTSR_RAISE(InvalidValue, "bad event arguments");
         HIP_CHECK(hipEventSynchronize(ctx->events[s1]));
         HIP_CHECK(hipEventElapsedTime(ms, ctx->events[s0], ctx->events[s1]));
+    });
+}
+
+int rstsr_hip_graph_begin(rstsr_hip_ctx_t* ctx) {
+    return guarded([&] {
+        bind(ctx);
+        HIP_CHECK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeRelaxed));
+    });
+}
+
+int rstsr_hip_graph_end(rstsr_hip_ctx_t* ctx, void** graph_exec) {
+    return guarded([&] {
+        bind(ctx);
+        if (!graph_exec) RSTSR_RAISE(InvalidValue, "null output");
+        hipGraph_t g = nullptr;
+        HIP_CHECK(hipStreamEndCapture(ctx->stream, &g));
+        hipGraphExec_t ex = nullptr;
+        hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) throw Error(ErrorKind::DeviceError, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+        *graph_exec = (void*)ex;
+    });
+}
+
+int rstsr_hip_graph_launch(rstsr_hip_ctx_t* ctx, void* graph_exec) {
+    return guarded([&] {
+        bind(ctx);
+        if (!graph_exec) RSTSR_RAISE(InvalidValue, "null graph");
+        HIP_CHECK(hipGraphLaunch((hipGraphExec_t)graph_exec, ctx->stream));
+    });
+}
+
+int rstsr_hip_graph_destroy(void* graph_exec) {
+    return guarded([&] {
+        if (graph_exec) HIP_CHECK(hipGraphExecDestroy((hipGraphExec_t)graph_exec));
     });
 }
 

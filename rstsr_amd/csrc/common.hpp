@@ -66,7 +66,9 @@ struct rstsr_hip_ctx {
     int num_cu = 0;
     size_t l2_bytes = 0;
     std::mutex mu;  // serialises workspace growth and host staging on one context
-    rstsr::hip::Workspace ws;
+    rstsr::hip::Workspace ws;   // partials, residue planes, scalar results
+    rstsr::hip::Workspace aux;  // small metadata (grouped GEMM descriptors)
+    std::vector<void*> retired;  // buffers replaced while a stream was being captured
     void* host_pinned = nullptr;  // 4 KiB pinned staging for scalar results
     hipEvent_t events[16] = {};
 };
@@ -81,8 +83,11 @@ inline void bind(rstsr_hip_ctx* ctx) {
 }
 
 // Return a device buffer of at least `bytes` owned by the context. Growth is
-// ordered after all work already enqueued on the context stream.
+// ordered after all work already enqueued on the context stream; during stream
+// capture the old buffer is retired (freed with the context) instead, so a
+// captured graph keeps valid addresses.
 void* workspace(rstsr_hip_ctx* ctx, size_t bytes);
+void* aux_buffer(rstsr_hip_ctx* ctx, size_t bytes);
 
 /* ------------------------------ element types ------------------------------ */
 

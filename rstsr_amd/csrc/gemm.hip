@@ -15,6 +15,10 @@
 // operand's unit-stride axis (template flags), so both row- and column-major
 // operands read coalesced. ZGEMM uses the same kernel with separate real and
 // imaginary LDS planes and four real MFMAs per complex tile (4M form).
+#include <cstdio>
+#include <cstdlib>
+#include <map>
+#include <string>
 #include <type_traits>
 
 #include "common.hpp"
@@ -79,25 +83,43 @@ __device__ __forceinline__ uint32_t xcd_remap(uint32_t t, uint32_t ntiles) {
     return (t % NX) * (ntiles / NX) + t / NX;
 }
 
+// Operands of one C tile's problem.
+template <class T>
+struct TileArgs {
+    int64_t M, N, K;
+    const T* A;
+    int64_t rsa, csa;
+    const T* B;
+    int64_t rsb, csb;
+    T* C;
+    int64_t rsc, csc;
+};
+
+// LDS planes of one block (re/im planes for complex).
+template <class T, int WM, int WN>
+struct MfmaSmem {
+    static constexpr int NP = CplxOf<T>::value ? 2 : 1;
+    static constexpr int BM = 2 * 16 * WM;
+    static constexpr int BN = 2 * 16 * WN;
+    double As[NP][GBK][BM + GPAD];
+    double Bs[NP][GBK][BN + GPAD];
+};
+
+// One (16*2*WM) x (16*2*WN) C tile: C = alpha * A B + beta * C (beta_zero: C not read).
 template <class T, int WM, int WN, bool A_MC, bool B_KC>
-__global__ void __launch_bounds__(GT) k_gemm_mfma(GemmArgs<T> g) {
+__device__ __forceinline__ void mfma_tile(const TileArgs<T>& g, uint32_t tm, uint32_t tn, T alpha, T beta,
+                                          int beta_zero, MfmaSmem<T, WM, WN>& sm) {
     constexpr bool CX = CplxOf<T>::value;
-    constexpr int NP = CX ? 2 : 1;  // LDS planes (re, im)
-    constexpr int BM = 2 * 16 * WM;
-    constexpr int BN = 2 * 16 * WN;
+    constexpr int NP = MfmaSmem<T, WM, WN>::NP;
+    constexpr int BM = MfmaSmem<T, WM, WN>::BM;
+    constexpr int BN = MfmaSmem<T, WM, WN>::BN;
     constexpr int PA = GBK * BM / GT;  // A elements per thread per slice
     constexpr int PB = GBK * BN / GT;
-    __shared__ double As[NP][GBK][BM + GPAD];
-    __shared__ double Bs[NP][GBK][BN + GPAD];
-
-    const uint32_t ntiles = g.tiles_m * g.tiles_n;
-    const uint32_t tile = xcd_remap(blockIdx.x, ntiles);
-    const uint32_t tm = tile / g.tiles_n, tn = tile - (tile / g.tiles_n) * g.tiles_n;
-    int64_t boa, bob, boc;
-    batch_offsets(g.bd, g.batch0 + blockIdx.y, boa, bob, boc);
-    const T* A = g.a + boa;
-    const T* B = g.b + bob;
-    T* C = g.c + boc;
+    auto& As = sm.As;
+    auto& Bs = sm.Bs;
+    const T* A = g.A;
+    const T* B = g.B;
+    T* C = g.C;
 
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
@@ -218,21 +240,73 @@ __global__ void __launch_bounds__(GT) k_gemm_mfma(GemmArgs<T> g) {
                 if (gm < g.M && gn < g.N) {
                     T* cp = C + gm * g.rsc + gn * g.csc;
                     if constexpr (!CX) {
-                        double v = g.alpha * acc[0][i][j][r];
-                        if (!g.beta_zero) v += g.beta * (*cp);
+                        double v = alpha * acc[0][i][j][r];
+                        if (!beta_zero) v += beta * (*cp);
                         *cp = v;
                     } else {
                         double xr = acc[0][i][j][r], xi = acc[1][i][j][r];
-                        c128 v{g.alpha.re * xr - g.alpha.im * xi, g.alpha.re * xi + g.alpha.im * xr};
-                        if (!g.beta_zero) {
+                        c128 v{alpha.re * xr - alpha.im * xi, alpha.re * xi + alpha.im * xr};
+                        if (!beta_zero) {
                             c128 o = *cp;
-                            v.re += g.beta.re * o.re - g.beta.im * o.im;
-                            v.im += g.beta.re * o.im + g.beta.im * o.re;
+                            v.re += beta.re * o.re - beta.im * o.im;
+                            v.im += beta.re * o.im + beta.im * o.re;
                         }
                         *cp = v;
                     }
                 }
             }
+}
+
+// Strided-batched GEMM: one C tile per block, batch index from blockIdx.y.
+// run_if (nullable): the whole grid exits unless *run_if != 0 (device-side
+// selection between alternative paths without a host round trip).
+template <class T, int WM, int WN, bool A_MC, bool B_KC>
+__global__ void __launch_bounds__(GT) k_gemm_mfma(GemmArgs<T> g, const int* run_if) {
+    if (run_if && *run_if == 0) return;
+    __shared__ MfmaSmem<T, WM, WN> sm;
+    const uint32_t ntiles = g.tiles_m * g.tiles_n;
+    const uint32_t tile = xcd_remap(blockIdx.x, ntiles);
+    const uint32_t tm = tile / g.tiles_n, tn = tile - (tile / g.tiles_n) * g.tiles_n;
+    int64_t boa, bob, boc;
+    batch_offsets(g.bd, g.batch0 + blockIdx.y, boa, bob, boc);
+    TileArgs<T> t{g.M, g.N, g.K, g.a + boa, g.rsa, g.csa, g.b + bob, g.rsb, g.csb, g.c + boc, g.rsc, g.csc};
+    mfma_tile<T, WM, WN, A_MC, B_KC>(t, tm, tn, g.alpha, g.beta, g.beta_zero, sm);
+}
+
+// Grouped (uneven) GEMMs in one persistent launch. Problem g owns tiles
+// [prefix[g], prefix[g+1]); blocks stride over all tiles. flags bit0: A
+// m-contiguous, bit1: B k-contiguous (selects the coalesced load mapping).
+template <class T>
+struct GroupProblem {
+    TileArgs<T> t;
+    uint32_t tiles_n;
+    uint32_t flags;
+};
+
+template <class T, int WM, int WN>
+__global__ void __launch_bounds__(GT) k_gemm_grouped(const GroupProblem<T>* __restrict__ probs, int count,
+                                                     const uint32_t* __restrict__ prefix, uint32_t total, T alpha,
+                                                     T beta, int beta_zero) {
+    __shared__ MfmaSmem<T, WM, WN> sm;
+    for (uint32_t tile = blockIdx.x; tile < total; tile += gridDim.x) {
+        // problem of this tile: last g with prefix[g] <= tile
+        int lo = 0, hi = count - 1;
+        while (lo < hi) {
+            int mid = (lo + hi + 1) >> 1;
+            if (prefix[mid] <= tile) lo = mid;
+            else hi = mid - 1;
+        }
+        const GroupProblem<T> gp = probs[lo];
+        const uint32_t local = tile - prefix[lo];
+        const uint32_t tm = local / gp.tiles_n, tn = local - (local / gp.tiles_n) * gp.tiles_n;
+        switch (gp.flags & 3) {
+            case 3: mfma_tile<T, WM, WN, true, true>(gp.t, tm, tn, alpha, beta, beta_zero, sm); break;
+            case 1: mfma_tile<T, WM, WN, true, false>(gp.t, tm, tn, alpha, beta, beta_zero, sm); break;
+            case 2: mfma_tile<T, WM, WN, false, true>(gp.t, tm, tn, alpha, beta, beta_zero, sm); break;
+            default: mfma_tile<T, WM, WN, false, false>(gp.t, tm, tn, alpha, beta, beta_zero, sm); break;
+        }
+        __syncthreads();  // LDS reuse by the next tile
+    }
 }
 
 /* ---------------- SIMT fallback: f32, i32, i64, c64 ---------------- */
@@ -292,8 +366,131 @@ __global__ void __launch_bounds__(256) k_gemm_simt(GemmArgs<T> g) {
 
 /* ---------------- host side ---------------- */
 
+/* ---------------- tile configurations and the autotuner ---------------- */
+
+// MFMA tile configurations: (WM, WN) -> block tile (32*WM) x (32*WN).
+struct TileCfg {
+    int wm, wn;
+};
+static const TileCfg CFG_F64[] = {{4, 4}, {2, 4}, {2, 2}};
+static const TileCfg CFG_C128[] = {{2, 4}, {2, 2}};
+
+// The autotuner times every candidate configuration once per problem class
+// (sizes bucketed to powers of two, layout flags, batch bucket) on the real
+// call and keeps the fastest. Timing runs only when C is write-only (beta == 0)
+// and the stream is not being captured; otherwise the cached choice or the
+// size heuristic is used. RSTSR_HIP_TUNE_CACHE=<file> persists choices.
+static std::mutex g_tune_mu;
+static std::map<std::string, int>& tune_table() {
+    static std::map<std::string, int> t;
+    static bool loaded = false;
+    if (!loaded) {
+        loaded = true;
+        if (const char* f = std::getenv("RSTSR_HIP_TUNE_CACHE")) {
+            if (FILE* fp = std::fopen(f, "r")) {
+                char key[256];
+                int v;
+                while (std::fscanf(fp, "%255s %d", key, &v) == 2) t[key] = v;
+                std::fclose(fp);
+            }
+        }
+    }
+    return t;
+}
+
+static int bucket(int64_t x) {
+    int b = 0;
+    while ((1ll << b) < x) ++b;
+    return b;
+}
+
+static bool stream_capturing(rstsr_hip_ctx* ctx) {
+    hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+    HIP_CHECK(hipStreamIsCapturing(ctx->stream, &st));
+    return st != hipStreamCaptureStatusNone;
+}
+
+// run(c) performs the whole call with configuration c. Returns the chosen index.
+template <class F>
+static int tuned_run(rstsr_hip_ctx* ctx, const std::string& key, int ncand, int heuristic, bool can_time, F&& run) {
+    {
+        std::lock_guard<std::mutex> g(g_tune_mu);
+        auto& t = tune_table();
+        auto it = t.find(key);
+        if (it != t.end() && it->second < ncand) {
+            run(it->second);
+            return it->second;
+        }
+    }
+    if (!can_time || ncand == 1 || stream_capturing(ctx) || std::getenv("RSTSR_HIP_NO_TUNE")) {
+        run(heuristic);
+        return heuristic;
+    }
+    float best = 1e30f;
+    int bi = heuristic;
+    for (int c = 0; c < ncand; ++c) {
+        HIP_CHECK(hipEventRecord(ctx->events[14], ctx->stream));
+        run(c);
+        HIP_CHECK(hipEventRecord(ctx->events[15], ctx->stream));
+        HIP_CHECK(hipEventSynchronize(ctx->events[15]));
+        float ms = 0;
+        HIP_CHECK(hipEventElapsedTime(&ms, ctx->events[14], ctx->events[15]));
+        if (ms < best) {
+            best = ms;
+            bi = c;
+        }
+    }
+    // the last candidate's output is already in C; every configuration produces
+    // identical bits (the K order per element does not depend on the tile shape)
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    tune_table()[key] = bi;
+    if (const char* f = std::getenv("RSTSR_HIP_TUNE_CACHE")) {
+        if (FILE* fp = std::fopen(f, "a")) {
+            std::fprintf(fp, "%s %d\n", key.c_str(), bi);
+            std::fclose(fp);
+        }
+    }
+    return bi;
+}
+
+template <class T, int WM, int WN>
+static void launch_mfma_cfg(rstsr_hip_ctx* ctx, GemmArgs<T> g, uint32_t nb, bool amc, bool bkc, const int* run_if) {
+    constexpr int BM = 32 * WM, BN = 32 * WN;
+    g.tiles_m = ceil_div_u32(g.M, BM);
+    g.tiles_n = ceil_div_u32(g.N, BN);
+    uint64_t nt = (uint64_t)g.tiles_m * g.tiles_n;
+    if (nt >= (1ull << 31)) RSTSR_RAISE(UnImplemented, "GEMM grid too large");
+    dim3 grid((uint32_t)nt, nb);
+    if (amc && bkc) hipLaunchKernelGGL((k_gemm_mfma<T, WM, WN, true, true>), grid, dim3(GT), 0, ctx->stream, g, run_if);
+    else if (amc) hipLaunchKernelGGL((k_gemm_mfma<T, WM, WN, true, false>), grid, dim3(GT), 0, ctx->stream, g, run_if);
+    else if (bkc) hipLaunchKernelGGL((k_gemm_mfma<T, WM, WN, false, true>), grid, dim3(GT), 0, ctx->stream, g, run_if);
+    else hipLaunchKernelGGL((k_gemm_mfma<T, WM, WN, false, false>), grid, dim3(GT), 0, ctx->stream, g, run_if);
+    check_launch("gemm");
+}
+
 template <class T>
-static void launch_gemm_t(rstsr_hip_ctx* ctx, const GemmProblem& P, T alpha, T beta, bool beta_zero) {
+static void launch_mfma_any(rstsr_hip_ctx* ctx, const TileCfg& c, const GemmArgs<T>& g, uint32_t nb, bool amc,
+                            bool bkc, const int* run_if) {
+    if constexpr (std::is_same<T, double>::value) {
+        if (c.wm == 4 && c.wn == 4) return launch_mfma_cfg<T, 4, 4>(ctx, g, nb, amc, bkc, run_if);
+        if (c.wm == 2 && c.wn == 4) return launch_mfma_cfg<T, 2, 4>(ctx, g, nb, amc, bkc, run_if);
+        return launch_mfma_cfg<T, 2, 2>(ctx, g, nb, amc, bkc, run_if);
+    } else {
+        if (c.wm == 2 && c.wn == 4) return launch_mfma_cfg<T, 2, 4>(ctx, g, nb, amc, bkc, run_if);
+        return launch_mfma_cfg<T, 2, 2>(ctx, g, nb, amc, bkc, run_if);
+    }
+}
+
+template <class T>
+static int heuristic_cfg(int64_t m, int64_t n) {
+    int64_t s = std::min(m, n);
+    if constexpr (std::is_same<T, double>::value) return s >= 512 ? 0 : (s >= 128 ? 1 : 2);
+    else return s >= 256 ? 0 : 1;
+}
+
+template <class T>
+static void launch_gemm_t(rstsr_hip_ctx* ctx, const GemmProblem& P, T alpha, T beta, bool beta_zero,
+                          const int* run_if = nullptr) {
     GemmArgs<T> g;
     g.M = P.M;
     g.N = P.N;
@@ -313,37 +510,105 @@ static void launch_gemm_t(rstsr_hip_ctx* ctx, const GemmProblem& P, T alpha, T b
     g.bd = P.bd;
     const uint64_t YMAX = 65535;
     constexpr bool MF = std::is_same<T, double>::value || std::is_same<T, c128>::value;
-    for (uint64_t b0 = 0; b0 < P.nbatch; b0 += YMAX) {
-        uint32_t nb = (uint32_t)std::min<uint64_t>(YMAX, P.nbatch - b0);
-        g.batch0 = (uint32_t)b0;
-        if constexpr (MF) {
-            constexpr bool CX = std::is_same<T, c128>::value;
-            constexpr int WM = CX ? 2 : 4, WN = 4;
-            constexpr int BM = 32 * WM, BN = 32 * WN;
-            g.tiles_m = ceil_div_u32(P.M, BM);
-            g.tiles_n = ceil_div_u32(P.N, BN);
-            uint64_t nt = (uint64_t)g.tiles_m * g.tiles_n;
-            if (nt >= (1ull << 31)) RSTSR_RAISE(UnImplemented, "GEMM grid too large");
-            bool amc = std::llabs(P.rsa) == 1 && std::llabs(P.csa) != 1;
-            bool bkc = std::llabs(P.rsb) == 1 && std::llabs(P.csb) != 1;
-            dim3 grid((uint32_t)nt, nb);
-            if (amc && bkc) hipLaunchKernelGGL((k_gemm_mfma<T, WM, WN, true, true>), grid, dim3(GT), 0, ctx->stream, g);
-            else if (amc) hipLaunchKernelGGL((k_gemm_mfma<T, WM, WN, true, false>), grid, dim3(GT), 0, ctx->stream, g);
-            else if (bkc) hipLaunchKernelGGL((k_gemm_mfma<T, WM, WN, false, true>), grid, dim3(GT), 0, ctx->stream, g);
-            else hipLaunchKernelGGL((k_gemm_mfma<T, WM, WN, false, false>), grid, dim3(GT), 0, ctx->stream, g);
-        } else {
+    if constexpr (MF) {
+        constexpr bool CX = std::is_same<T, c128>::value;
+        const TileCfg* cands = CX ? CFG_C128 : CFG_F64;
+        const int ncand = CX ? 2 : 3;
+        bool amc = std::llabs(P.rsa) == 1 && std::llabs(P.csa) != 1;
+        bool bkc = std::llabs(P.rsb) == 1 && std::llabs(P.csb) != 1;
+        auto run = [&](int c) {
+            for (uint64_t b0 = 0; b0 < P.nbatch; b0 += YMAX) {
+                GemmArgs<T> gg = g;
+                gg.batch0 = (uint32_t)b0;
+                uint32_t nb = (uint32_t)std::min<uint64_t>(YMAX, P.nbatch - b0);
+                launch_mfma_any<T>(ctx, cands[c], gg, nb, amc, bkc, run_if);
+            }
+        };
+        char key[160];
+        std::snprintf(key, sizeof(key), "gemm:%s:m%d:n%d:k%d:b%d:f%d", CX ? "z" : "d", bucket(P.M), bucket(P.N),
+                      bucket(P.K), bucket((int64_t)P.nbatch), (amc ? 1 : 0) | (bkc ? 2 : 0));
+        tuned_run(ctx, key, ncand, heuristic_cfg<T>(P.M, P.N), beta_zero && run_if == nullptr, run);
+    } else {
+        for (uint64_t b0 = 0; b0 < P.nbatch; b0 += YMAX) {
+            uint32_t nb = (uint32_t)std::min<uint64_t>(YMAX, P.nbatch - b0);
+            g.batch0 = (uint32_t)b0;
             g.tiles_m = ceil_div_u32(P.M, 16);
             g.tiles_n = ceil_div_u32(P.N, 16);
             uint64_t nt = (uint64_t)g.tiles_m * g.tiles_n;
             if (nt >= (1ull << 31)) RSTSR_RAISE(UnImplemented, "GEMM grid too large");
             hipLaunchKernelGGL(k_gemm_simt<T>, dim3((uint32_t)nt, nb), dim3(256), 0, ctx->stream, g);
+            check_launch("gemm");
         }
-        check_launch("gemm");
     }
 }
 
-void plain_dgemm(rstsr_hip_ctx* ctx, const GemmProblem& P, double alpha, double beta, bool beta_zero) {
-    launch_gemm_t<double>(ctx, P, alpha, beta, beta_zero);
+void plain_dgemm(rstsr_hip_ctx* ctx, const GemmProblem& P, double alpha, double beta, bool beta_zero,
+                 const int* run_if) {
+    launch_gemm_t<double>(ctx, P, alpha, beta, beta_zero, run_if);
+}
+
+/* ---------------- grouped GEMM ---------------- */
+
+template <class T, int WM, int WN>
+static void launch_grouped_cfg(rstsr_hip_ctx* ctx, const std::vector<TileArgs<T>>& probs, T alpha, T beta,
+                               bool beta_zero) {
+    constexpr int BM = 32 * WM, BN = 32 * WN;
+    std::vector<GroupProblem<T>> gp(probs.size());
+    std::vector<uint32_t> prefix(probs.size() + 1, 0);
+    uint64_t total = 0;
+    for (size_t i = 0; i < probs.size(); ++i) {
+        const auto& t = probs[i];
+        uint32_t tm = ceil_div_u32(t.M, BM), tn = ceil_div_u32(t.N, BN);
+        gp[i].t = t;
+        gp[i].tiles_n = tn;
+        bool amc = std::llabs(t.rsa) == 1 && std::llabs(t.csa) != 1;
+        bool bkc = std::llabs(t.rsb) == 1 && std::llabs(t.csb) != 1;
+        gp[i].flags = (amc ? 1u : 0u) | (bkc ? 2u : 0u);
+        prefix[i] = (uint32_t)total;
+        total += (uint64_t)tm * tn;
+    }
+    prefix[probs.size()] = (uint32_t)total;
+    if (total == 0) return;
+    if (total >= (1ull << 31)) RSTSR_RAISE(UnImplemented, "grouped GEMM has too many tiles");
+    size_t bytes = gp.size() * sizeof(GroupProblem<T>) + prefix.size() * sizeof(uint32_t) + 256;
+    char* meta = (char*)aux_buffer(ctx, bytes);
+    HIP_CHECK(hipMemcpyAsync(meta, gp.data(), gp.size() * sizeof(GroupProblem<T>), hipMemcpyHostToDevice, ctx->stream));
+    size_t poff = (gp.size() * sizeof(GroupProblem<T>) + 255) & ~(size_t)255;
+    HIP_CHECK(hipMemcpyAsync(meta + poff, prefix.data(), prefix.size() * sizeof(uint32_t), hipMemcpyHostToDevice,
+                             ctx->stream));
+    // persistent grid: enough resident blocks to fill every CU twice
+    uint32_t grid = (uint32_t)std::min<uint64_t>(total, (uint64_t)std::max(ctx->num_cu, 1) * 2);
+    hipLaunchKernelGGL((k_gemm_grouped<T, WM, WN>), dim3(grid), dim3(GT), 0, ctx->stream,
+                       (const GroupProblem<T>*)meta, (int)gp.size(), (const uint32_t*)(meta + poff), (uint32_t)total,
+                       alpha, beta, beta_zero ? 1 : 0);
+    check_launch("grouped gemm");
+    // the host staging vectors die with this call: wait for the copies
+    HIP_CHECK(hipStreamSynchronize(ctx->stream));
+}
+
+template <class T>
+static void grouped_t(rstsr_hip_ctx* ctx, const std::vector<TileArgs<T>>& probs, T alpha, T beta, bool beta_zero) {
+    if (probs.empty()) return;
+    constexpr bool CX = std::is_same<T, c128>::value;
+    double mean = 0;
+    for (auto& p : probs) mean += (double)std::min(p.M, p.N);
+    mean /= (double)probs.size();
+    int64_t tot_mnk = 0;
+    for (auto& p : probs) tot_mnk += p.M * p.N * p.K;
+    auto run = [&](int c) {
+        if constexpr (CX) {
+            if (c == 0) launch_grouped_cfg<T, 2, 4>(ctx, probs, alpha, beta, beta_zero);
+            else launch_grouped_cfg<T, 2, 2>(ctx, probs, alpha, beta, beta_zero);
+        } else {
+            if (c == 0) launch_grouped_cfg<T, 4, 4>(ctx, probs, alpha, beta, beta_zero);
+            else if (c == 1) launch_grouped_cfg<T, 2, 4>(ctx, probs, alpha, beta, beta_zero);
+            else launch_grouped_cfg<T, 2, 2>(ctx, probs, alpha, beta, beta_zero);
+        }
+    };
+    char key[160];
+    std::snprintf(key, sizeof(key), "grouped:%s:n%d:mean%d:work%d", CX ? "z" : "d", bucket((int64_t)probs.size()),
+                  bucket((int64_t)mean), bucket(tot_mnk));
+    tuned_run(ctx, key, CX ? 2 : 3, heuristic_cfg<T>((int64_t)mean, (int64_t)mean), beta_zero, run);
 }
 
 // Zero-fill C (k == 0 with beta == 0: crates-device/rstsr-openblas/src/matmul_impl.rs:78-81).
@@ -392,14 +657,46 @@ static bool is_zero_scalar(rstsr_dtype_t t, const void* v) {
     }
 }
 
+// Complex split GEMM from four real split products on the real/imaginary
+// planes (stride-2 double views): Cr = a ArBr - a AiBi + b Cr, Ci = a ArBi + a AiBr + b Ci.
+static void ozaki_zgemm(rstsr_hip_ctx* ctx, const GemmProblem& P, c128 alpha, c128 beta, bool beta_zero) {
+    if (alpha.im != 0.0 || beta.im != 0.0)
+        RSTSR_RAISE(UnImplemented, "the complex split GEMM takes real alpha and beta");
+    auto part = [&](const void* base, int im) { return (const void*)((const double*)base + im); };
+    GemmProblem R = P;
+    R.rsa *= 2;
+    R.csa *= 2;
+    R.rsb *= 2;
+    R.csb *= 2;
+    R.rsc *= 2;
+    R.csc *= 2;
+    for (int d = 0; d < R.bd.nbd; ++d) {
+        R.bd.sa[d] *= 2;
+        R.bd.sb[d] *= 2;
+        R.bd.sc[d] *= 2;
+    }
+    const double a = alpha.re, b = beta.re;
+    auto prod = [&](int ai, int bi, int ci, double al, double be, bool bz) {
+        GemmProblem Q = R;
+        Q.a = part(P.a, ai);
+        Q.b = part(P.b, bi);
+        Q.c = (void*)((double*)P.c + ci);
+        ozaki_gemm_f64(ctx, Q, al, be, bz);
+    };
+    prod(0, 0, 0, a, b, beta_zero);   // Cr  = a Ar Br + b Cr
+    prod(1, 1, 0, -a, 1.0, false);    // Cr -= a Ai Bi
+    prod(0, 1, 1, a, b, beta_zero);   // Ci  = a Ar Bi + b Ci
+    prod(1, 0, 1, a, 1.0, false);     // Ci += a Ai Br
+}
+
 void run_gemm(rstsr_hip_ctx* ctx, rstsr_dtype_t t, rstsr_gemm_mode_t mode, const GemmProblem& P, const void* alpha,
               const void* beta) {
     if (!alpha || !beta) RSTSR_RAISE(InvalidValue, "null alpha/beta");
     if (P.M == 0 || P.N == 0 || P.nbatch == 0) return;
     bool bz = is_zero_scalar(t, beta);
     if (mode != RSTSR_GEMM_PLAIN && mode != RSTSR_GEMM_OZAKI_INT8) RSTSR_RAISE(InvalidValue, "unknown GEMM mode");
-    if (mode == RSTSR_GEMM_OZAKI_INT8 && t != RSTSR_F64)
-        RSTSR_RAISE(UnImplemented, "the Ozaki split GEMM is provided for f64");
+    if (mode == RSTSR_GEMM_OZAKI_INT8 && t != RSTSR_F64 && t != RSTSR_C128)
+        RSTSR_RAISE(UnImplemented, "the Ozaki split GEMM is provided for f64 and c128");
     auto body = [&](auto tag) {
         using T = decltype(tag);
         T al, be;
@@ -412,6 +709,12 @@ void run_gemm(rstsr_hip_ctx* ctx, rstsr_dtype_t t, rstsr_gemm_mode_t mode, const
         if constexpr (std::is_same<T, double>::value) {
             if (mode == RSTSR_GEMM_OZAKI_INT8) {
                 ozaki_gemm_f64(ctx, P, al, be, bz);
+                return;
+            }
+        }
+        if constexpr (std::is_same<T, c128>::value) {
+            if (mode == RSTSR_GEMM_OZAKI_INT8) {
+                ozaki_zgemm(ctx, P, al, be, bz);
                 return;
             }
         }
@@ -610,6 +913,64 @@ int rstsr_hip_matmul(rstsr_hip_ctx_t* ctx, rstsr_dtype_t t, void* c, const rstsr
         }
         std::lock_guard<std::mutex> g(ctx->mu);
         run_gemm(ctx, t, mode, P, alpha, beta);
+    });
+}
+
+int rstsr_hip_gemm_grouped(rstsr_hip_ctx_t* ctx, rstsr_dtype_t t, rstsr_gemm_mode_t mode, size_t count,
+                           const rstsr_gemm_desc_t* descs, const void* alpha, const void* beta) {
+    return guarded([&] {
+        bind(ctx);
+        if (count && !descs) RSTSR_RAISE(InvalidValue, "null descriptors");
+        if (!alpha || !beta) RSTSR_RAISE(InvalidValue, "null alpha/beta");
+        size_t es = dtype_size(t);
+        std::vector<GemmProblem> probs;
+        for (size_t i = 0; i < count; ++i) {
+            const auto& d = descs[i];
+            GemmProblem P{};
+            P.M = (int64_t)d.m;
+            P.N = (int64_t)d.n;
+            P.K = (int64_t)d.k;
+            P.a = d.a;
+            P.rsa = d.rsa;
+            P.csa = d.csa;
+            P.b = d.b;
+            P.rsb = d.rsb;
+            P.csb = d.csb;
+            P.c = d.c;
+            P.rsc = d.rsc;
+            P.csc = d.csc;
+            P.bd.nbd = 0;
+            P.nbatch = 1;
+            check_gemm_extent(P, es);
+            if (P.M && P.N) probs.push_back(P);
+        }
+        std::lock_guard<std::mutex> g(ctx->mu);
+        bool plain_mfma = mode == RSTSR_GEMM_PLAIN && (t == RSTSR_F64 || t == RSTSR_C128);
+        std::vector<GemmProblem> rest;
+        if (plain_mfma) {
+            auto go = [&](auto tag) {
+                using T = decltype(tag);
+                T al, be;
+                std::memcpy(&al, alpha, sizeof(T));
+                std::memcpy(&be, beta, sizeof(T));
+                std::vector<TileArgs<T>> ta;
+                for (auto& P : probs) {
+                    if (P.K == 0) {
+                        rest.push_back(P);
+                        continue;
+                    }
+                    ta.push_back(TileArgs<T>{P.M, P.N, P.K, (const T*)P.a, P.rsa, P.csa, (const T*)P.b, P.rsb, P.csb,
+                                             (T*)P.c, P.rsc, P.csc});
+                }
+                grouped_t<T>(ctx, ta, al, be, is_zero_scalar(t, beta));
+            };
+            if (t == RSTSR_F64) go(double{});
+            else go(c128{});
+        } else {
+            rest = probs;
+        }
+        // split-precision and non-MFMA dtypes: one call per problem, in order
+        for (auto& P : rest) run_gemm(ctx, t, mode, P, alpha, beta);
     });
 }
 

@@ -43,8 +43,10 @@ inline void batch_offsets_host(const BatchDesc& bd, uint64_t bi, int64_t& oa, in
 // Ozaki-II split GEMM for f64 (ozaki.hip). Falls back to the plain kernel for
 // non-finite inputs.
 void ozaki_gemm_f64(rstsr_hip_ctx* ctx, const GemmProblem& P, double alpha, double beta, bool beta_zero);
-// Plain FP64 MFMA DGEMM of one problem (gemm.hip).
-void plain_dgemm(rstsr_hip_ctx* ctx, const GemmProblem& P, double alpha, double beta, bool beta_zero);
+// Plain FP64 MFMA DGEMM of one problem (gemm.hip). run_if (device, nullable):
+// the kernel exits unless *run_if != 0.
+void plain_dgemm(rstsr_hip_ctx* ctx, const GemmProblem& P, double alpha, double beta, bool beta_zero,
+                 const int* run_if = nullptr);
 
 }  // namespace hip
 }  // namespace rstsr

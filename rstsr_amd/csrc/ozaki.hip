@@ -176,7 +176,9 @@ __device__ __forceinline__ int sym_residue(double x, double m, double inv) {
 // reads follow the source's unit-stride axis and plane writes follow t.
 __global__ void __launch_bounds__(256) k_split(const double* __restrict__ s, int64_t nv, int64_t len, int64_t sv,
                                                int64_t st, const int* __restrict__ ev, int64_t nvp, int64_t lenp,
-                                               int8_t* __restrict__ planes, ModSet ms, int read_along_t) {
+                                               int8_t* __restrict__ planes, ModSet ms, int read_along_t,
+                                               const int* __restrict__ skip) {
+    if (*skip) return;  // non-finite input: the plain kernel computes this product
     __shared__ double tile[64][65];
     const int64_t v0 = (int64_t)blockIdx.y * 64, t0 = (int64_t)blockIdx.x * 64;
     const int tid = threadIdx.x;
@@ -237,7 +239,9 @@ __device__ __forceinline__ uint32_t xcd_remap8(uint32_t t, uint32_t ntiles) {
 template <bool ROW4G>
 __global__ void __launch_bounds__(256) k_i8gemm_mod(const int8_t* __restrict__ A, const int8_t* __restrict__ B,
                                                     uint8_t* __restrict__ R, int64_t M, int64_t N, int64_t Kp,
-                                                    int64_t ldr, int mod, uint32_t tiles_n, uint32_t ntiles) {
+                                                    int64_t ldr, int mod, uint32_t tiles_n, uint32_t ntiles,
+                                                    const int* __restrict__ skip) {
+    if (*skip) return;
     __shared__ __attribute__((aligned(16))) int8_t As[2][I8_BM][I8_BK + I8_PAD];
     __shared__ __attribute__((aligned(16))) int8_t Bs[2][I8_BN][I8_BK + I8_PAD];
     const uint32_t tile = xcd_remap8(blockIdx.x, ntiles);
@@ -319,6 +323,12 @@ __global__ void __launch_bounds__(256) k_i8gemm_mod(const int8_t* __restrict__ A
             }
 }
 
+// zero n ints (a kernel rather than a memset node, so the sequence can be graph-captured)
+__global__ void k_zero_i32(int* p, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
 /* ---------------- CRT reconstruction ---------------- */
 
 __device__ __forceinline__ double u128_to_double(u128 x) {
@@ -328,9 +338,10 @@ __device__ __forceinline__ double u128_to_double(u128 x) {
 
 __global__ void __launch_bounds__(256) k_crt(const uint8_t* __restrict__ R, int64_t M, int64_t N, CrtConst cc,
                                              const int* __restrict__ eA, const int* __restrict__ fB, double* C,
-                                             int64_t rsc, int64_t csc, double alpha, double beta, int beta_zero) {
+                                             int64_t rsc, int64_t csc, double alpha, double beta, int beta_zero,
+                                             const int* __restrict__ skip) {
     const int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (idx >= M * N) return;
+    if (idx >= M * N || *skip) return;
     const int64_t i = idx / N, j = idx - (idx / N) * N;
     const size_t plane = (size_t)M * N;
     u128 X = 0;
@@ -473,9 +484,11 @@ void ozaki_gemm_f64(rstsr_hip_ctx* ctx, const GemmProblem& P, double alpha, doub
         const double* A = (const double*)P.a + oa;
         const double* B = (const double*)P.b + ob;
         double* C = (double*)P.c + oc;
-        HIP_CHECK(hipMemsetAsync(flag, 0, sizeof(int), ctx->stream));
-        HIP_CHECK(hipMemsetAsync(eA, 0, Mp * sizeof(int), ctx->stream));
-        HIP_CHECK(hipMemsetAsync(fB, 0, Np * sizeof(int), ctx->stream));
+        {
+            int64_t nz = (int64_t)((o_fB + Np * sizeof(int) - o_flag) / sizeof(int));  // flag, eA, fB are adjacent
+            hipLaunchKernelGGL(k_zero_i32, dim3(ceil_div_u32(nz, 256)), dim3(256), 0, ctx->stream, flag, nz);
+            check_launch("ozaki zero");
+        }
         // vector norms: rows of A (elements along k: stride csa), columns of B (along k: stride rsb)
         auto vnorm = [&](const double* x, int64_t nvec, int64_t len, int64_t sv, int64_t st, int* ev) {
             if (std::llabs(st) == 1 || std::llabs(sv) > std::llabs(st)) {
@@ -494,40 +507,37 @@ void ozaki_gemm_f64(rstsr_hip_ctx* ctx, const GemmProblem& P, double alpha, doub
         };
         vnorm(A, M, K, P.rsa, P.csa, eA);
         vnorm(B, N, K, P.csb, P.rsb, fB);
-        // non-finite inputs: the plain kernel handles IEEE special values
-        int hflag = 0;
-        HIP_CHECK(hipMemcpyAsync(ctx->host_pinned, flag, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
-        HIP_CHECK(hipStreamSynchronize(ctx->stream));
-        std::memcpy(&hflag, ctx->host_pinned, sizeof(int));
-        if (hflag) {
+        // Non-finite inputs set *flag: the split kernels below then exit at once
+        // and the plain FP64 kernel (which runs only when *flag != 0) computes C
+        // with IEEE special values. No host round trip, so this is capturable.
+        {
             GemmProblem Q = P;
             Q.a = A;
             Q.b = B;
             Q.c = C;
             Q.bd.nbd = 0;
             Q.nbatch = 1;
-            plain_dgemm(ctx, Q, alpha, beta, beta_zero);
-            continue;
+            plain_dgemm(ctx, Q, alpha, beta, beta_zero, flag);
         }
         // residue planes (A rows along k with stride csa; B columns along k with stride rsb)
         {
             dim3 ga((uint32_t)(Kp / 64), (uint32_t)(Mp / 64));
             hipLaunchKernelGGL(k_split, ga, dim3(256), 0, ctx->stream, A, M, K, P.rsa, P.csa, eA, Mp, Kp, A8, ms,
-                               std::llabs(P.csa) == 1 ? 1 : 0);
+                               std::llabs(P.csa) == 1 ? 1 : 0, flag);
             dim3 gb((uint32_t)(Kp / 64), (uint32_t)(Np / 64));
             hipLaunchKernelGGL(k_split, gb, dim3(256), 0, ctx->stream, B, N, K, P.csb, P.rsb, fB, Np, Kp, B8, ms,
-                               std::llabs(P.rsb) == 1 ? 1 : 0);
+                               std::llabs(P.rsb) == 1 ? 1 : 0, flag);
             check_launch("ozaki split");
         }
         for (int l = 0; l < n; ++l) {
             hipLaunchKernelGGL(k_i8gemm_mod<I8_ROW4G>, dim3(ntiles), dim3(256), 0, ctx->stream,
                                A8 + (size_t)l * Mp * Kp, B8 + (size_t)l * Np * Kp, R + (size_t)l * M * N, M, N, Kp,
-                               N, (int)cc.mod[l], tiles_n, ntiles);
+                               N, (int)cc.mod[l], tiles_n, ntiles, flag);
             check_launch("ozaki int8 gemm");
         }
         uint64_t mn = (uint64_t)M * N;
         hipLaunchKernelGGL(k_crt, dim3(ceil_div_u32(mn, 256)), dim3(256), 0, ctx->stream, R, M, N, cc, eA, fB, C, P.rsc,
-                           P.csc, alpha, beta, beta_zero ? 1 : 0);
+                           P.csc, alpha, beta, beta_zero ? 1 : 0, flag);
         check_launch("ozaki crt");
     }
 }

@@ -211,3 +211,94 @@ def test_ozaki_nonfinite_falls_back(dev):
     B = np.ones((30, 10))
     got = rt.matmul(rt.asarray(A, device=dev), rt.asarray(B, device=dev), mode="ozaki").to_numpy()
     assert np.all(np.isinf(got[3])) and np.all(got[:3] == 30.0)
+
+
+# ---------------- grouped (uneven) GEMM, complex split, graph capture ----------------
+
+@pytest.mark.parametrize("dtype,mode", [(np.float64, "plain"), (np.complex128, "plain"), (np.float64, "ozaki")])
+def test_grouped_uneven(dev, oracle, dtype, mode):
+    rng = np.random.default_rng(11)
+    probs, hosts = [], []
+    for g in range(40):
+        m, n, k = (int(x) for x in rng.integers(1, 40, size=3) * 8)
+        A = rng.normal(size=(m, k)) if g % 3 else rng.normal(size=(k, m)).T
+        B = rng.normal(size=(k, n)) if g % 2 else rng.normal(size=(n, k)).T
+        if dtype == np.complex128:
+            A = A + 1j * rng.normal(size=A.shape)
+            B = B + 1j * rng.normal(size=B.shape)
+        ta, _ = from_np(dev, A)
+        tb, _ = from_np(dev, B)
+        tc = rt.empty((m, n), dtype, device=dev)
+        probs.append((ta, tb, tc))
+        hosts.append((A, B))
+    rt.gemm_grouped(probs, mode=mode)
+    for (A, B), (_, _, tc) in zip(hosts, probs):
+        got = tc.to_numpy()
+        if dtype == np.complex128:
+            ref = A.astype(np.clongdouble) @ B.astype(np.clongdouble)
+            scale = np.abs(A) @ np.abs(B)
+            assert np.all(np.abs(got - ref) <= 2 * A.shape[1] * 2.0 ** -53 * scale)
+        else:
+            ref = oracle.gemm_dot2(np.ascontiguousarray(A), np.ascontiguousarray(B))
+            if mode == "plain":
+                ok, worst = comp_bound_ok(got, A, B, ref, 2.0 ** -53)
+                assert ok, worst
+            else:
+                assert np.max(np.abs(got - ref)) <= 1e-14 * np.max(np.abs(ref))
+
+
+def test_grouped_alpha_beta(dev, oracle):
+    rng = np.random.default_rng(12)
+    A = rng.normal(size=(24, 40))
+    B = rng.normal(size=(40, 16))
+    C0 = rng.normal(size=(24, 16))
+    tc = rt.asarray(C0, device=dev)
+    rt.gemm_grouped([(rt.asarray(A, device=dev), rt.asarray(B, device=dev), tc)], alpha=2.0, beta=-1.0)
+    ref = 2.0 * oracle.gemm_dot2(A, B) - C0
+    assert np.max(np.abs(tc.to_numpy() - ref)) < 1e-13 * np.max(np.abs(ref))
+
+
+@pytest.mark.parametrize("shape", [(64, 96, 80), (300, 130, 257)])
+def test_zgemm_ozaki(dev, shape):
+    m, k, n = shape
+    rng = np.random.default_rng(m)
+    A = wide_exponent(rng, (m, k)) + 1j * wide_exponent(rng, (m, k))
+    B = wide_exponent(rng, (k, n)) + 1j * wide_exponent(rng, (k, n))
+    got = rt.matmul(rt.asarray(A, device=dev), rt.asarray(B, device=dev), mode="ozaki").to_numpy()
+    ref = A.astype(np.clongdouble) @ B.astype(np.clongdouble)
+    err = float(np.max(np.abs(got - ref)) / np.max(np.abs(ref)))
+    assert err <= 1e-14, err
+
+
+@pytest.mark.parametrize("mode", ["plain", "ozaki"])
+def test_graph_capture_chain(dev, mode):
+    """add -> mul -> matmul -> sum_axes captured once, replayed 3 times with eager work between
+    replays; every replay must equal the eager sequence bit for bit."""
+    rng = np.random.default_rng(13)
+    n = 192
+    z1h = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
+    z2h = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
+    vh = rng.normal(size=n) + 1j * rng.normal(size=n)
+    z2 = rt.asarray(z2h, device=dev)
+    v = rt.asarray(vh, device=dev)
+
+    def eager():
+        z1 = rt.asarray(z1h, device=dev)
+        z1 += v
+        z3 = z1 * z2
+        z4 = rt.matmul(z3, z2, mode=mode)
+        return z4.sum_axes([1]).to_numpy()
+
+    ref = eager()
+    z1 = rt.asarray(z1h, device=dev)
+    with rt.capture(dev) as g:
+        z1 += v
+        z3 = z1 * z2
+        z4 = rt.matmul(z3, z2, mode=mode)
+        s = z4.sum_axes([1])
+    for rep in range(3):
+        z1.assign(rt.asarray(z1h, device=dev))  # eager reset between replays
+        _ = eager()                              # unrelated eager work between replays
+        g.launch()
+        dev.synchronize()
+        assert np.array_equal(s.to_numpy(), ref), rep
