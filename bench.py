@@ -464,10 +464,10 @@ def chain_leg(dev, rt, orc, modes, n=4096, steps=20):
         with rt.capture(dev) as g:
             for _ in range(steps):
                 Z1 += V
-                Z3 = Z1 * Z2
-                Z4 = rt.matmul(Z3, Z2, mode=mode)
-                s = Z4.sum_axes([1])
-                keep.append((Z3, Z4, s))
+                z3 = Z1 * Z2
+                z4 = rt.matmul(z3, Z2, mode=mode)
+                keep.append((z3, z4, z4.sum_axes([1])))
+            del z3, z4
         Z1.assign(rt.asarray(z1h, device=dev))
         dev.synchronize()
         t0 = time.perf_counter()

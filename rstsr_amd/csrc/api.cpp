@@ -152,6 +152,7 @@ int rstsr_hip_ctx_destroy(rstsr_hip_ctx_t* ctx) {
         if (ctx->ws.ptr) HIP_CHECK(hipFree(ctx->ws.ptr));
         if (ctx->aux.ptr) HIP_CHECK(hipFree(ctx->aux.ptr));
         for (void* p : ctx->retired) HIP_CHECK(hipFree(p));
+        for (void* p : ctx->deferred) HIP_CHECK(hipFree(p));
         for (auto& ev : ctx->events)
             if (ev) HIP_CHECK(hipEventDestroy(ev));
         if (ctx->host_pinned) HIP_CHECK(hipHostFree(ctx->host_pinned));
@@ -200,9 +201,18 @@ int rstsr_hip_free(rstsr_hip_ctx_t* ctx, void* ptr) {
         if (!ptr) return;
         bind(ctx);
         if (!registry_remove(ptr)) RSTSR_RAISE(InvalidValue, "free of a pointer not returned by rstsr_hip_malloc");
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        HIP_CHECK(hipStreamIsCapturing(ctx->stream, &cs));
+        if (cs != hipStreamCaptureStatusNone) {
+            // a capturing stream cannot be synchronised: release after capture ends
+            ctx->deferred.push_back(ptr);
+            return;
+        }
         // frees are stream ordered with respect to this context's queued work
         HIP_CHECK(hipStreamSynchronize(ctx->stream));
         HIP_CHECK(hipFree(ptr));
+        for (void* p : ctx->deferred) HIP_CHECK(hipFree(p));
+        ctx->deferred.clear();
     });
 }
 

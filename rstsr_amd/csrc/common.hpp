@@ -69,6 +69,7 @@ struct rstsr_hip_ctx {
     rstsr::hip::Workspace ws;   // partials, residue planes, scalar results
     rstsr::hip::Workspace aux;  // small metadata (grouped GEMM descriptors)
     std::vector<void*> retired;  // buffers replaced while a stream was being captured
+    std::vector<void*> deferred;  // caller buffers freed while the stream was being captured
     void* host_pinned = nullptr;  // 4 KiB pinned staging for scalar results
     hipEvent_t events[16] = {};
 };
@@ -150,11 +151,6 @@ inline void check_launch(const char* what) {
 }
 
 inline uint32_t ceil_div_u32(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
-
-// Verify that every element of a layout lies inside [0, bytes/elem) of a buffer
-// whose extent the caller does not tell us; only the sign is checkable here,
-// the upper bound is the caller's contract (as for the reference's Vec<T>).
-inline void check_layout_nonnegative(const Layout& l) { (void)l.bounds_index(); }
 
 }  // namespace hip
 }  // namespace rstsr

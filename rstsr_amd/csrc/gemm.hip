@@ -50,20 +50,6 @@ struct GemmArgs {
     BatchDesc bd;
 };
 
-__device__ __forceinline__ void batch_offsets(const BatchDesc& bd, uint32_t bi, int64_t& oa, int64_t& ob, int64_t& oc) {
-    oa = ob = oc = 0;
-#pragma unroll
-    for (int d = 0; d < MAXD; ++d) {
-        if (d >= bd.nbd) break;
-        uint32_t q = fdiv(bi, bd.fd[d]);
-        uint32_t r = bi - q * bd.shape[d];
-        oa += (int64_t)r * bd.sa[d];
-        ob += (int64_t)r * bd.sb[d];
-        oc += (int64_t)r * bd.sc[d];
-        bi = q;
-    }
-}
-
 template <class T>
 struct CplxOf : std::false_type {};
 template <>
@@ -261,7 +247,7 @@ __device__ __forceinline__ void mfma_tile(const TileArgs<T>& g, uint32_t tm, uin
 // run_if (nullable): the whole grid exits unless *run_if != 0 (device-side
 // selection between alternative paths without a host round trip).
 template <class T, int WM, int WN, bool A_MC, bool B_KC>
-__global__ void __launch_bounds__(GT) k_gemm_mfma(GemmArgs<T> g, const int* run_if) {
+__global__ void __launch_bounds__(GT, 2) k_gemm_mfma(GemmArgs<T> g, const int* run_if) {
     if (run_if && *run_if == 0) return;
     __shared__ MfmaSmem<T, WM, WN> sm;
     const uint32_t ntiles = g.tiles_m * g.tiles_n;
@@ -284,7 +270,7 @@ struct GroupProblem {
 };
 
 template <class T, int WM, int WN>
-__global__ void __launch_bounds__(GT) k_gemm_grouped(const GroupProblem<T>* __restrict__ probs, int count,
+__global__ void __launch_bounds__(GT, 2) k_gemm_grouped(const GroupProblem<T>* __restrict__ probs, int count,
                                                      const uint32_t* __restrict__ prefix, uint32_t total, T alpha,
                                                      T beta, int beta_zero) {
     __shared__ MfmaSmem<T, WM, WN> sm;

@@ -28,6 +28,21 @@ struct GemmProblem {
     uint64_t nbatch = 1;
 };
 
+// Device offsets of batch `bi` (innermost batch axis first).
+__device__ __forceinline__ void batch_offsets(const BatchDesc& bd, uint32_t bi, int64_t& oa, int64_t& ob, int64_t& oc) {
+    oa = ob = oc = 0;
+#pragma unroll
+    for (int d = 0; d < MAXD; ++d) {
+        if (d >= bd.nbd) break;
+        uint32_t q = fdiv(bi, bd.fd[d]);
+        uint32_t r = bi - q * bd.shape[d];
+        oa += (int64_t)r * bd.sa[d];
+        ob += (int64_t)r * bd.sb[d];
+        oc += (int64_t)r * bd.sc[d];
+        bi = q;
+    }
+}
+
 // Host offsets of batch `bi` (outermost-last unravel of bd).
 inline void batch_offsets_host(const BatchDesc& bd, uint64_t bi, int64_t& oa, int64_t& ob, int64_t& oc) {
     oa = ob = oc = 0;

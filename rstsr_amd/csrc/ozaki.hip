@@ -177,7 +177,7 @@ __device__ __forceinline__ int sym_residue(double x, double m, double inv) {
 __global__ void __launch_bounds__(256) k_split(const double* __restrict__ s, int64_t nv, int64_t len, int64_t sv,
                                                int64_t st, const int* __restrict__ ev, int64_t nvp, int64_t lenp,
                                                int8_t* __restrict__ planes, ModSet ms, int read_along_t,
-                                               const int* __restrict__ skip) {
+                                               const int* __restrict__ skip, int64_t plane) {
     if (*skip) return;  // non-finite input: the plain kernel computes this product
     __shared__ double tile[64][65];
     const int64_t v0 = (int64_t)blockIdx.y * 64, t0 = (int64_t)blockIdx.x * 64;
@@ -207,7 +207,6 @@ __global__ void __launch_bounds__(256) k_split(const double* __restrict__ s, int
     const int tq = (tid & 3) * 16;  // 4 threads x 16 t = 64 t per row
     const int64_t gv = v0 + vv;
     if (gv >= nvp) return;
-    const size_t plane = (size_t)nvp * lenp;
 #pragma unroll
     for (int c = 0; c < 16; c += 4) {
         int64_t gt = t0 + tq + c;
@@ -236,12 +235,17 @@ __device__ __forceinline__ uint32_t xcd_remap8(uint32_t t, uint32_t ntiles) {
 // A8: [Mp][Kp], B8: [Np][Kp] (both K contiguous, zero padded); R: [M][ldr] uint8
 // residues of (A8 B8^T) mod `mod`. 256 threads = 4 waves (2x2), 64x64 per wave
 // from 4x4 v_mfma_i32_16x16x64_i8 tiles; LDS double buffer, one barrier per slice.
+// Batches on blockIdx.y: plane offsets sA / sB (0 for a broadcast operand) and sR.
 template <bool ROW4G>
 __global__ void __launch_bounds__(256) k_i8gemm_mod(const int8_t* __restrict__ A, const int8_t* __restrict__ B,
                                                     uint8_t* __restrict__ R, int64_t M, int64_t N, int64_t Kp,
                                                     int64_t ldr, int mod, uint32_t tiles_n, uint32_t ntiles,
+                                                    int64_t sA, int64_t sB, int64_t sR,
                                                     const int* __restrict__ skip) {
     if (*skip) return;
+    A += (int64_t)blockIdx.y * sA;
+    B += (int64_t)blockIdx.y * sB;
+    R += (int64_t)blockIdx.y * sR;
     __shared__ __attribute__((aligned(16))) int8_t As[2][I8_BM][I8_BK + I8_PAD];
     __shared__ __attribute__((aligned(16))) int8_t Bs[2][I8_BN][I8_BK + I8_PAD];
     const uint32_t tile = xcd_remap8(blockIdx.x, ntiles);
@@ -336,19 +340,24 @@ __device__ __forceinline__ double u128_to_double(u128 x) {
     return ldexp((double)hi, 64) + (double)lo;
 }
 
-__global__ void __launch_bounds__(256) k_crt(const uint8_t* __restrict__ R, int64_t M, int64_t N, CrtConst cc,
-                                             const int* __restrict__ eA, const int* __restrict__ fB, double* C,
-                                             int64_t rsc, int64_t csc, double alpha, double beta, int beta_zero,
-                                             const int* __restrict__ skip) {
-    const int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (idx >= M * N || *skip) return;
-    const int64_t i = idx / N, j = idx - (idx / N) * N;
-    const size_t plane = (size_t)M * N;
+// R planes: [l][batch][Mp][N] (plane stride pl); eA [nbA][Mp], fB [nbB][Np].
+__global__ void __launch_bounds__(256) k_crt(const uint8_t* __restrict__ R, int64_t M, int64_t N, int64_t Mp,
+                                             int64_t Np, uint64_t nbatch, int64_t pl, int a_batched, int b_batched,
+                                             BatchDesc bd, CrtConst cc, const int* __restrict__ eA,
+                                             const int* __restrict__ fB, double* C, int64_t rsc, int64_t csc,
+                                             double alpha, double beta, int beta_zero, const int* __restrict__ skip) {
+    const uint64_t idx = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    const uint64_t mn = (uint64_t)M * N;
+    if (idx >= mn * nbatch || *skip) return;
+    const uint64_t b = idx / mn;
+    const uint64_t e = idx - b * mn;
+    const int64_t i = (int64_t)(e / N), j = (int64_t)(e - (e / N) * N);
+    const uint64_t r_off = (b * Mp + i) * N + j;
     u128 X = 0;
 #pragma unroll
     for (int l = 0; l < OZ_MAXMOD; ++l) {
         if (l >= cc.n) break;
-        X += (u128)R[l * plane + idx] * cc.w[l];
+        X += (u128)R[l * pl + r_off] * cc.w[l];
     }
     // X < n * 256 * M < 2^128: reduce mod M with a floating quotient estimate
     double qd = floor(u128_to_double(X) / cc.Md);
@@ -360,8 +369,10 @@ __global__ void __launch_bounds__(256) k_crt(const uint8_t* __restrict__ R, int6
     if (Y >= (i128)cc.M) Y -= (i128)cc.M;
     if (Y >= (i128)cc.M - Y) Y -= (i128)cc.M;  // symmetric range (-M/2, M/2]
     double v = Y < 0 ? -u128_to_double((u128)(-Y)) : u128_to_double((u128)Y);
-    v = ldexp(v, -(eA[i] + fB[j]));
-    double* cp = C + i * rsc + j * csc;
+    v = ldexp(v, -(eA[(a_batched ? b : 0) * Mp + i] + fB[(b_batched ? b : 0) * Np + j]));
+    int64_t oa, ob, oc;
+    batch_offsets(bd, (uint32_t)b, oa, ob, oc);
+    double* cp = C + oc + i * rsc + j * csc;
     double out = alpha * v;
     if (!beta_zero) out += beta * (*cp);
     *cp = out;
@@ -452,21 +463,37 @@ void ozaki_gemm_f64(rstsr_hip_ctx* ctx, const GemmProblem& P, double alpha, doub
     const int64_t Kp = (K + I8_BK - 1) / I8_BK * I8_BK;
     const int64_t nchunk = std::max<int64_t>(1, std::min<int64_t>(64, K / 256));
     const int64_t chunk = (K + nchunk - 1) / nchunk;
-    // workspace layout
+    const uint64_t nb = P.nbatch;
+    if (nb >= 65536) RSTSR_RAISE(UnImplemented, "too many batches for the split GEMM");
+    // an operand whose batch strides are all zero is split once and shared by every batch
+    bool a_b = false, b_b = false;
+    for (int d = 0; d < P.bd.nbd; ++d) {
+        if (P.bd.sa[d] != 0) a_b = true;
+        if (P.bd.sb[d] != 0) b_b = true;
+    }
+    const uint64_t nbA = a_b ? nb : 1, nbB = b_b ? nb : 1;
+    // workspace layout: flag | eA [nbA][Mp] | fB [nbB][Np] | norm partials | A8 [l][nbA][Mp][Kp] |
+    // B8 [l][nbB][Np][Kp] | R [l][nb][Mp][N]
     size_t off = 0;
     auto take = [&](size_t bytes) {
         size_t o = off;
         off += (bytes + 255) & ~(size_t)255;
         return o;
     };
-    const size_t o_flag = take(256);
-    const size_t o_eA = take(Mp * sizeof(int));
-    const size_t o_fB = take(Np * sizeof(int));
+    const size_t o_flag = 0;
+    off = sizeof(int);
+    const size_t o_eA = off;
+    off += nbA * Mp * sizeof(int);
+    const size_t o_fB = off;
+    off += nbB * Np * sizeof(int);
+    const size_t n_zero = off / sizeof(int);
+    off = (off + 255) & ~(size_t)255;
     const size_t o_pm = take((size_t)nchunk * std::max(Mp, Np) * sizeof(double));
     const size_t o_ps = take((size_t)nchunk * std::max(Mp, Np) * sizeof(double));
-    const size_t o_A8 = take((size_t)n * Mp * Kp);
-    const size_t o_B8 = take((size_t)n * Np * Kp);
-    const size_t o_R = take((size_t)n * M * N);
+    const size_t planeA = (size_t)nbA * Mp * Kp, planeB = (size_t)nbB * Np * Kp, planeR = (size_t)nb * Mp * N;
+    const size_t o_A8 = take((size_t)n * planeA);
+    const size_t o_B8 = take((size_t)n * planeB);
+    const size_t o_R = take((size_t)n * planeR);
     char* ws = (char*)workspace(ctx, off);
     int* flag = (int*)(ws + o_flag);
     int* eA = (int*)(ws + o_eA);
@@ -478,68 +505,60 @@ void ozaki_gemm_f64(rstsr_hip_ctx* ctx, const GemmProblem& P, double alpha, doub
     uint8_t* R = (uint8_t*)(ws + o_R);
     const uint32_t tiles_n = (uint32_t)(Np / I8_BN), ntiles = (uint32_t)((Mp / I8_BM) * tiles_n);
 
-    for (uint64_t bi = 0; bi < P.nbatch; ++bi) {
+    hipLaunchKernelGGL(k_zero_i32, dim3(ceil_div_u32(n_zero, 256)), dim3(256), 0, ctx->stream, flag, (int64_t)n_zero);
+    check_launch("ozaki zero");
+    // vector norms: rows of A (elements along k: stride csa), columns of B (along k: stride rsb)
+    auto vnorm = [&](const double* x, int64_t nvec, int64_t len, int64_t sv, int64_t st, int* ev) {
+        if (std::llabs(st) == 1 || std::llabs(sv) > std::llabs(st)) {
+            hipLaunchKernelGGL(k_vnorm_block, dim3((uint32_t)nvec), dim3(256), 0, ctx->stream, x, nvec, len, sv, st, g,
+                               ev, flag);
+        } else {
+            dim3 grid(ceil_div_u32(nvec, 256), (uint32_t)nchunk);
+            hipLaunchKernelGGL(k_vnorm_cols_max, grid, dim3(256), 0, ctx->stream, x, nvec, len, sv, st, chunk, pm, flag);
+            hipLaunchKernelGGL(k_vnorm_cols_ss, grid, dim3(256), 0, ctx->stream, x, nvec, len, sv, st, chunk, nchunk, pm,
+                               ps);
+            hipLaunchKernelGGL(k_vnorm_cols_fin, dim3(ceil_div_u32(nvec, 256)), dim3(256), 0, ctx->stream, nvec, nchunk,
+                               pm, ps, g, ev);
+        }
+        check_launch("ozaki norms");
+    };
+    for (uint64_t b = 0; b < nbA; ++b) {
         int64_t oa, ob, oc;
-        batch_offsets_host(P.bd, bi, oa, ob, oc);
+        batch_offsets_host(P.bd, b, oa, ob, oc);
         const double* A = (const double*)P.a + oa;
-        const double* B = (const double*)P.b + ob;
-        double* C = (double*)P.c + oc;
-        {
-            int64_t nz = (int64_t)((o_fB + Np * sizeof(int) - o_flag) / sizeof(int));  // flag, eA, fB are adjacent
-            hipLaunchKernelGGL(k_zero_i32, dim3(ceil_div_u32(nz, 256)), dim3(256), 0, ctx->stream, flag, nz);
-            check_launch("ozaki zero");
-        }
-        // vector norms: rows of A (elements along k: stride csa), columns of B (along k: stride rsb)
-        auto vnorm = [&](const double* x, int64_t nvec, int64_t len, int64_t sv, int64_t st, int* ev) {
-            if (std::llabs(st) == 1 || std::llabs(sv) > std::llabs(st)) {
-                hipLaunchKernelGGL(k_vnorm_block, dim3((uint32_t)nvec), dim3(256), 0, ctx->stream, x, nvec, len, sv, st,
-                                   g, ev, flag);
-            } else {
-                dim3 grid(ceil_div_u32(nvec, 256), (uint32_t)nchunk);
-                hipLaunchKernelGGL(k_vnorm_cols_max, grid, dim3(256), 0, ctx->stream, x, nvec, len, sv, st, chunk, pm,
-                                   flag);
-                hipLaunchKernelGGL(k_vnorm_cols_ss, grid, dim3(256), 0, ctx->stream, x, nvec, len, sv, st, chunk, nchunk,
-                                   pm, ps);
-                hipLaunchKernelGGL(k_vnorm_cols_fin, dim3(ceil_div_u32(nvec, 256)), dim3(256), 0, ctx->stream, nvec,
-                                   nchunk, pm, ps, g, ev);
-            }
-            check_launch("ozaki norms");
-        };
-        vnorm(A, M, K, P.rsa, P.csa, eA);
-        vnorm(B, N, K, P.csb, P.rsb, fB);
-        // Non-finite inputs set *flag: the split kernels below then exit at once
-        // and the plain FP64 kernel (which runs only when *flag != 0) computes C
-        // with IEEE special values. No host round trip, so this is capturable.
-        {
-            GemmProblem Q = P;
-            Q.a = A;
-            Q.b = B;
-            Q.c = C;
-            Q.bd.nbd = 0;
-            Q.nbatch = 1;
-            plain_dgemm(ctx, Q, alpha, beta, beta_zero, flag);
-        }
-        // residue planes (A rows along k with stride csa; B columns along k with stride rsb)
-        {
-            dim3 ga((uint32_t)(Kp / 64), (uint32_t)(Mp / 64));
-            hipLaunchKernelGGL(k_split, ga, dim3(256), 0, ctx->stream, A, M, K, P.rsa, P.csa, eA, Mp, Kp, A8, ms,
-                               std::llabs(P.csa) == 1 ? 1 : 0, flag);
-            dim3 gb((uint32_t)(Kp / 64), (uint32_t)(Np / 64));
-            hipLaunchKernelGGL(k_split, gb, dim3(256), 0, ctx->stream, B, N, K, P.csb, P.rsb, fB, Np, Kp, B8, ms,
-                               std::llabs(P.rsb) == 1 ? 1 : 0, flag);
-            check_launch("ozaki split");
-        }
-        for (int l = 0; l < n; ++l) {
-            hipLaunchKernelGGL(k_i8gemm_mod<I8_ROW4G>, dim3(ntiles), dim3(256), 0, ctx->stream,
-                               A8 + (size_t)l * Mp * Kp, B8 + (size_t)l * Np * Kp, R + (size_t)l * M * N, M, N, Kp,
-                               N, (int)cc.mod[l], tiles_n, ntiles, flag);
-            check_launch("ozaki int8 gemm");
-        }
-        uint64_t mn = (uint64_t)M * N;
-        hipLaunchKernelGGL(k_crt, dim3(ceil_div_u32(mn, 256)), dim3(256), 0, ctx->stream, R, M, N, cc, eA, fB, C, P.rsc,
-                           P.csc, alpha, beta, beta_zero ? 1 : 0, flag);
-        check_launch("ozaki crt");
+        vnorm(A, M, K, P.rsa, P.csa, eA + b * Mp);
+        dim3 ga((uint32_t)(Kp / 64), (uint32_t)(Mp / 64));
+        hipLaunchKernelGGL(k_split, ga, dim3(256), 0, ctx->stream, A, M, K, P.rsa, P.csa, eA + b * Mp, Mp, Kp,
+                           A8 + b * Mp * Kp, ms, std::llabs(P.csa) == 1 ? 1 : 0, flag, (int64_t)planeA);
+        check_launch("ozaki split A");
     }
+    for (uint64_t b = 0; b < nbB; ++b) {
+        int64_t oa, ob, oc;
+        batch_offsets_host(P.bd, b, oa, ob, oc);
+        const double* B = (const double*)P.b + ob;
+        vnorm(B, N, K, P.csb, P.rsb, fB + b * Np);
+        dim3 gb((uint32_t)(Kp / 64), (uint32_t)(Np / 64));
+        hipLaunchKernelGGL(k_split, gb, dim3(256), 0, ctx->stream, B, N, K, P.csb, P.rsb, fB + b * Np, Np, Kp,
+                           B8 + b * Np * Kp, ms, std::llabs(P.rsb) == 1 ? 1 : 0, flag, (int64_t)planeB);
+        check_launch("ozaki split B");
+    }
+    // Non-finite inputs set *flag: the split kernels exit at once and the plain
+    // FP64 kernel (which runs only when *flag != 0) computes C with IEEE special
+    // values. No host round trip, so the sequence can be graph-captured.
+    plain_dgemm(ctx, P, alpha, beta, beta_zero, flag);
+    for (int l = 0; l < n; ++l) {
+        hipLaunchKernelGGL(k_i8gemm_mod<I8_ROW4G>, dim3(ntiles, (uint32_t)nb), dim3(256), 0, ctx->stream,
+                           A8 + (size_t)l * planeA, B8 + (size_t)l * planeB, R + (size_t)l * planeR, Mp, N, Kp, N,
+                           (int)cc.mod[l], tiles_n, ntiles, a_b ? Mp * Kp : (int64_t)0, b_b ? Np * Kp : (int64_t)0,
+                           Mp * N, flag);
+        check_launch("ozaki int8 gemm");
+    }
+    uint64_t tot = (uint64_t)M * N * nb;
+    if (tot >= (1ull << 40)) RSTSR_RAISE(UnImplemented, "split GEMM output too large");
+    hipLaunchKernelGGL(k_crt, dim3(ceil_div_u32(tot, 256)), dim3(256), 0, ctx->stream, R, M, N, Mp, Np, nb,
+                       (int64_t)planeR, a_b ? 1 : 0, b_b ? 1 : 0, P.bd, cc, eA, fB, (double*)P.c, P.rsc, P.csc, alpha,
+                       beta, beta_zero ? 1 : 0, flag);
+    check_launch("ozaki crt");
 }
 
 }  // namespace hip

@@ -281,17 +281,29 @@ __global__ void __launch_bounds__(RB) k_red_row(const T* __restrict__ a, typenam
     }
 }
 
-template <class A, class T>
+// VEC consecutive elements loaded as one aligned access
+template <class T, int VEC>
+struct alignas(sizeof(T) * VEC) Pack {
+    T v[VEC];
+};
+
+// VEC adjacent outputs per thread along the unit-stride kept axis (VEC * sizeof(T)
+// = 16 B vector loads when the host proved alignment), sequential over the reduced range.
+template <class A, class T, int VEC>
 __global__ void __launch_bounds__(RB) k_red_col(const T* __restrict__ a, typename A::Out* out,
                                                 typename A::S* part, RedPlan p) {
     using S = typename A::S;
+    using vt = Pack<T, VEC>;
     const uint64_t t = (uint64_t)blockIdx.x * RB + threadIdx.x;
-    if (t >= (uint64_t)p.M * p.nsplit) return;
-    const uint32_t sp = (uint32_t)(t / p.M);
-    const uint32_t o = (uint32_t)(t - (uint64_t)sp * p.M);
+    const uint32_t MV = p.M / VEC;
+    if (t >= (uint64_t)MV * p.nsplit) return;
+    const uint32_t sp = (uint32_t)(t / MV);
+    const uint32_t o = (uint32_t)(t - (uint64_t)sp * MV) * VEC;
     int64_t oin, oout;
     kept_offsets(p, o, oin, oout);
-    S acc = A::init();
+    S acc[VEC];
+#pragma unroll
+    for (int v = 0; v < VEC; ++v) acc[v] = A::init();
     const uint64_t chunk = p.upw;  // reduced elements per split
     uint64_t r0 = (uint64_t)sp * chunk;
     uint64_t r1 = min(p.R, r0 + chunk);
@@ -322,16 +334,23 @@ __global__ void __launch_bounds__(RB) k_red_col(const T* __restrict__ a, typenam
             const T* rowp = base + off;
             uint64_t j = 0;
             for (; j + 4 <= n; j += 4) {
-                T x0 = rowp[(int64_t)j * s0];
-                T x1 = rowp[(int64_t)(j + 1) * s0];
-                T x2 = rowp[(int64_t)(j + 2) * s0];
-                T x3 = rowp[(int64_t)(j + 3) * s0];
-                A::add(acc, x0, li + j * l0);
-                A::add(acc, x1, li + (j + 1) * l0);
-                A::add(acc, x2, li + (j + 2) * l0);
-                A::add(acc, x3, li + (j + 3) * l0);
+                vt x0 = *(const vt*)(rowp + (int64_t)j * s0);
+                vt x1 = *(const vt*)(rowp + (int64_t)(j + 1) * s0);
+                vt x2 = *(const vt*)(rowp + (int64_t)(j + 2) * s0);
+                vt x3 = *(const vt*)(rowp + (int64_t)(j + 3) * s0);
+#pragma unroll
+                for (int v = 0; v < VEC; ++v) {
+                    A::add(acc[v], x0.v[v], li + j * l0);
+                    A::add(acc[v], x1.v[v], li + (j + 1) * l0);
+                    A::add(acc[v], x2.v[v], li + (j + 2) * l0);
+                    A::add(acc[v], x3.v[v], li + (j + 3) * l0);
+                }
             }
-            for (; j < n; ++j) A::add(acc, rowp[(int64_t)j * s0], li + j * l0);
+            for (; j < n; ++j) {
+                vt x = *(const vt*)(rowp + (int64_t)j * s0);
+#pragma unroll
+                for (int v = 0; v < VEC; ++v) A::add(acc[v], x.v[v], li + j * l0);
+            }
             remaining -= n;
             if (remaining == 0) break;
             // carry into dims >= 1
@@ -351,8 +370,12 @@ __global__ void __launch_bounds__(RB) k_red_col(const T* __restrict__ a, typenam
             }
         }
     }
-    if (p.nsplit == 1) out[oout] = A::fin(acc, a[oin]);
-    else part[(uint64_t)o * p.nsplit + sp] = acc;
+#pragma unroll
+    for (int v = 0; v < VEC; ++v) {
+        // kept axis 0 has input stride +1 when VEC > 1
+        if (p.nsplit == 1) out[oout + v * p.kout[0]] = A::fin(acc[v], a[oin + v]);
+        else part[(uint64_t)(o + v) * p.nsplit + sp] = acc[v];
+    }
 }
 
 // One wave per output: lane l combines partials l, l+64, ... in order, then a
@@ -522,9 +545,21 @@ static void run_reduce(rstsr_hip_ctx* ctx, const RedSetup& s, const T* a, typena
         uint32_t nb = ceil_div_u32(nwaves, RB / 64);
         hipLaunchKernelGGL((k_red_row<A, T>), dim3(nb), dim3(RB), 0, ctx->stream, a, out, part, p);
     } else {
-        uint64_t nthr = (uint64_t)p.M * p.nsplit;
-        uint32_t nb = ceil_div_u32(nthr, RB);
-        hipLaunchKernelGGL((k_red_col<A, T>), dim3(nb), dim3(RB), 0, ctx->stream, a, out, part, p);
+        // 16-byte loads when every address a thread touches is 16-byte aligned
+        constexpr int V16 = (std::is_arithmetic<T>::value && (sizeof(T) == 8 || sizeof(T) == 4)) ? (int)(16 / sizeof(T)) : 1;
+        bool vec = V16 > 1 && p.nk >= 1 && p.kin[0] == 1 && p.kshape[0] % V16 == 0 && p.M % V16 == 0 &&
+                   (p.in_off % V16) == 0 && ((uintptr_t)a % 16) == 0;
+        for (int d = 1; d < p.nk && vec; ++d) vec = (p.kin[d] % V16) == 0;
+        for (int d = 0; d < p.nr && vec; ++d) vec = (p.rin[d] % V16) == 0;
+        if (vec) {
+            uint64_t nthr = (uint64_t)(p.M / V16) * p.nsplit;
+            hipLaunchKernelGGL((k_red_col<A, T, V16>), dim3(ceil_div_u32(nthr, RB)), dim3(RB), 0, ctx->stream, a, out,
+                               part, p);
+        } else {
+            uint64_t nthr = (uint64_t)p.M * p.nsplit;
+            hipLaunchKernelGGL((k_red_col<A, T, 1>), dim3(ceil_div_u32(nthr, RB)), dim3(RB), 0, ctx->stream, a, out,
+                               part, p);
+        }
     }
     check_launch("reduce");
     if (p.nsplit > 1) {

@@ -302,3 +302,41 @@ def test_graph_capture_chain(dev, mode):
         g.launch()
         dev.synchronize()
         assert np.array_equal(s.to_numpy(), ref), rep
+
+
+@pytest.mark.parametrize("sa,sb", [((3, 40, 60), (60, 30)), ((40, 60), (3, 60, 30)), ((3, 40, 60), (3, 60, 30)),
+                                   ((2, 1, 24, 36), (4, 36, 20))])
+def test_ozaki_batch_rules(dev, oracle, sa, sb):
+    """Rules 5/6/7 in one batched split launch (a broadcast operand is split once)."""
+    rng = np.random.default_rng(len(sa) * 7 + len(sb))
+    A = wide_exponent(rng, sa)
+    B = wide_exponent(rng, sb)
+    got = rt.matmul(rt.asarray(A, device=dev), rt.asarray(B, device=dev), mode="ozaki").to_numpy()
+    ref = np.matmul(A.astype(np.longdouble), B.astype(np.longdouble))
+    assert got.shape == ref.shape
+    assert np.max(np.abs(got - ref)) <= 1e-14 * np.max(np.abs(ref))
+
+
+def test_ri_contraction_small(dev, oracle):
+    """(ia|jb) = sum_P B[P,i,a] B[P,j,b] as one rule-6 matmul of permuted views, plain and split."""
+    rng = np.random.default_rng(21)
+    nP, no, nv = 96, 6, 40
+    Bh = rng.normal(0.0, 0.1, size=(nP, no, nv))
+    tB = rt.asarray(Bh, device=dev)
+    lhs = tB.permute([1, 2, 0])              # [i, a, P], a contiguous
+    rhs = tB.reshape([nP, no * nv])          # [P, (j b)]
+    ref = np.einsum("pia,pjb->iajb", Bh.astype(np.longdouble), Bh.astype(np.longdouble)).reshape(no, nv, no * nv)
+    for mode in ("plain", "ozaki"):
+        got = rt.matmul(lhs, rhs, mode=mode).to_numpy()
+        assert np.max(np.abs(got - ref)) <= 1e-14 * np.max(np.abs(ref)), mode
+
+
+def test_free_during_capture_is_deferred(dev):
+    """Dropping a pre-capture tensor inside a capture must not break the capture."""
+    a = rt.asarray(np.arange(64.0).reshape(8, 8), device=dev)
+    tmp = rt.asarray(np.ones((8, 8)), device=dev)
+    with rt.capture(dev) as g:
+        tmp = a * a  # rebinding drops the old tmp while the stream is capturing
+    g.launch()
+    dev.synchronize()
+    assert np.array_equal(tmp.to_numpy(), np.arange(64.0).reshape(8, 8) ** 2)
