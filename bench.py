@@ -236,8 +236,17 @@ def main():
     dom = max(step_ops, key=lambda o: per_op[o.name]["ms"]) if step_ops else ops[0]
     dms = per_op[dom.name]["ms"]
     achieved = dom.nbytes / 1e9 / (dms / 1e3)
+    # HBM bytes per launch measured by rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs, scaled by
+    # the same counters on a contiguous copy of known size (profiles/pmc_traffic.json)
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", "pmc_traffic.json")
+    if os.path.exists(tf):
+        with open(tf) as f:
+            t = json.load(f).get("ops", {}).get(dom.name)
+        if t:
+            traffic = t["hbm_bytes_per_launch"]
     roofline = {"bound": "hbm", "kernel": dom.name, "achieved": round(achieved, 1), "peak": HBM_PEAK,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 4), "traffic": None,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 4), "traffic": traffic,
                 "algorithmic_bytes_per_launch": dom.nbytes, "avg_launch_ms": dms}
 
     # ---------------- FP64 GEMM leg ----------------

@@ -258,23 +258,24 @@ __global__ void __launch_bounds__(EW_BLOCK) k_ew_rows(Body body, EwPlan p, uint3
     }
 }
 
-// tiled transpose: dims 0 (= dc, output unit axis) and DI (input unit axis) form
-// a TILE x TILE tile; remaining dims are unravelled from the block index.
-// Operands flagged in STAGE (bit0 = A, bit1 = B) are read along DI into LDS.
-// Each thread keeps one tile column (phase 1) / one tile row position (phase 2)
-// fixed, so per-element addressing is a constant stride added to a per-thread
-// base; interior tiles skip the bounds tests.
-template <class Body, int TILE, int STAGE>
+// tiled transpose: dims 0 (= dc, output unit axis) and DI (input unit axis)
+// form a TI (along DI) x TC (along dc) tile; remaining dims are unravelled from
+// the block index. Operands flagged in STAGE (bit0 = A, bit1 = B) are read
+// along DI into LDS. Each thread keeps one tile column (phase 1) / one tile
+// row position (phase 2) fixed, so per-element addressing is a constant stride
+// added to a per-thread base; interior tiles skip the bounds tests; an operand
+// that is broadcast along DI is loaded once per thread in phase 2.
+template <class Body, int TI, int TC, int STAGE>
 __global__ void __launch_bounds__(EW_BLOCK) k_ew_tiled(Body body, EwPlan p, int di, uint32_t nti, uint32_t ntc,
                                                        FastDiv fti, FastDiv ftc) {
     using TA = typename Body::TA;
     using TB = typename Body::TB;
     constexpr bool SA = (STAGE & 1) != 0;
     constexpr bool SB = (STAGE & 2) != 0;
-    constexpr int ROWS = EW_BLOCK / TILE;  // tile rows covered per pass
-    constexpr int PER = TILE / ROWS;       // passes
-    __shared__ TA lds_a[SA ? TILE : 1][SA ? TILE + 1 : 1];
-    __shared__ TB lds_b[SB ? TILE : 1][SB ? TILE + 1 : 1];
+    constexpr int ROWS1 = EW_BLOCK / TI, PER1 = TC / ROWS1;  // phase 1: TI lanes along DI
+    constexpr int ROWS2 = EW_BLOCK / TC, PER2 = TI / ROWS2;  // phase 2: TC lanes along dc
+    __shared__ TA lds_a[SA ? TC : 1][SA ? TI + 1 : 1];
+    __shared__ TB lds_b[SB ? TC : 1][SB ? TI + 1 : 1];
 
     uint32_t b = blockIdx.x;
     uint32_t q1 = fdiv(b, ftc);
@@ -298,57 +299,68 @@ __global__ void __launch_bounds__(EW_BLOCK) k_ew_tiled(Body body, EwPlan p, int 
         }
     }
     const uint32_t Nc = p.shape[0], Ni = p.shape[di];
-    const uint32_t c0 = tc * TILE, i0 = ti * TILE;
+    const uint32_t c0 = tc * TC, i0 = ti * TI;
     const int64_t sc_c = p.st[0][0], sc_i = p.st[0][di];
     const int64_t sa_c = p.st[1][0], sa_i = p.st[1][di];
     const int64_t sb_c = p.st[2][0], sb_i = p.st[2][di];
-    const bool full = (i0 + TILE <= Ni) && (c0 + TILE <= Nc);
+    const bool full = (i0 + TI <= Ni) && (c0 + TC <= Nc);
     const int tid = threadIdx.x;
 
     if constexpr (SA || SB) {
-        // phase 1: ii = tid % TILE fixed, cc = k*ROWS + tid/TILE -> coalesced along the input axis
-        const int ii = tid % TILE, cb = tid / TILE;
-        const int64_t pa = oa + (int64_t)(i0 + ii) * sa_i + (int64_t)(c0 + cb) * sa_c, da = (int64_t)ROWS * sa_c;
-        const int64_t pb = ob + (int64_t)(i0 + ii) * sb_i + (int64_t)(c0 + cb) * sb_c, db = (int64_t)ROWS * sb_c;
+        // phase 1: ii = tid % TI fixed, cc = k*ROWS1 + tid/TI -> coalesced along the input axis
+        const int ii = tid % TI, cb = tid / TI;
+        const int64_t pa = oa + (int64_t)(i0 + ii) * sa_i + (int64_t)(c0 + cb) * sa_c, da = (int64_t)ROWS1 * sa_c;
+        const int64_t pb = ob + (int64_t)(i0 + ii) * sb_i + (int64_t)(c0 + cb) * sb_c, db = (int64_t)ROWS1 * sb_c;
         if (full) {
 #pragma unroll
-            for (int k = 0; k < PER; ++k) {
-                if constexpr (SA) lds_a[k * ROWS + cb][ii] = body.la(pa + k * da);
-                if constexpr (SB) lds_b[k * ROWS + cb][ii] = body.lb(pb + k * db);
+            for (int k = 0; k < PER1; ++k) {
+                if constexpr (SA) lds_a[k * ROWS1 + cb][ii] = body.la(pa + k * da);
+                if constexpr (SB) lds_b[k * ROWS1 + cb][ii] = body.lb(pb + k * db);
             }
         } else {
 #pragma unroll
-            for (int k = 0; k < PER; ++k) {
-                if (i0 + ii < Ni && c0 + k * ROWS + cb < Nc) {
-                    if constexpr (SA) lds_a[k * ROWS + cb][ii] = body.la(pa + k * da);
-                    if constexpr (SB) lds_b[k * ROWS + cb][ii] = body.lb(pb + k * db);
+            for (int k = 0; k < PER1; ++k) {
+                if (i0 + ii < Ni && c0 + k * ROWS1 + cb < Nc) {
+                    if constexpr (SA) lds_a[k * ROWS1 + cb][ii] = body.la(pa + k * da);
+                    if constexpr (SB) lds_b[k * ROWS1 + cb][ii] = body.lb(pb + k * db);
                 }
             }
         }
         __syncthreads();
     }
-    // phase 2: cc = tid % TILE fixed, ii = k*ROWS + tid/TILE -> coalesced along the output axis
-    const int cc = tid % TILE, ib = tid / TILE;
-    const int64_t pc = oc + (int64_t)(c0 + cc) * sc_c + (int64_t)(i0 + ib) * sc_i, dc2 = (int64_t)ROWS * sc_i;
-    const int64_t qa = oa + (int64_t)(c0 + cc) * sa_c + (int64_t)(i0 + ib) * sa_i, da2 = (int64_t)ROWS * sa_i;
-    const int64_t qb = ob + (int64_t)(c0 + cc) * sb_c + (int64_t)(i0 + ib) * sb_i, db2 = (int64_t)ROWS * sb_i;
-    auto one = [&](int k) {
-        TA x;
-        TB y;
-        if constexpr (SA) x = lds_a[cc][k * ROWS + ib];
-        else x = body.la(qa + k * da2);
-        if constexpr (SB) y = lds_b[cc][k * ROWS + ib];
-        else y = body.lb(qb + k * db2);
-        body.st(pc + k * dc2, x, y);
+    // phase 2: cc = tid % TC fixed, ii = k*ROWS2 + tid/TC -> coalesced along the output axis
+    const int cc = tid % TC, ib = tid / TC;
+    const int64_t pc = oc + (int64_t)(c0 + cc) * sc_c + (int64_t)(i0 + ib) * sc_i, dc2 = (int64_t)ROWS2 * sc_i;
+    const int64_t qa = oa + (int64_t)(c0 + cc) * sa_c + (int64_t)(i0 + ib) * sa_i, da2 = (int64_t)ROWS2 * sa_i;
+    const int64_t qb = ob + (int64_t)(c0 + cc) * sb_c + (int64_t)(i0 + ib) * sb_i, db2 = (int64_t)ROWS2 * sb_i;
+    const bool col_ok = c0 + cc < Nc;
+    auto run = [&](auto hoist_b) {
+        constexpr bool HB = decltype(hoist_b)::value;
+        TB yb{};
+        if constexpr (HB) {
+            if (col_ok) yb = body.lb(qb);
+        }
+        auto one = [&](int k) {
+            TA x;
+            TB y;
+            if constexpr (SA) x = lds_a[cc][k * ROWS2 + ib];
+            else x = body.la(qa + k * da2);
+            if constexpr (SB) y = lds_b[cc][k * ROWS2 + ib];
+            else if constexpr (HB) y = yb;
+            else y = body.lb(qb + k * db2);
+            body.st(pc + k * dc2, x, y);
+        };
+        if (full) {
+#pragma unroll
+            for (int k = 0; k < PER2; ++k) one(k);
+        } else {
+#pragma unroll
+            for (int k = 0; k < PER2; ++k)
+                if (col_ok && i0 + k * ROWS2 + ib < Ni) one(k);
+        }
     };
-    if (full) {
-#pragma unroll
-        for (int k = 0; k < PER; ++k) one(k);
-    } else {
-#pragma unroll
-        for (int k = 0; k < PER; ++k)
-            if (c0 + cc < Nc && i0 + k * ROWS + ib < Ni) one(k);
-    }
+    if (!SB && sb_i == 0) run(std::true_type{});
+    else run(std::false_type{});
 }
 
 /* =============================== host planning =============================== */
@@ -499,23 +511,27 @@ static void launch_ew(rstsr_hip_ctx* ctx, const Body& body, Dims d, const EwChoi
             hipLaunchKernelGGL(k_ew_rows<Body>, dim3((uint32_t)nb), dim3(EW_BLOCK), 0, ctx->stream, body, p, nchunk,
                                FastDiv(nchunk));
         } else {
-            constexpr int TILE = (sizeof(typename Body::TA) > 8 || sizeof(typename Body::TB) > 8) ? 32 : 64;
-            uint32_t ntc = ceil_div_u32(dd.shape[0], TILE);
-            uint32_t nti = ceil_div_u32(dd.shape[ch.di], TILE);
+            // tiles: 64x64 for 8-byte elements, 128 (input axis) x 64 for 4-byte and smaller, 32x32 for 16-byte
+            constexpr size_t ES = sizeof(typename Body::TA) > sizeof(typename Body::TB) ? sizeof(typename Body::TA)
+                                                                                          : sizeof(typename Body::TB);
+            constexpr int TI = ES > 8 ? 32 : (ES > 4 ? 64 : 128);
+            constexpr int TC = ES > 8 ? 32 : 64;
+            uint32_t ntc = ceil_div_u32(dd.shape[0], TC);
+            uint32_t nti = ceil_div_u32(dd.shape[ch.di], TI);
             uint64_t nouter = total / (dd.shape[0] * dd.shape[ch.di]);
             uint64_t nb = nouter * nti * ntc;
             if (nb >= LIM) RSTSR_RAISE(UnImplemented, "grid too large");
             switch (ch.stage) {
                 case 1:
-                    hipLaunchKernelGGL((k_ew_tiled<Body, TILE, 1>), dim3((uint32_t)nb), dim3(EW_BLOCK), 0, ctx->stream,
+                    hipLaunchKernelGGL((k_ew_tiled<Body, TI, TC, 1>), dim3((uint32_t)nb), dim3(EW_BLOCK), 0, ctx->stream,
                                        body, p, ch.di, nti, ntc, FastDiv(nti), FastDiv(ntc));
                     break;
                 case 2:
-                    hipLaunchKernelGGL((k_ew_tiled<Body, TILE, 2>), dim3((uint32_t)nb), dim3(EW_BLOCK), 0, ctx->stream,
+                    hipLaunchKernelGGL((k_ew_tiled<Body, TI, TC, 2>), dim3((uint32_t)nb), dim3(EW_BLOCK), 0, ctx->stream,
                                        body, p, ch.di, nti, ntc, FastDiv(nti), FastDiv(ntc));
                     break;
                 default:
-                    hipLaunchKernelGGL((k_ew_tiled<Body, TILE, 3>), dim3((uint32_t)nb), dim3(EW_BLOCK), 0, ctx->stream,
+                    hipLaunchKernelGGL((k_ew_tiled<Body, TI, TC, 3>), dim3((uint32_t)nb), dim3(EW_BLOCK), 0, ctx->stream,
                                        body, p, ch.di, nti, ntc, FastDiv(nti), FastDiv(ntc));
                     break;
             }
