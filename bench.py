@@ -50,6 +50,8 @@ def parse():
     p.add_argument("--only", default=None, help="run only this op (for profiling)")
     p.add_argument("--gemm-n", type=int, default=8192)
     p.add_argument("--gemm-modes", default="plain,ozaki")
+    p.add_argument("--gemm-only", action="store_true", help="skip the strided suite (profiling the GEMM legs)")
+    p.add_argument("--gemm-legs", default="dense,batch,ri,grouped,chain")
     return p.parse_args()
 
 
@@ -71,6 +73,9 @@ def main():
 
     L = abi.lib()
     dev = rt.DeviceHIP(0)
+    if args.gemm_only:
+        print(json.dumps({"gemm": gemm_leg(args, dev, rt)}), flush=True)
+        return
     ctx = C.c_void_p(dev._ctx_ptr())
     ev = abi.Events(dev._ctx_ptr())
 
@@ -309,6 +314,7 @@ def gemm_leg(args, dev, rt):
     def wide(rng, shape):
         return rng.standard_normal(shape) * np.exp2(rng.integers(-8, 9, size=shape))
 
+    legs = set(args.gemm_legs.split(","))
     # ---- square n^3 with wide exponents ----
     n = args.gemm_n
     rng = np.random.default_rng(20261204)
@@ -320,7 +326,7 @@ def gemm_leg(args, dev, rt):
     rows = np.array([0, n // 3, (2 * n) // 3, n - 1])
     ref = orc.gemm_dot2(A[rows], B, nthreads=0)
     res = {"shape": [n, n, n], "flop": 2.0 * n ** 3}
-    for mode in modes:
+    for mode in (modes if "dense" in legs else []):
         holder = {}
 
         def run():
@@ -343,7 +349,7 @@ def gemm_leg(args, dev, rt):
     tA3, tB2 = rt.asarray(A3, device=dev), rt.asarray(B2, device=dev)
     ref = orc.gemm_dot2(A3[7, :4], B2)
     res = {"shape": [64, 1024, 1024, 1024], "flop": 64 * 2.0 * 1024 ** 3}
-    for mode in modes:
+    for mode in (modes if "batch" in legs else []):
         holder = {}
 
         def run():
@@ -367,7 +373,7 @@ def gemm_leg(args, dev, rt):
     Ai = np.ascontiguousarray(Bri[:, 5, :].T[:4])  # rows a=0..3 of C_5
     ref = orc.gemm_dot2(Ai, Bri.reshape(1600, 24000))
     res = {"shape": [60, 400, 24000, 1600], "flop": 60 * 2.0 * 400 * 24000 * 1600}
-    for mode in modes:
+    for mode in (modes if "ri" in legs else []):
         holder = {}
 
         def run():
@@ -408,7 +414,7 @@ def gemm_leg(args, dev, rt):
         Ah = pool[offs[g, 0]:offs[g, 0] + m * k].reshape(m, k)
         Bh = pool[offs[g, 1]:offs[g, 1] + k * n].reshape(k, n)
         refs[g] = orc.gemm_dot2(Ah, Bh)
-    for mode in modes:
+    for mode in (modes if "grouped" in legs else []):
         ms = _timed(ev, lambda: rt.gemm_grouped(probs, mode=mode), 3)
         err = max(float(np.max(np.abs(probs[g][2].to_numpy() - refs[g])) / np.max(np.abs(refs[g]))) for g in check)
         tf = flop / (ms / 1e3) / 1e12
@@ -418,7 +424,8 @@ def gemm_leg(args, dev, rt):
     del probs, tpool, tC
 
     # ---- hipGraph-captured c128 chain, 20 steps ----
-    out["graph_chain_c128"] = chain_leg(dev, rt, orc, modes)
+    if "chain" in legs:
+        out["graph_chain_c128"] = chain_leg(dev, rt, orc, modes)
     return out
 
 
