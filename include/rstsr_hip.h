@@ -177,10 +177,12 @@ typedef struct {
     void* c;
     ptrdiff_t rsc, csc;
 } rstsr_gemm_desc_t;
-/* Grouped GEMM: C_g = alpha A_g B_g + beta C_g for count uneven problems. Plain f64/c128 run as
- * ONE persistent launch whose tile shape the autotuner picks; the split mode and other dtypes
- * run one call per problem. No trait in the reference takes a list of problems; the closest is
- * DeviceFaer's batched loop over matmul tasks (device_faer/matmul.rs:151-199). */
+/* Grouped GEMM: C_g = alpha A_g B_g + beta C_g for count uneven problems whose outputs do not
+ * overlap. Plain f64/c128 run as ONE persistent launch whose tile shape the autotuner picks;
+ * the f64 split mode runs each of its stages (norms, residue split, one INT8 launch over all
+ * moduli, CRT) as one launch over the whole group; other dtypes run one call per problem. No
+ * trait in the reference takes a list of problems; the closest is DeviceFaer's batched loop
+ * over matmul tasks (device_faer/matmul.rs:151-199). */
 int rstsr_hip_gemm_grouped(rstsr_hip_ctx_t* ctx, rstsr_dtype_t t, rstsr_gemm_mode_t mode, size_t count,
                            const rstsr_gemm_desc_t* descs, const void* alpha, const void* beta);
 

@@ -57,6 +57,23 @@ void check_extent(const void* ptr, const Layout& l, size_t elem, const char* nam
         RSTSR_RAISE(ValueOutOfRange, std::string("layout of operand ") + name + " reaches outside its buffer");
 }
 
+void check_extent_rel(const void* ptr, size_t nd, const size_t* shape, const int64_t* stride, size_t elem,
+                      const char* name) {
+    int64_t lo = 0, hi = 0;
+    for (size_t d = 0; d < nd; ++d) {
+        if (shape[d] == 0) return;
+        const int64_t span = (int64_t)(shape[d] - 1) * stride[d];
+        (span < 0 ? lo : hi) += span;
+    }
+    if (!ptr) RSTSR_RAISE(InvalidValue, std::string("null device pointer for operand ") + name);
+    auto a = registry_find(ptr);
+    if (a.second == 0)
+        RSTSR_RAISE(InvalidValue, std::string("operand ") + name + " is not inside a buffer from rstsr_hip_malloc");
+    const intptr_t p0 = (intptr_t)ptr;
+    if (p0 + lo * (intptr_t)elem < (intptr_t)a.first || p0 + (hi + 1) * (intptr_t)elem > (intptr_t)(a.first + a.second))
+        RSTSR_RAISE(ValueOutOfRange, std::string("strides of operand ") + name + " reach outside its buffer");
+}
+
 void check_extent_bytes(const void* ptr, size_t bytes, const char* name) {
     if (bytes == 0) return;
     auto a = registry_find(ptr);

@@ -272,7 +272,8 @@ struct GroupProblem {
 template <class T, int WM, int WN>
 __global__ void __launch_bounds__(GT, 2) k_gemm_grouped(const GroupProblem<T>* __restrict__ probs, int count,
                                                      const uint32_t* __restrict__ prefix, uint32_t total, T alpha,
-                                                     T beta, int beta_zero) {
+                                                     T beta, int beta_zero, const int* __restrict__ run_if) {
+    if (run_if && *run_if == 0) return;
     __shared__ MfmaSmem<T, WM, WN> sm;
     for (uint32_t tile = blockIdx.x; tile < total; tile += gridDim.x) {
         // problem of this tile: last g with prefix[g] <= tile
@@ -537,7 +538,7 @@ void plain_dgemm(rstsr_hip_ctx* ctx, const GemmProblem& P, double alpha, double 
 
 template <class T, int WM, int WN>
 static void launch_grouped_cfg(rstsr_hip_ctx* ctx, const std::vector<TileArgs<T>>& probs, T alpha, T beta,
-                               bool beta_zero) {
+                               bool beta_zero, const int* run_if = nullptr) {
     constexpr int BM = 32 * WM, BN = 32 * WN;
     std::vector<GroupProblem<T>> gp(probs.size());
     std::vector<uint32_t> prefix(probs.size() + 1, 0);
@@ -566,7 +567,7 @@ static void launch_grouped_cfg(rstsr_hip_ctx* ctx, const std::vector<TileArgs<T>
     uint32_t grid = (uint32_t)std::min<uint64_t>(total, (uint64_t)std::max(ctx->num_cu, 1) * 2);
     hipLaunchKernelGGL((k_gemm_grouped<T, WM, WN>), dim3(grid), dim3(GT), 0, ctx->stream,
                        (const GroupProblem<T>*)meta, (int)gp.size(), (const uint32_t*)(meta + poff), (uint32_t)total,
-                       alpha, beta, beta_zero ? 1 : 0);
+                       alpha, beta, beta_zero ? 1 : 0, run_if);
     check_launch("grouped gemm");
     // the host staging vectors die with this call: wait for the copies
     HIP_CHECK(hipStreamSynchronize(ctx->stream));
@@ -595,6 +596,16 @@ static void grouped_t(rstsr_hip_ctx* ctx, const std::vector<TileArgs<T>>& probs,
     std::snprintf(key, sizeof(key), "grouped:%s:n%d:mean%d:work%d", CX ? "z" : "d", bucket((int64_t)probs.size()),
                   bucket((int64_t)mean), bucket(tot_mnk));
     tuned_run(ctx, key, CX ? 2 : 3, heuristic_cfg<T>((int64_t)mean, (int64_t)mean), beta_zero, run);
+}
+
+void plain_grouped_f64(rstsr_hip_ctx* ctx, const std::vector<GemmProblem>& probs, double alpha, double beta,
+                       bool beta_zero, const int* run_if) {
+    std::vector<TileArgs<double>> ta;
+    ta.reserve(probs.size());
+    for (auto& P : probs)
+        ta.push_back(TileArgs<double>{P.M, P.N, P.K, (const double*)P.a, P.rsa, P.csa, (const double*)P.b, P.rsb, P.csb,
+                                      (double*)P.c, P.rsc, P.csc});
+    if (!ta.empty()) launch_grouped_cfg<double, 2, 4>(ctx, ta, alpha, beta, beta_zero, run_if);
 }
 
 // Zero-fill C (k == 0 with beta == 0: crates-device/rstsr-openblas/src/matmul_impl.rs:78-81).
@@ -720,17 +731,18 @@ void run_gemm(rstsr_hip_ctx* ctx, rstsr_dtype_t t, rstsr_gemm_mode_t mode, const
 // Check that each operand's matrices of every batch stay inside its buffer.
 static void check_gemm_extent(const GemmProblem& P, size_t es) {
     auto mk = [&](const void* p, int64_t r, int64_t cdim, int64_t rs, int64_t cs, const int64_t* bs, const char* nm) {
-        Shape s;
-        Stride t;
-        for (int d = 0; d < P.bd.nbd; ++d) {
-            s.push_back(P.bd.shape[d]);
-            t.push_back(bs[d]);
+        size_t s[MAXD + 2];
+        int64_t t[MAXD + 2];
+        int nd = 0;
+        for (int d = 0; d < P.bd.nbd; ++d, ++nd) {
+            s[nd] = P.bd.shape[d];
+            t[nd] = bs[d];
         }
-        s.push_back((size_t)r);
-        t.push_back(rs);
-        s.push_back((size_t)cdim);
-        t.push_back(cs);
-        check_extent(p, Layout(s, t, 0), es, nm);
+        s[nd] = (size_t)r;
+        t[nd++] = rs;
+        s[nd] = (size_t)cdim;
+        t[nd++] = cs;
+        check_extent_rel(p, (size_t)nd, s, t, es, nm);
     };
     mk(P.a, P.M, P.K, P.rsa, P.csa, P.bd.sa, "a");
     mk(P.b, P.K, P.N, P.rsb, P.csb, P.bd.sb, "b");
@@ -952,10 +964,17 @@ int rstsr_hip_gemm_grouped(rstsr_hip_ctx_t* ctx, rstsr_dtype_t t, rstsr_gemm_mod
             };
             if (t == RSTSR_F64) go(double{});
             else go(c128{});
+        } else if (mode == RSTSR_GEMM_OZAKI_INT8 && t == RSTSR_F64) {
+            double al, be;
+            std::memcpy(&al, alpha, sizeof(double));
+            std::memcpy(&be, beta, sizeof(double));
+            std::vector<GemmProblem> oz;
+            for (auto& P : probs) (P.K == 0 ? rest : oz).push_back(P);
+            ozaki_grouped_f64(ctx, oz, al, be, is_zero_scalar(t, beta));
         } else {
             rest = probs;
         }
-        // split-precision and non-MFMA dtypes: one call per problem, in order
+        // complex split mode and non-MFMA dtypes: one call per problem, in order
         for (auto& P : rest) run_gemm(ctx, t, mode, P, alpha, beta);
     });
 }

@@ -1,6 +1,8 @@
 // Shared GEMM problem description (gemm.hip, ozaki.hip).
 #pragma once
 
+#include <vector>
+
 #include "common.hpp"
 
 namespace rstsr {
@@ -62,6 +64,14 @@ void ozaki_gemm_f64(rstsr_hip_ctx* ctx, const GemmProblem& P, double alpha, doub
 // the kernel exits unless *run_if != 0.
 void plain_dgemm(rstsr_hip_ctx* ctx, const GemmProblem& P, double alpha, double beta, bool beta_zero,
                  const int* run_if = nullptr);
+// Grouped split GEMM of unbatched f64 problems with M, N, K > 0 (ozaki.hip).
+// Outputs must not overlap.
+void ozaki_grouped_f64(rstsr_hip_ctx* ctx, const std::vector<GemmProblem>& probs, double alpha, double beta,
+                       bool beta_zero);
+// Plain grouped FP64 GEMM in one persistent launch with a fixed tile shape
+// (gemm.hip); run_if as for plain_dgemm.
+void plain_grouped_f64(rstsr_hip_ctx* ctx, const std::vector<GemmProblem>& probs, double alpha, double beta,
+                       bool beta_zero, const int* run_if);
 
 }  // namespace hip
 }  // namespace rstsr

@@ -29,6 +29,8 @@
 #include <cfloat>
 #include <cmath>
 #include <cstdlib>
+#include <cstring>
+#include <type_traits>
 
 #include "gemm.hpp"
 
@@ -43,114 +45,134 @@ constexpr int OZ_MAXMOD = 14;  // X = sum r_l w_l < 14 * 256 * M < 2^128 needs M
 // largest pairwise coprime moduli <= 256
 static const uint32_t OZ_MODULI[16] = {256, 255, 253, 251, 247, 241, 239, 233, 229, 227, 223, 217, 211, 199, 197, 193};
 
+// CRT constants: w_l = (M/m_l) ((M/m_l)^-1 mod m_l) and M = prod m_l, both
+// also in 32-bit limbs (least significant first) for the device sums.
 struct CrtConst {
     int n;
     uint32_t mod[OZ_MAXMOD];
-    u128 w[OZ_MAXMOD];
-    u128 M, Mhalf;
-    double Md;
+    uint32_t w32[OZ_MAXMOD][4];
+    uint32_t M32[4];
+    u128 M;
+    double Md, invM;
 };
 
 /* ---------------- vector norms -> scale exponents ---------------- */
 
-// Exponent e with ||x 2^e||_2 <= 2^g given amax = max|x| and ss = sum (x 2^-E)^2,
-// E = ilogb(amax). Zero vectors get e = 0.
-__device__ __forceinline__ int scale_exponent(double amax, double ss, int g) {
-    if (!(amax > 0.0)) return 0;
-    int E = ilogb(amax);
-    double l2 = (double)E + 0.5 * log2(ss);  // log2 of the norm
+// Running (max |x|, E = ilogb(max), ss = sum (x 2^-E)^2) in one pass: when the
+// max moves to a new binade the sum is rescaled by a power of two (exact, or
+// the old terms underflow and are negligible). Non-finite elements are skipped
+// and reported.
+struct NormAcc {
+    double m;
+    int E;
+    double ss;
+};
+
+__device__ __forceinline__ void norm_add(NormAcc& a, double x, bool& bad) {
+    if (!isfinite(x)) {
+        bad = true;
+        return;
+    }
+    const double ax = fabs(x);
+    if (ax > a.m) {
+        const int En = ilogb(ax);
+        if (a.m > 0.0) a.ss = ldexp(a.ss, 2 * (a.E - En));
+        a.E = En;
+        a.m = ax;
+    }
+    const double t = ldexp(x, -a.E);
+    a.ss += t * t;
+}
+
+// Commutative, so every lane of a butterfly ends with the same value.
+__device__ __forceinline__ void norm_merge(NormAcc& a, const NormAcc& b) {
+    if (!(b.m > 0.0)) return;
+    if (!(a.m > 0.0)) {
+        a = b;
+        return;
+    }
+    const int E = max(a.E, b.E);
+    a.ss = ldexp(a.ss, 2 * (a.E - E)) + ldexp(b.ss, 2 * (b.E - E));
+    a.E = E;
+    a.m = fmax(a.m, b.m);
+}
+
+__device__ __forceinline__ NormAcc wave_norm(NormAcc a) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        NormAcc b{__shfl_xor(a.m, o), __shfl_xor(a.E, o), __shfl_xor(a.ss, o)};
+        norm_merge(a, b);
+    }
+    return a;
+}
+
+// Exponent e with ||x 2^e||_2 <= 2^g; zero vectors get e = 0.
+__device__ __forceinline__ int scale_exponent(const NormAcc& a, int g) {
+    if (!(a.m > 0.0)) return 0;
+    const double l2 = (double)a.E + 0.5 * log2(a.ss);  // log2 of the norm
     return g - (int)ceil(l2 + 1e-12);
 }
 
-// One block per vector (vector elements strided by st): two passes, max then
-// scaled sum of squares. flag[0] |= 1 on a non-finite element.
-__global__ void __launch_bounds__(256) k_vnorm_block(const double* __restrict__ x, int64_t nvec, int64_t len,
-                                                     int64_t sv, int64_t st, int g, int* __restrict__ ev,
-                                                     int* __restrict__ flag) {
-    __shared__ double red[256];
-    const int64_t v = blockIdx.x;
-    const double* p = x + v * sv;
-    double m = 0.0;
+// Scale exponents of nvec vectors of length len (element t of vector v at
+// x[v*sv + t*st]) for every batch b = blockIdx.y of one operand (`which`: 0 = A,
+// 1 = B offsets of bd); exponents to ev[b*evs + v]. One wave per vector: for
+// vectors whose elements are contiguous.
+__global__ void __launch_bounds__(256) k_norm_rows(const double* __restrict__ x, int64_t nvec, int64_t len,
+                                                   int64_t sv, int64_t st, BatchDesc bd, int which, int64_t evs, int g,
+                                                   int* __restrict__ ev, int* __restrict__ flag) {
+    const int lane = threadIdx.x & 63;
+    const int64_t v = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const uint32_t b = blockIdx.y;
+    if (v >= nvec) return;  // uniform across the wave
+    int64_t oa, ob, oc;
+    batch_offsets(bd, b, oa, ob, oc);
+    const double* xv = x + (which ? ob : oa) + v * sv;
+    NormAcc a{0.0, 0, 0.0};
     bool bad = false;
-    for (int64_t t = threadIdx.x; t < len; t += 256) {
-        double a = p[t * st];
-        if (!isfinite(a)) bad = true;
-        m = fmax(m, fabs(a));
-    }
-    red[threadIdx.x] = m;
-    __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-        if (threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
-        __syncthreads();
-    }
-    const double amax = red[0];
-    __syncthreads();
-    double ss = 0.0;
-    if (amax > 0.0 && isfinite(amax)) {
-        int E = ilogb(amax);
-        for (int64_t t = threadIdx.x; t < len; t += 256) {
-            double a = ldexp(p[t * st], -E);
-            ss += a * a;
-        }
-    }
-    red[threadIdx.x] = ss;
-    __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-        if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
-        __syncthreads();
-    }
+    for (int64_t t = lane; t < len; t += 64) norm_add(a, xv[t * st], bad);
+    a = wave_norm(a);
     if (bad) atomicOr(flag, 1);
-    if (threadIdx.x == 0) ev[v] = isfinite(amax) ? scale_exponent(amax, red[0], g) : 0;
+    if (lane == 0) ev[(int64_t)b * evs + v] = scale_exponent(a, g);
 }
 
-// Vectors adjacent in memory (|sv| small): thread per vector, partial max over
-// chunks of `chunk` elements into pm[c][v].
-__global__ void __launch_bounds__(256) k_vnorm_cols_max(const double* __restrict__ x, int64_t nvec, int64_t len,
-                                                        int64_t sv, int64_t st, int64_t chunk,
-                                                        double* __restrict__ pm, int* __restrict__ flag) {
-    const int64_t v = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    const int64_t c = blockIdx.y;
-    if (v >= nvec) return;
-    const int64_t t0 = c * chunk, t1 = min(len, t0 + chunk);
-    double m = 0.0;
+// The same for vectors side by side: a lane per vector (64 per block), the four
+// waves and gridDim.y chunks of `chunk` elements split the length, batch on
+// blockIdx.z. With several chunks the partials go to part[(b*nchunk + c)*nvec + v].
+__global__ void __launch_bounds__(256) k_norm_cols(const double* __restrict__ x, int64_t nvec, int64_t len,
+                                                   int64_t sv, int64_t st, int64_t chunk, BatchDesc bd, int which,
+                                                   int64_t evs, int g, NormAcc* __restrict__ part,
+                                                   int* __restrict__ ev, int* __restrict__ flag) {
+    __shared__ NormAcc red[4][64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int64_t v = (int64_t)blockIdx.x * 64 + lane;
+    const uint32_t c = blockIdx.y, b = blockIdx.z;
+    int64_t oa, ob, oc;
+    batch_offsets(bd, b, oa, ob, oc);
+    const double* xb = x + (which ? ob : oa);
+    NormAcc a{0.0, 0, 0.0};
     bool bad = false;
-    for (int64_t t = t0; t < t1; ++t) {
-        double a = x[v * sv + t * st];
-        if (!isfinite(a)) bad = true;
-        m = fmax(m, fabs(a));
+    if (v < nvec) {
+        const int64_t t1 = min(len, (int64_t)(c + 1) * chunk);
+        for (int64_t t = (int64_t)c * chunk + w; t < t1; t += 4) norm_add(a, xb[v * sv + t * st], bad);
     }
+    red[w][lane] = a;
+    __syncthreads();
     if (bad) atomicOr(flag, 1);
-    pm[c * nvec + v] = m;
+    if (w != 0 || v >= nvec) return;
+#pragma unroll
+    for (int q = 1; q < 4; ++q) norm_merge(a, red[q][lane]);
+    if (gridDim.y == 1) ev[(int64_t)b * evs + v] = scale_exponent(a, g);
+    else part[((int64_t)b * gridDim.y + c) * nvec + v] = a;
 }
 
-__global__ void __launch_bounds__(256) k_vnorm_cols_ss(const double* __restrict__ x, int64_t nvec, int64_t len,
-                                                       int64_t sv, int64_t st, int64_t chunk, int64_t nchunk,
-                                                       const double* __restrict__ pm, double* __restrict__ ps) {
+__global__ void __launch_bounds__(256) k_norm_fin(const NormAcc* __restrict__ part, int64_t nvec, int nchunk,
+                                                  int64_t evs, int g, int* __restrict__ ev) {
     const int64_t v = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    const int64_t c = blockIdx.y;
+    const uint32_t b = blockIdx.y;
     if (v >= nvec) return;
-    double amax = 0.0;
-    for (int64_t q = 0; q < nchunk; ++q) amax = fmax(amax, pm[q * nvec + v]);
-    double ss = 0.0;
-    if (amax > 0.0 && isfinite(amax)) {
-        int E = ilogb(amax);
-        const int64_t t0 = c * chunk, t1 = min(len, t0 + chunk);
-        for (int64_t t = t0; t < t1; ++t) {
-            double a = ldexp(x[v * sv + t * st], -E);
-            ss += a * a;
-        }
-    }
-    ps[c * nvec + v] = ss;
-}
-
-__global__ void __launch_bounds__(256) k_vnorm_cols_fin(int64_t nvec, int64_t nchunk, const double* __restrict__ pm,
-                                                        const double* __restrict__ ps, int g, int* __restrict__ ev) {
-    const int64_t v = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (v >= nvec) return;
-    double amax = 0.0, ss = 0.0;
-    for (int64_t q = 0; q < nchunk; ++q) amax = fmax(amax, pm[q * nvec + v]);
-    for (int64_t q = 0; q < nchunk; ++q) ss += ps[q * nvec + v];
-    ev[v] = isfinite(amax) ? scale_exponent(amax, ss, g) : 0;
+    NormAcc a = part[(int64_t)b * nchunk * nvec + v];
+    for (int c = 1; c < nchunk; ++c) norm_merge(a, part[((int64_t)b * nchunk + c) * nvec + v]);
+    ev[(int64_t)b * evs + v] = scale_exponent(a, g);
 }
 
 /* ---------------- split into residue planes ---------------- */
@@ -171,160 +193,226 @@ __device__ __forceinline__ int sym_residue(double x, double m, double inv) {
     return ri > 127 ? ri - (int)m : ri;
 }
 
-// Source S(v, t) = s[v*sv + t*st] for v < nv, t < len; planes P[l][v][t] for
-// v < nvp, t < lenp (zero padded). 64 x 64 tiles staged through LDS so that
-// reads follow the source's unit-stride axis and plane writes follow t.
-__global__ void __launch_bounds__(256) k_split(const double* __restrict__ s, int64_t nv, int64_t len, int64_t sv,
-                                               int64_t st, const int* __restrict__ ev, int64_t nvp, int64_t lenp,
-                                               int8_t* __restrict__ planes, ModSet ms, int read_along_t,
-                                               const int* __restrict__ skip, int64_t plane) {
-    if (*skip) return;  // non-finite input: the plain kernel computes this product
-    __shared__ double tile[64][65];
-    const int64_t v0 = (int64_t)blockIdx.y * 64, t0 = (int64_t)blockIdx.x * 64;
+__device__ __forceinline__ uint32_t pack_residues(const double* x, double m, double inv) {
+    uint32_t w = 0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) w |= (uint32_t)(uint8_t)(int8_t)sym_residue(x[i], m, inv) << (8 * i);
+    return w;
+}
+
+// Tile (v0, t0) of the split: source S(v, t) = s[v*sv + t*st] for v < nv,
+// t < len; planes P[l][v][t] for v < nvp, t < lenp (zero padded; lenp a
+// multiple of 64). The 64 x 64 tile is staged through LDS (`tile`) so that
+// reads run along the source's unit-stride axis and each thread writes 16
+// consecutive plane bytes per modulus.
+__device__ __forceinline__ void split_tile(const double* __restrict__ s, int64_t nv, int64_t len, int64_t sv,
+                                           int64_t st, const int* __restrict__ ev, int64_t nvp, int64_t lenp,
+                                           int8_t* __restrict__ planes, const ModSet& ms, bool read_along_t,
+                                           int64_t plane, int64_t v0, int64_t t0, double (&tile)[64][65]) {
     const int tid = threadIdx.x;
-    // phase 1: load (scaled, rounded) values
 #pragma unroll
     for (int k = 0; k < 16; ++k) {
-        int e = k * 256 + tid;
-        int a, b;  // a fastest
-        if (read_along_t) {
-            a = e & 63;
-            b = e >> 6;
-        } else {
-            b = e & 63;
-            a = e >> 6;
-        }
-        // a indexes t when reading along t, else b indexes t
-        int tt = read_along_t ? a : b, vv = read_along_t ? b : a;
-        int64_t gv = v0 + vv, gt = t0 + tt;
+        const int e = k * 256 + tid, fast = e & 63, slow = e >> 6;
+        const int tt = read_along_t ? fast : slow, vv = read_along_t ? slow : fast;
+        const int64_t gv = v0 + vv, gt = t0 + tt;
         double x = 0.0;
         if (gv < nv && gt < len) x = rint(ldexp(s[gv * sv + gt * st], ev[gv]));
         tile[vv][tt] = x;
     }
     __syncthreads();
-    // phase 2: each thread packs 4 consecutive t of one v, for every modulus
-    const int vv = tid >> 2;       // 0..63
-    const int tq = (tid & 3) * 16;  // 4 threads x 16 t = 64 t per row
+    const int vv = tid >> 2, tq = (tid & 3) * 16;
     const int64_t gv = v0 + vv;
     if (gv >= nvp) return;
+    double x[16];
 #pragma unroll
-    for (int c = 0; c < 16; c += 4) {
-        int64_t gt = t0 + tq + c;
-        if (gt >= lenp) continue;
-        double x0 = tile[vv][tq + c], x1 = tile[vv][tq + c + 1], x2 = tile[vv][tq + c + 2], x3 = tile[vv][tq + c + 3];
-        for (int l = 0; l < ms.n; ++l) {
-            uint32_t b0 = (uint8_t)(int8_t)sym_residue(x0, ms.m[l], ms.inv[l]);
-            uint32_t b1 = (uint8_t)(int8_t)sym_residue(x1, ms.m[l], ms.inv[l]);
-            uint32_t b2 = (uint8_t)(int8_t)sym_residue(x2, ms.m[l], ms.inv[l]);
-            uint32_t b3 = (uint8_t)(int8_t)sym_residue(x3, ms.m[l], ms.inv[l]);
-            uint32_t packed = b0 | (b1 << 8) | (b2 << 16) | (b3 << 24);
-            *(uint32_t*)(planes + l * plane + (size_t)gv * lenp + gt) = packed;
-        }
+    for (int i = 0; i < 16; ++i) x[i] = tile[vv][tq + i];
+    int8_t* dst = planes + gv * lenp + t0 + tq;
+    for (int l = 0; l < ms.n; ++l) {
+        const double m = ms.m[l], inv = ms.inv[l];
+        uint4 w;
+        w.x = pack_residues(x, m, inv);
+        w.y = pack_residues(x + 4, m, inv);
+        w.z = pack_residues(x + 8, m, inv);
+        w.w = pack_residues(x + 12, m, inv);
+        *(uint4*)(dst + l * plane) = w;
     }
+}
+
+// Batch b = blockIdx.z of one operand (`which`: 0 = A, 1 = B offsets of bd):
+// exponents ev + b*nvp, planes + b*nvp*lenp.
+__global__ void __launch_bounds__(256) k_split(const double* __restrict__ s, int64_t nv, int64_t len, int64_t sv,
+                                               int64_t st, BatchDesc bd, int which, const int* __restrict__ ev,
+                                               int64_t nvp, int64_t lenp, int8_t* __restrict__ planes, ModSet ms,
+                                               int read_along_t, const int* __restrict__ skip, int64_t plane) {
+    if (*skip) return;  // non-finite input: the plain kernel computes this product
+    __shared__ double tile[64][65];
+    const uint32_t b = blockIdx.z;
+    int64_t oa, ob, oc;
+    batch_offsets(bd, b, oa, ob, oc);
+    split_tile(s + (which ? ob : oa), nv, len, sv, st, ev + (int64_t)b * nvp, nvp, lenp,
+               planes + (int64_t)b * nvp * lenp, ms, read_along_t != 0, plane, (int64_t)blockIdx.y * 64,
+               (int64_t)blockIdx.x * 64, tile);
 }
 
 /* ---------------- INT8 MFMA GEMM per modulus ---------------- */
 
-constexpr int I8_BM = 128, I8_BN = 128, I8_BK = 128, I8_PAD = 16;
+// INT8 tile configurations: a BM x BN block tile from WVM x WVN waves, K slices
+// of BK bytes double-buffered in LDS (rows padded by PAD bytes against bank
+// conflicts of the 16-byte fragment reads).
+template <int BM_, int BN_, int BK_, int WVM_, int WVN_, int OCC_, int PAD_>
+struct I8Cfg {
+    static constexpr int BM = BM_, BN = BN_, BK = BK_, WVM = WVM_, WVN = WVN_, OCC = OCC_, PAD = PAD_;
+    static constexpr int NT = 64 * WVM * WVN;
+};
+// large products: 64 x 128 per wave
+using I8Wide = I8Cfg<128, 256, 64, 2, 2, 1, 32>;
+// small and grouped products: 64 x 64 per wave, two blocks per CU
+using I8Sq = I8Cfg<128, 128, 128, 2, 2, 2, 16>;
+// planes: rows padded to I8_MPAD (every BM divides it), K to I8_KPAD (every BK divides it)
+constexpr int I8_MPAD = 128, I8_KPAD = 128;
+// one launch sums at most I8_KCHUNK products per output: |sum| <= 2^16 * 2^14 < 2^31
+constexpr int64_t I8_KCHUNK = 65536;
+static_assert(I8Wide::BM == I8_MPAD && I8Sq::BM == I8_MPAD, "tiles_m = Mp / I8_MPAD");
 
 __device__ __forceinline__ uint32_t xcd_remap8(uint32_t t, uint32_t ntiles) {
     if (ntiles % 8 != 0) return t;
     return (t % 8) * (ntiles / 8) + t / 8;
 }
 
-// A8: [Mp][Kp], B8: [Np][Kp] (both K contiguous, zero padded); R: [M][ldr] uint8
-// residues of (A8 B8^T) mod `mod`. 256 threads = 4 waves (2x2), 64x64 per wave
-// from 4x4 v_mfma_i32_16x16x64_i8 tiles; LDS double buffer, one barrier per slice.
-// Batches on blockIdx.y: plane offsets sA / sB (0 for a broadcast operand) and sR.
-template <bool ROW4G>
-__global__ void __launch_bounds__(256) k_i8gemm_mod(const int8_t* __restrict__ A, const int8_t* __restrict__ B,
-                                                    uint8_t* __restrict__ R, int64_t M, int64_t N, int64_t Kp,
-                                                    int64_t ldr, int mod, uint32_t tiles_n, uint32_t ntiles,
-                                                    int64_t sA, int64_t sB, int64_t sR,
-                                                    const int* __restrict__ skip) {
-    if (*skip) return;
-    A += (int64_t)blockIdx.y * sA;
-    B += (int64_t)blockIdx.y * sB;
-    R += (int64_t)blockIdx.y * sR;
-    __shared__ __attribute__((aligned(16))) int8_t As[2][I8_BM][I8_BK + I8_PAD];
-    __shared__ __attribute__((aligned(16))) int8_t Bs[2][I8_BN][I8_BK + I8_PAD];
-    const uint32_t tile = xcd_remap8(blockIdx.x, ntiles);
-    const uint32_t tm = tile / tiles_n, tn = tile - (tile / tiles_n) * tiles_n;
-    const int8_t* Ab = A + (size_t)tm * I8_BM * Kp;
-    const int8_t* Bb = B + (size_t)tn * I8_BN * Kp;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wm = wave >> 1, wn = wave & 1;
-    const int lr = tid >> 3, lc = (tid & 7) * 16;  // loader: 8 threads x 16 B per 128 B row, 32 rows per pass
+// Block id -> output tile. Ids are round-robined over the 8 XCDs, so each XCD
+// first gets a contiguous range of tiles; inside it, tiles run down groups of
+// GM tile rows, so the blocks resident on one XCD at a time share GM row panels
+// of A and a few column panels of B in that XCD's L2.
+template <int GM>
+__device__ __forceinline__ void tile_coords(uint32_t id, uint32_t tiles_m, uint32_t tiles_n, uint32_t& tm,
+                                            uint32_t& tn) {
+    const uint32_t t = xcd_remap8(id, tiles_m * tiles_n);
+    const uint32_t per = GM * tiles_n, grp = t / per, first = grp * GM;
+    const uint32_t gsz = min((uint32_t)GM, tiles_m - first), r = t - grp * per;
+    tm = first + r % gsz;
+    tn = r / gsz;
+}
 
-    v4i ra[4], rb[4];
+// x mod m in [0, m) for |x| < 2^31, m <= 256, inv = 1/m: x * inv is within
+// 2^-21 of x / m, so its floor is the quotient or (x a multiple of m rounding
+// below) one less; the remainder is then in [0, 2m) and corrected.
+__device__ __forceinline__ int mod_small(int x, int m, double inv) {
+    const int q = (int)floor((double)x * inv);
+    int r = x - q * m;
+    if (r < 0) r += m;
+    if (r >= m) r -= m;
+    return r;
+}
+
+template <class CF>
+struct I8Smem {
+    int8_t As[2][CF::BM][CF::BK + CF::PAD];
+    int8_t Bs[2][CF::BN][CF::BK + CF::PAD];
+};
+
+// Output tile (tm, tn) of one residue product over a K range of Kc <= I8_KCHUNK
+// bytes: A8 [>= (tm+1) BM][ldk] and B8 [>= (tn+1) BN][ldk] (K contiguous, zero
+// padded); R [M][ldr] gets the uint8 residues of (A8 B8^T) mod `mod` for rows
+// < M and columns < N, added to the residues already there when `accum`. Each
+// wave owns (BM / WVM) x (BN / WVN) outputs from v_mfma_i32_16x16x64_i8 tiles;
+// global loads for slice k+1 are in flight while slice k is multiplied. The
+// accumulators see nothing but MFMAs until the epilogue, so they stay in AGPRs.
+template <class CF, bool ROW4G>
+__device__ __forceinline__ void i8_tile(const int8_t* __restrict__ A, const int8_t* __restrict__ B,
+                                        uint8_t* __restrict__ R, int64_t M, int64_t N, int64_t ldk, int64_t Kc,
+                                        int64_t ldr, int mod, bool accum, uint32_t tm, uint32_t tn, I8Smem<CF>& sm) {
+    constexpr int BM = CF::BM, BN = CF::BN, BK = CF::BK, NT = CF::NT;
+    constexpr int WTM = BM / CF::WVM, WTN = BN / CF::WVN;
+    constexpr int FM = WTM / 16, FN = WTN / 16;
+    constexpr int CPR = BK / 16, RPP = NT / CPR, LA = BM / RPP, LB = BN / RPP;
+    static_assert(BM % RPP == 0 && BN % RPP == 0 && BK % 64 == 0 && I8_KPAD % BK == 0, "tile shape");
+    auto& As = sm.As;
+    auto& Bs = sm.Bs;
+    const int8_t* Ab = A + (size_t)tm * BM * ldk;
+    const int8_t* Bb = B + (size_t)tn * BN * ldk;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave / CF::WVN, wn = wave % CF::WVN;
+    const int lr = tid / CPR, lc = (tid % CPR) * 16;  // loader: CPR threads x 16 B per row, RPP rows per pass
+
+    v4i ra[LA], rb[LB];
     auto load = [&](int64_t k0) {
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            ra[i] = *(const v4i*)(Ab + (size_t)(lr + 32 * i) * Kp + k0 + lc);
-            rb[i] = *(const v4i*)(Bb + (size_t)(lr + 32 * i) * Kp + k0 + lc);
-        }
+        for (int i = 0; i < LA; ++i) ra[i] = *(const v4i*)(Ab + (size_t)(lr + RPP * i) * ldk + k0 + lc);
+#pragma unroll
+        for (int i = 0; i < LB; ++i) rb[i] = *(const v4i*)(Bb + (size_t)(lr + RPP * i) * ldk + k0 + lc);
     };
     auto store = [&](int buf) {
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            *(v4i*)&As[buf][lr + 32 * i][lc] = ra[i];
-            *(v4i*)&Bs[buf][lr + 32 * i][lc] = rb[i];
-        }
+        for (int i = 0; i < LA; ++i) *(v4i*)&As[buf][lr + RPP * i][lc] = ra[i];
+#pragma unroll
+        for (int i = 0; i < LB; ++i) *(v4i*)&Bs[buf][lr + RPP * i][lc] = rb[i];
     };
-    v4i acc[4][4];
+    v4i acc[FM][FN];
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < FM; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = v4i{0, 0, 0, 0};
+        for (int j = 0; j < FN; ++j) acc[i][j] = v4i{0, 0, 0, 0};
 
     load(0);
     store(0);
     __syncthreads();
     const int fr = lane & 15, fk = (lane >> 4) * 16;
     int buf = 0;
-    for (int64_t k0 = 0; k0 < Kp; k0 += I8_BK) {
-        const bool more = k0 + I8_BK < Kp;
-        if (more) load(k0 + I8_BK);
+    for (int64_t k0 = 0; k0 < Kc; k0 += BK) {
+        const bool more = k0 + BK < Kc;
+        if (more) load(k0 + BK);
 #pragma unroll
-        for (int kk = 0; kk < I8_BK; kk += 64) {
-            v4i af[4], bf[4];
+        for (int kk = 0; kk < BK; kk += 64) {
+            v4i af[FM], bf[FN];
 #pragma unroll
-            for (int i = 0; i < 4; ++i) af[i] = *(const v4i*)&As[buf][wm * 64 + i * 16 + fr][kk + fk];
+            for (int i = 0; i < FM; ++i) af[i] = *(const v4i*)&As[buf][wm * WTM + i * 16 + fr][kk + fk];
 #pragma unroll
-            for (int j = 0; j < 4; ++j) bf[j] = *(const v4i*)&Bs[buf][wn * 64 + j * 16 + fr][kk + fk];
+            for (int j = 0; j < FN; ++j) bf[j] = *(const v4i*)&Bs[buf][wn * WTN + j * 16 + fr][kk + fk];
 #pragma unroll
-            for (int i = 0; i < 4; ++i)
+            for (int i = 0; i < FM; ++i)
 #pragma unroll
-                for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(af[i], bf[j], acc[i][j], 0, 0, 0);
-        }
-        // keep |acc| < 2^31: reduce every 2^16 of K (each step adds < 128*128*128 per element)
-        if (((k0 + I8_BK) & 65535) == 0) {
-#pragma unroll
-            for (int i = 0; i < 4; ++i)
-#pragma unroll
-                for (int j = 0; j < 4; ++j)
-#pragma unroll
-                    for (int r = 0; r < 4; ++r) acc[i][j][r] %= mod;
+                for (int j = 0; j < FN; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(af[i], bf[j], acc[i][j], 0, 0, 0);
         }
         if (more) store(buf ^ 1);
         __syncthreads();
         buf ^= 1;
     }
+    const double inv = 1.0 / (double)mod;
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < FM; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j)
+        for (int j = 0; j < FN; ++j)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int lrow = ROW4G ? (4 * (lane >> 4) + r) : (4 * r + (lane >> 4));
-                int64_t gm = (int64_t)tm * I8_BM + wm * 64 + i * 16 + lrow;
-                int64_t gn = (int64_t)tn * I8_BN + wn * 64 + j * 16 + (lane & 15);
+                int64_t gm = (int64_t)tm * BM + wm * WTM + i * 16 + lrow;
+                int64_t gn = (int64_t)tn * BN + wn * WTN + j * 16 + (lane & 15);
                 if (gm < M && gn < N) {
-                    int v = acc[i][j][r] % mod;
-                    if (v < 0) v += mod;
+                    int v = mod_small(acc[i][j][r], mod, inv);
+                    if (accum) {
+                        v += R[gm * ldr + gn];
+                        if (v >= mod) v -= mod;
+                    }
                     R[gm * ldr + gn] = (uint8_t)v;
                 }
             }
+}
+
+// Batches on blockIdx.y: plane offsets sA / sB (0 for a broadcast operand) and sR.
+template <class CF, bool ROW4G>
+__global__ void __launch_bounds__(CF::NT, CF::OCC) k_i8gemm_mod(const int8_t* __restrict__ A,
+                                                                const int8_t* __restrict__ B, uint8_t* __restrict__ R,
+                                                                int64_t M, int64_t N, int64_t ldk, int64_t Kc,
+                                                                int64_t ldr, int mod, int accum, uint32_t tiles_m,
+                                                                uint32_t tiles_n, int64_t sA, int64_t sB, int64_t sR,
+                                                                const int* __restrict__ skip) {
+    if (*skip) return;
+    __shared__ __attribute__((aligned(16))) I8Smem<CF> sm;
+    uint32_t tm, tn;
+    tile_coords<4>(blockIdx.x, tiles_m, tiles_n, tm, tn);
+    i8_tile<CF, ROW4G>(A + (int64_t)blockIdx.y * sA, B + (int64_t)blockIdx.y * sB, R + (int64_t)blockIdx.y * sR, M, N,
+                       ldk, Kc, ldr, mod, accum != 0, tm, tn, sm);
 }
 
 // zero n ints (a kernel rather than a memset node, so the sequence can be graph-captured)
@@ -340,39 +428,223 @@ __device__ __forceinline__ double u128_to_double(u128 x) {
     return ldexp((double)hi, 64) + (double)lo;
 }
 
-// R planes: [l][batch][Mp][N] (plane stride pl); eA [nbA][Mp], fB [nbB][Np].
-__global__ void __launch_bounds__(256) k_crt(const uint8_t* __restrict__ R, int64_t M, int64_t N, int64_t Mp,
-                                             int64_t Np, uint64_t nbatch, int64_t pl, int a_batched, int b_batched,
-                                             BatchDesc bd, CrtConst cc, const int* __restrict__ eA,
-                                             const int* __restrict__ fB, double* C, int64_t rsc, int64_t csc,
-                                             double alpha, double beta, int beta_zero, const int* __restrict__ skip) {
-    const uint64_t idx = (uint64_t)blockIdx.x * 256 + threadIdx.x;
-    const uint64_t mn = (uint64_t)M * N;
-    if (idx >= mn * nbatch || *skip) return;
-    const uint64_t b = idx / mn;
-    const uint64_t e = idx - b * mn;
-    const int64_t i = (int64_t)(e / N), j = (int64_t)(e - (e / N) * N);
-    const uint64_t r_off = (b * Mp + i) * N + j;
-    u128 X = 0;
+// The integer in (-M/2, M/2] with residues r[l] mod m_l, as a double.
+// X = sum_l r_l w_l is summed per 32-bit limb of w (each limb sum < 14 * 2^8 *
+// 2^32 < 2^44), then X < 14 * 2^8 * M < 2^124 is reduced mod M with a
+// floating quotient estimate q < 2^12 and an exact correction.
+__device__ __forceinline__ double crt_from(const uint32_t (&r)[OZ_MAXMOD], const CrtConst& cc) {
+    uint64_t S0 = 0, S1 = 0, S2 = 0, S3 = 0;
 #pragma unroll
     for (int l = 0; l < OZ_MAXMOD; ++l) {
         if (l >= cc.n) break;
-        X += (u128)R[l * pl + r_off] * cc.w[l];
+        const uint64_t rl = r[l];
+        S0 += rl * cc.w32[l][0];
+        S1 += rl * cc.w32[l][1];
+        S2 += rl * cc.w32[l][2];
+        S3 += rl * cc.w32[l][3];
     }
-    // X < n * 256 * M < 2^128: reduce mod M with a floating quotient estimate
-    double qd = floor(u128_to_double(X) / cc.Md);
+    const u128 X = (u128)S0 + ((u128)S1 << 32) + ((u128)S2 << 64) + ((u128)S3 << 96);
+    double qd = floor(u128_to_double(X) * cc.invM);
     if (qd < 0.0) qd = 0.0;
-    u128 qM = (u128)(uint64_t)qd * cc.M;
+    const uint64_t q = (uint64_t)qd;
+    const u128 qM = (u128)(q * cc.M32[0]) + ((u128)(q * cc.M32[1]) << 32) + ((u128)(q * cc.M32[2]) << 64) +
+                    ((u128)(q * cc.M32[3]) << 96);
     i128 Y = (i128)(X - qM);  // X - qM in (-M, 2M)
     if (qM > X) Y = -(i128)(qM - X);
     if (Y < 0) Y += (i128)cc.M;
     if (Y >= (i128)cc.M) Y -= (i128)cc.M;
     if (Y >= (i128)cc.M - Y) Y -= (i128)cc.M;  // symmetric range (-M/2, M/2]
-    double v = Y < 0 ? -u128_to_double((u128)(-Y)) : u128_to_double((u128)Y);
-    v = ldexp(v, -(eA[(a_batched ? b : 0) * Mp + i] + fB[(b_batched ? b : 0) * Np + j]));
+    return Y < 0 ? -u128_to_double((u128)(-Y)) : u128_to_double((u128)Y);
+}
+
+// The same from residue bytes p[l * pl].
+__device__ __forceinline__ double crt_value(const uint8_t* __restrict__ p, int64_t pl, const CrtConst& cc) {
+    uint32_t r[OZ_MAXMOD];
+#pragma unroll
+    for (int l = 0; l < OZ_MAXMOD; ++l) r[l] = l < cc.n ? p[l * pl] : 0u;
+    return crt_from(r, cc);
+}
+
+// R planes: [l][batch][Mp][N] (plane stride pl); eA [nbA][Mp], fB [nbB][Np].
+// C = alpha 2^-(eA_i + fB_j) CRT(R) + beta C: columns along x (V consecutive
+// per thread, read as one V-byte word per plane), rows along y, batches along z.
+template <int V>
+__global__ void __launch_bounds__(256) k_crt(const uint8_t* __restrict__ R, int64_t M, int64_t N, int64_t Mp,
+                                             int64_t Np, int64_t pl, int a_batched, int b_batched, BatchDesc bd,
+                                             CrtConst cc, const int* __restrict__ eA, const int* __restrict__ fB,
+                                             double* C, int64_t rsc, int64_t csc, double alpha, double beta,
+                                             int beta_zero, const int* __restrict__ skip) {
+    static_assert(V == 1 || V == 4, "word size");
+    if (*skip) return;
+    const int64_t j0 = ((int64_t)blockIdx.x * 256 + threadIdx.x) * V;
+    if (j0 >= N) return;
+    const uint32_t b = blockIdx.z;
     int64_t oa, ob, oc;
-    batch_offsets(bd, (uint32_t)b, oa, ob, oc);
-    double* cp = C + oc + i * rsc + j * csc;
+    batch_offsets(bd, b, oa, ob, oc);
+    const int* eAb = eA + (a_batched ? (int64_t)b * Mp : 0);
+    const int* fBb = fB + (b_batched ? (int64_t)b * Np : 0);
+    for (int64_t i = blockIdx.y; i < M; i += gridDim.y) {
+        const uint8_t* rp = R + ((int64_t)b * Mp + i) * N + j0;
+        double* cp = C + oc + i * rsc + j0 * csc;
+        const int ei = eAb[i];
+        if constexpr (V == 1) {
+            const double v = ldexp(crt_value(rp, pl, cc), -(ei + fBb[j0]));
+            *cp = beta_zero ? alpha * v : alpha * v + beta * (*cp);
+        } else {
+            uint32_t w[OZ_MAXMOD];
+#pragma unroll
+            for (int l = 0; l < OZ_MAXMOD; ++l) w[l] = l < cc.n ? *(const uint32_t*)(rp + l * pl) : 0u;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                uint32_t r[OZ_MAXMOD];
+#pragma unroll
+                for (int l = 0; l < OZ_MAXMOD; ++l) r[l] = (w[l] >> (8 * k)) & 255u;
+                const double v = ldexp(crt_from(r, cc), -(ei + fBb[j0 + k]));
+                double* ck = cp + k * csc;
+                *ck = beta_zero ? alpha * v : alpha * v + beta * (*ck);
+            }
+        }
+    }
+}
+
+/* ---------------- grouped (uneven) split GEMM ---------------- */
+
+// One problem of a grouped split GEMM, with its offsets into the group's shared
+// arrays: oA / oB / oR elements into one A8 / B8 / R plane, oe / of into eA / fB.
+struct OzProb {
+    int64_t M, N, K, Mp, Np, Kp;
+    const double* a;
+    int64_t rsa, csa;
+    const double* b;
+    int64_t rsb, csb;
+    double* c;
+    int64_t rsc, csc;
+    int64_t oA, oB, oR, oe, of;
+    FastDiv fdN;  // division by N (M N < 2^32)
+};
+
+// Units of a grouped launch are numbered problem by problem: unit u belongs to
+// the last problem p with pre[p] <= u.
+__device__ __forceinline__ int find_prob(const uint32_t* __restrict__ pre, int count, uint32_t u) {
+    int lo = 0, hi = count - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (pre[mid] <= u) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// Scale exponents for a group: one block per 64 rows of an A or 64 columns of a
+// B. A vector whose elements are contiguous is read by one wave along the
+// vector; vectors side by side are read a lane per vector, the four waves
+// splitting the elements. flag[0] |= 1 on a non-finite element.
+__global__ void __launch_bounds__(256) k_gnorm(const OzProb* __restrict__ probs, int count,
+                                               const uint32_t* __restrict__ pre, int g, int* __restrict__ eA,
+                                               int* __restrict__ fB, int* __restrict__ flag) {
+    __shared__ NormAcc red[4][64];
+    const uint32_t u = blockIdx.x;
+    const int p = find_prob(pre, count, u);
+    const OzProb& P = probs[p];
+    const uint32_t local = u - pre[p];
+    const uint32_t ua = (uint32_t)((P.M + 63) / 64);
+    const bool isA = local < ua;
+    const double* x = isA ? P.a : P.b;
+    const int64_t nvec = isA ? P.M : P.N, sv = isA ? P.rsa : P.csb, st = isA ? P.csa : P.rsb, len = P.K;
+    int* ev = isA ? eA + P.oe : fB + P.of;
+    const int64_t v0 = (int64_t)(isA ? local : local - ua) * 64;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    bool bad = false;
+    if ((st == 1 || st == -1) && !(sv == 1 || sv == -1)) {
+        for (int q = w; q < 64; q += 4) {
+            const int64_t v = v0 + q;
+            if (v >= nvec) break;  // uniform across the wave
+            const double* xv = x + v * sv;
+            NormAcc a{0.0, 0, 0.0};
+            for (int64_t t = lane; t < len; t += 64) norm_add(a, xv[t * st], bad);
+            a = wave_norm(a);
+            if (lane == 0) ev[v] = scale_exponent(a, g);
+        }
+    } else {
+        const int64_t v = v0 + lane;
+        NormAcc a{0.0, 0, 0.0};
+        if (v < nvec)
+            for (int64_t t = w; t < len; t += 4) norm_add(a, x[v * sv + t * st], bad);
+        red[w][lane] = a;
+        __syncthreads();
+        if (w == 0 && v < nvec) {
+#pragma unroll
+            for (int q = 1; q < 4; ++q) norm_merge(a, red[q][lane]);
+            ev[v] = scale_exponent(a, g);
+        }
+    }
+    if (bad) atomicOr(flag, 1);
+}
+
+// Residue planes for a group: one block per 64 x 64 tile of an A8 or B8 plane.
+__global__ void __launch_bounds__(256) k_gsplit(const OzProb* __restrict__ probs, int count,
+                                                const uint32_t* __restrict__ pre, const int* __restrict__ eA,
+                                                const int* __restrict__ fB, int8_t* __restrict__ A8,
+                                                int8_t* __restrict__ B8, int64_t planeA, int64_t planeB, ModSet ms,
+                                                const int* __restrict__ skip) {
+    if (*skip) return;
+    __shared__ double tile[64][65];
+    const uint32_t u = blockIdx.x;
+    const int p = find_prob(pre, count, u);
+    const OzProb& P = probs[p];
+    const uint32_t local = u - pre[p];
+    const uint32_t kt = (uint32_t)(P.Kp / 64);
+    const uint32_t ta = (uint32_t)(P.Mp / 64) * kt;
+    if (local < ta) {
+        const uint32_t vt = local / kt, tt = local - vt * kt;
+        split_tile(P.a, P.M, P.K, P.rsa, P.csa, eA + P.oe, P.Mp, P.Kp, A8 + P.oA, ms, P.csa == 1 || P.csa == -1, planeA,
+                   (int64_t)vt * 64, (int64_t)tt * 64, tile);
+    } else {
+        const uint32_t lb = local - ta;
+        const uint32_t vt = lb / kt, tt = lb - vt * kt;
+        split_tile(P.b, P.N, P.K, P.csb, P.rsb, fB + P.of, P.Np, P.Kp, B8 + P.oB, ms, P.rsb == 1 || P.rsb == -1, planeB,
+                   (int64_t)vt * 64, (int64_t)tt * 64, tile);
+    }
+}
+
+// Residue products for a group: blockIdx.x runs over every output tile of every
+// problem, blockIdx.y over the moduli.
+template <class CF, bool ROW4G>
+__global__ void __launch_bounds__(CF::NT, CF::OCC) k_gi8gemm(const OzProb* __restrict__ probs, int count,
+                                                 const uint32_t* __restrict__ pre, uint32_t total,
+                                                 const int8_t* __restrict__ A8, const int8_t* __restrict__ B8,
+                                                 uint8_t* __restrict__ R, int64_t planeA, int64_t planeB,
+                                                 int64_t planeR, ModSet ms, const int* __restrict__ skip) {
+    if (*skip) return;
+    __shared__ __attribute__((aligned(16))) I8Smem<CF> sm;
+    const uint32_t u = xcd_remap8(blockIdx.x, total);
+    const int p = find_prob(pre, count, u);
+    const OzProb& P = probs[p];
+    const uint32_t local = u - pre[p];
+    const uint32_t tiles_n = (uint32_t)(P.Np / CF::BN);
+    const uint32_t tm = local / tiles_n, tn = local - tm * tiles_n;
+    const int64_t l = blockIdx.y;
+    i8_tile<CF, ROW4G>(A8 + l * planeA + P.oA, B8 + l * planeB + P.oB, R + l * planeR + P.oR, P.Mp, P.N, P.Kp, P.Kp,
+                       P.N, (int)ms.m[l], false, tm, tn, sm);
+}
+
+// CRT for a group: one thread per output element, ceil(M N / 256) blocks per problem.
+__global__ void __launch_bounds__(256) k_gcrt(const OzProb* __restrict__ probs, int count,
+                                              const uint32_t* __restrict__ pre, const uint8_t* __restrict__ R,
+                                              int64_t planeR, CrtConst cc, const int* __restrict__ eA,
+                                              const int* __restrict__ fB, double alpha, double beta, int beta_zero,
+                                              const int* __restrict__ skip) {
+    if (*skip) return;
+    const uint32_t u = blockIdx.x;
+    const int p = find_prob(pre, count, u);
+    const OzProb& P = probs[p];
+    const uint32_t idx = (u - pre[p]) * 256 + threadIdx.x;
+    if ((uint64_t)idx >= (uint64_t)P.M * (uint64_t)P.N) return;
+    const uint32_t iq = fdiv(idx, P.fdN);
+    const int64_t i = iq, j = (int64_t)(idx - iq * (uint32_t)P.N);
+    double v = crt_value(R + P.oR + i * P.N + j, planeR, cc);
+    v = ldexp(v, -(eA[P.oe + i] + fB[P.of + j]));
+    double* cp = P.c + i * P.rsc + j * P.csc;
     double out = alpha * v;
     if (!beta_zero) out += beta * (*cp);
     *cp = out;
@@ -421,13 +693,15 @@ static CrtConst make_crt(int n) {
         M *= OZ_MODULI[l];
     }
     cc.M = M;
-    cc.Mhalf = M / 2;
+    for (int k = 0; k < 4; ++k) cc.M32[k] = (uint32_t)(M >> (32 * k));
     cc.Md = (double)(uint64_t)(M >> 64) * 18446744073709551616.0 + (double)(uint64_t)M;
+    cc.invM = 1.0 / cc.Md;
     for (int l = 0; l < n; ++l) {
         u128 Ml = M / OZ_MODULI[l];
         uint64_t r = (uint64_t)(Ml % OZ_MODULI[l]);
         uint64_t inv = inv_mod(r, OZ_MODULI[l]);
-        cc.w[l] = mul_mod(Ml, inv, M);
+        const u128 w = mul_mod(Ml, inv, M);
+        for (int k = 0; k < 4; ++k) cc.w32[l][k] = (uint32_t)(w >> (32 * k));
     }
     return cc;
 }
@@ -443,28 +717,58 @@ static int ozaki_moduli_count() {
 // row 4*(l/16) + r).
 static constexpr bool I8_ROW4G = true;
 
-void ozaki_gemm_f64(rstsr_hip_ctx* ctx, const GemmProblem& P, double alpha, double beta, bool beta_zero) {
-    const int n = ozaki_moduli_count();
-    CrtConst cc = make_crt(n);
-    const double log2M = std::log2(cc.Md);
+// Moduli, CRT constants and the scale target g for the configured modulus count.
+struct OzSetup {
+    int n;
+    CrtConst cc;
+    int g;
+    ModSet ms;
+};
+
+static OzSetup ozaki_setup() {
+    OzSetup su{};
+    su.n = ozaki_moduli_count();
+    su.cc = make_crt(su.n);
+    const double log2M = std::log2(su.cc.Md);
     // Cauchy-Schwarz: (2^g (1 + d))^2 < M/2 with a 1/2-bit margin; exactness of x = rint(A 2^e) as an
     // integer-valued double and of the residue arithmetic needs g <= 60
-    int g = (int)std::floor((log2M - 1.5) / 2.0);
-    if (g > 60) g = 60;
-    ModSet ms{};
-    ms.n = n;
-    for (int l = 0; l < n; ++l) {
-        ms.m[l] = (double)cc.mod[l];
-        ms.inv[l] = 1.0 / (double)cc.mod[l];
+    su.g = (int)std::floor((log2M - 1.5) / 2.0);
+    if (su.g > 60) su.g = 60;
+    su.ms.n = su.n;
+    for (int l = 0; l < su.n; ++l) {
+        su.ms.m[l] = (double)su.cc.mod[l];
+        su.ms.inv[l] = 1.0 / (double)su.cc.mod[l];
     }
+    return su;
+}
+
+// Tile shape of the residue products. The 128 x 128 tile at two blocks per CU
+// measured faster than the 128 x 256 tile at one block per CU on every product
+// size tried (8192^3, 64 x 1024^3, 60 x 400 x 24000 x 1600): its second block
+// hides the other's modular epilogue. RSTSR_HIP_OZAKI_TILE=wide selects the
+// larger tile.
+static bool use_wide_tile(int64_t M, int64_t N) {
+    const char* s = std::getenv("RSTSR_HIP_OZAKI_TILE");
+    if (s && std::strcmp(s, "wide") == 0) return true;
+    if (s && std::strcmp(s, "square") == 0) return false;
+    return false;
+}
+
+void ozaki_gemm_f64(rstsr_hip_ctx* ctx, const GemmProblem& P, double alpha, double beta, bool beta_zero) {
+    const OzSetup su = ozaki_setup();
+    const int n = su.n;
+    const CrtConst& cc = su.cc;
+    const int g = su.g;
+    const ModSet& ms = su.ms;
     const int64_t M = P.M, N = P.N, K = P.K;
     if (K > (int64_t)1 << 40) RSTSR_RAISE(UnImplemented, "K too large for the split GEMM");
-    const int64_t Mp = (M + I8_BM - 1) / I8_BM * I8_BM, Np = (N + I8_BN - 1) / I8_BN * I8_BN;
-    const int64_t Kp = (K + I8_BK - 1) / I8_BK * I8_BK;
-    const int64_t nchunk = std::max<int64_t>(1, std::min<int64_t>(64, K / 256));
-    const int64_t chunk = (K + nchunk - 1) / nchunk;
+    const bool wide = use_wide_tile(M, N);
+    const int64_t bn = wide ? I8Wide::BN : I8Sq::BN;
+    const int64_t Mp = (M + I8_MPAD - 1) / I8_MPAD * I8_MPAD, Np = (N + bn - 1) / bn * bn;
+    const int64_t Kp = (K + I8_KPAD - 1) / I8_KPAD * I8_KPAD;
     const uint64_t nb = P.nbatch;
     if (nb >= 65536) RSTSR_RAISE(UnImplemented, "too many batches for the split GEMM");
+    if (Mp / 64 > 65535 || Np / 64 > 65535) RSTSR_RAISE(UnImplemented, "matrix too tall for the split GEMM");
     // an operand whose batch strides are all zero is split once and shared by every batch
     bool a_b = false, b_b = false;
     for (int d = 0; d < P.bd.nbd; ++d) {
@@ -472,6 +776,33 @@ void ozaki_gemm_f64(rstsr_hip_ctx* ctx, const GemmProblem& P, double alpha, doub
         if (P.bd.sb[d] != 0) b_b = true;
     }
     const uint64_t nbA = a_b ? nb : 1, nbB = b_b ? nb : 1;
+    // Vector norms, one launch per operand over all its batches: rows of A (elements
+    // along k at stride csa), columns of B (along k at stride rsb). A wave per vector
+    // when the elements are the closer ones in memory, else a lane per vector with
+    // the length split into chunks until the grid has ~2048 blocks.
+    struct NormPlan {
+        const double* x;
+        int64_t nvec, sv, st, evs;
+        uint64_t nbx;
+        int which;
+        bool rows;
+        int64_t nchunk, chunk;
+    };
+    auto plan_norm = [&](const double* x, int64_t nvec, int64_t sv, int64_t st, int64_t evs, uint64_t nbx, int which) {
+        NormPlan q{x, nvec, sv, st, evs, nbx, which, std::llabs(st) <= std::llabs(sv), 1, K};
+        if (!q.rows) {
+            const int64_t blocks = (nvec + 63) / 64 * (int64_t)nbx;
+            q.nchunk = std::max<int64_t>(1, std::min<int64_t>({(2048 + blocks - 1) / blocks, K / 64, 4096}));
+            q.chunk = (K + q.nchunk - 1) / q.nchunk;
+            q.nchunk = (K + q.chunk - 1) / q.chunk;
+        }
+        return q;
+    };
+    const NormPlan qa = plan_norm((const double*)P.a, M, P.rsa, P.csa, Mp, nbA, 0);
+    const NormPlan qb = plan_norm((const double*)P.b, N, P.csb, P.rsb, Np, nbB, 1);
+    auto part_bytes = [](const NormPlan& q) {
+        return q.nchunk > 1 ? (size_t)q.nbx * q.nchunk * q.nvec * sizeof(NormAcc) : (size_t)0;
+    };
     // workspace layout: flag | eA [nbA][Mp] | fB [nbB][Np] | norm partials | A8 [l][nbA][Mp][Kp] |
     // B8 [l][nbB][Np][Kp] | R [l][nb][Mp][N]
     size_t off = 0;
@@ -488,8 +819,7 @@ void ozaki_gemm_f64(rstsr_hip_ctx* ctx, const GemmProblem& P, double alpha, doub
     off += nbB * Np * sizeof(int);
     const size_t n_zero = off / sizeof(int);
     off = (off + 255) & ~(size_t)255;
-    const size_t o_pm = take((size_t)nchunk * std::max(Mp, Np) * sizeof(double));
-    const size_t o_ps = take((size_t)nchunk * std::max(Mp, Np) * sizeof(double));
+    const size_t o_part = take(std::max(part_bytes(qa), part_bytes(qb)));
     const size_t planeA = (size_t)nbA * Mp * Kp, planeB = (size_t)nbB * Np * Kp, planeR = (size_t)nb * Mp * N;
     const size_t o_A8 = take((size_t)n * planeA);
     const size_t o_B8 = take((size_t)n * planeB);
@@ -498,67 +828,191 @@ void ozaki_gemm_f64(rstsr_hip_ctx* ctx, const GemmProblem& P, double alpha, doub
     int* flag = (int*)(ws + o_flag);
     int* eA = (int*)(ws + o_eA);
     int* fB = (int*)(ws + o_fB);
-    double* pm = (double*)(ws + o_pm);
-    double* ps = (double*)(ws + o_ps);
+    NormAcc* part = (NormAcc*)(ws + o_part);
     int8_t* A8 = (int8_t*)(ws + o_A8);
     int8_t* B8 = (int8_t*)(ws + o_B8);
     uint8_t* R = (uint8_t*)(ws + o_R);
-    const uint32_t tiles_n = (uint32_t)(Np / I8_BN), ntiles = (uint32_t)((Mp / I8_BM) * tiles_n);
+    const uint32_t tiles_n = (uint32_t)(Np / bn), ntiles = (uint32_t)((Mp / I8_MPAD) * tiles_n);
 
     hipLaunchKernelGGL(k_zero_i32, dim3(ceil_div_u32(n_zero, 256)), dim3(256), 0, ctx->stream, flag, (int64_t)n_zero);
     check_launch("ozaki zero");
-    // vector norms: rows of A (elements along k: stride csa), columns of B (along k: stride rsb)
-    auto vnorm = [&](const double* x, int64_t nvec, int64_t len, int64_t sv, int64_t st, int* ev) {
-        if (std::llabs(st) == 1 || std::llabs(sv) > std::llabs(st)) {
-            hipLaunchKernelGGL(k_vnorm_block, dim3((uint32_t)nvec), dim3(256), 0, ctx->stream, x, nvec, len, sv, st, g,
-                               ev, flag);
+    auto run_norm = [&](const NormPlan& q, int* ev) {
+        if (q.rows) {
+            hipLaunchKernelGGL(k_norm_rows, dim3(ceil_div_u32(q.nvec, 4), (uint32_t)q.nbx), dim3(256), 0, ctx->stream,
+                               q.x, q.nvec, K, q.sv, q.st, P.bd, q.which, q.evs, g, ev, flag);
         } else {
-            dim3 grid(ceil_div_u32(nvec, 256), (uint32_t)nchunk);
-            hipLaunchKernelGGL(k_vnorm_cols_max, grid, dim3(256), 0, ctx->stream, x, nvec, len, sv, st, chunk, pm, flag);
-            hipLaunchKernelGGL(k_vnorm_cols_ss, grid, dim3(256), 0, ctx->stream, x, nvec, len, sv, st, chunk, nchunk, pm,
-                               ps);
-            hipLaunchKernelGGL(k_vnorm_cols_fin, dim3(ceil_div_u32(nvec, 256)), dim3(256), 0, ctx->stream, nvec, nchunk,
-                               pm, ps, g, ev);
+            hipLaunchKernelGGL(k_norm_cols, dim3(ceil_div_u32(q.nvec, 64), (uint32_t)q.nchunk, (uint32_t)q.nbx),
+                               dim3(256), 0, ctx->stream, q.x, q.nvec, K, q.sv, q.st, q.chunk, P.bd, q.which, q.evs, g,
+                               part, ev, flag);
+            if (q.nchunk > 1)
+                hipLaunchKernelGGL(k_norm_fin, dim3(ceil_div_u32(q.nvec, 256), (uint32_t)q.nbx), dim3(256), 0,
+                                   ctx->stream, part, q.nvec, (int)q.nchunk, q.evs, g, ev);
         }
         check_launch("ozaki norms");
     };
-    for (uint64_t b = 0; b < nbA; ++b) {
-        int64_t oa, ob, oc;
-        batch_offsets_host(P.bd, b, oa, ob, oc);
-        const double* A = (const double*)P.a + oa;
-        vnorm(A, M, K, P.rsa, P.csa, eA + b * Mp);
-        dim3 ga((uint32_t)(Kp / 64), (uint32_t)(Mp / 64));
-        hipLaunchKernelGGL(k_split, ga, dim3(256), 0, ctx->stream, A, M, K, P.rsa, P.csa, eA + b * Mp, Mp, Kp,
-                           A8 + b * Mp * Kp, ms, std::llabs(P.csa) == 1 ? 1 : 0, flag, (int64_t)planeA);
-        check_launch("ozaki split A");
-    }
-    for (uint64_t b = 0; b < nbB; ++b) {
-        int64_t oa, ob, oc;
-        batch_offsets_host(P.bd, b, oa, ob, oc);
-        const double* B = (const double*)P.b + ob;
-        vnorm(B, N, K, P.csb, P.rsb, fB + b * Np);
-        dim3 gb((uint32_t)(Kp / 64), (uint32_t)(Np / 64));
-        hipLaunchKernelGGL(k_split, gb, dim3(256), 0, ctx->stream, B, N, K, P.csb, P.rsb, fB + b * Np, Np, Kp,
-                           B8 + b * Np * Kp, ms, std::llabs(P.rsb) == 1 ? 1 : 0, flag, (int64_t)planeB);
-        check_launch("ozaki split B");
-    }
+    run_norm(qa, eA);
+    hipLaunchKernelGGL(k_split, dim3((uint32_t)(Kp / 64), (uint32_t)(Mp / 64), (uint32_t)nbA), dim3(256), 0,
+                       ctx->stream, (const double*)P.a, M, K, P.rsa, P.csa, P.bd, 0, eA, Mp, Kp, A8, ms,
+                       std::llabs(P.csa) == 1 ? 1 : 0, flag, (int64_t)planeA);
+    check_launch("ozaki split A");
+    run_norm(qb, fB);  // reuses the partials buffer after the A norms (stream order)
+    hipLaunchKernelGGL(k_split, dim3((uint32_t)(Kp / 64), (uint32_t)(Np / 64), (uint32_t)nbB), dim3(256), 0,
+                       ctx->stream, (const double*)P.b, N, K, P.csb, P.rsb, P.bd, 1, fB, Np, Kp, B8, ms,
+                       std::llabs(P.rsb) == 1 ? 1 : 0, flag, (int64_t)planeB);
+    check_launch("ozaki split B");
     // Non-finite inputs set *flag: the split kernels exit at once and the plain
     // FP64 kernel (which runs only when *flag != 0) computes C with IEEE special
     // values. No host round trip, so the sequence can be graph-captured.
     plain_dgemm(ctx, P, alpha, beta, beta_zero, flag);
     for (int l = 0; l < n; ++l) {
-        hipLaunchKernelGGL(k_i8gemm_mod<I8_ROW4G>, dim3(ntiles, (uint32_t)nb), dim3(256), 0, ctx->stream,
-                           A8 + (size_t)l * planeA, B8 + (size_t)l * planeB, R + (size_t)l * planeR, Mp, N, Kp, N,
-                           (int)cc.mod[l], tiles_n, ntiles, a_b ? Mp * Kp : (int64_t)0, b_b ? Np * Kp : (int64_t)0,
-                           Mp * N, flag);
-        check_launch("ozaki int8 gemm");
+        // K beyond I8_KCHUNK: further launches add their residues into R
+        for (int64_t k0 = 0; k0 < Kp; k0 += I8_KCHUNK) {
+            const int64_t Kc = std::min<int64_t>(I8_KCHUNK, Kp - k0);
+            auto go = [&](auto cfg) {
+                using CF = decltype(cfg);
+                hipLaunchKernelGGL((k_i8gemm_mod<CF, I8_ROW4G>), dim3(ntiles, (uint32_t)nb), dim3(CF::NT), 0,
+                                   ctx->stream, A8 + (size_t)l * planeA + k0, B8 + (size_t)l * planeB + k0,
+                                   R + (size_t)l * planeR, Mp, N, Kp, Kc, N, (int)cc.mod[l], k0 > 0 ? 1 : 0,
+                                   (uint32_t)(Mp / I8_MPAD), tiles_n, a_b ? Mp * Kp : (int64_t)0,
+                                   b_b ? Np * Kp : (int64_t)0, Mp * N, flag);
+            };
+            if (wide) go(I8Wide{});
+            else go(I8Sq{});
+            check_launch("ozaki int8 gemm");
+        }
     }
-    uint64_t tot = (uint64_t)M * N * nb;
-    if (tot >= (1ull << 40)) RSTSR_RAISE(UnImplemented, "split GEMM output too large");
-    hipLaunchKernelGGL(k_crt, dim3(ceil_div_u32(tot, 256)), dim3(256), 0, ctx->stream, R, M, N, Mp, Np, nb,
-                       (int64_t)planeR, a_b ? 1 : 0, b_b ? 1 : 0, P.bd, cc, eA, fB, (double*)P.c, P.rsc, P.csc, alpha,
-                       beta, beta_zero ? 1 : 0, flag);
+    // four columns per thread when every row and plane of R starts on a 4-byte boundary
+    auto crt = [&](auto v) {
+        constexpr int V = decltype(v)::value;
+        hipLaunchKernelGGL((k_crt<V>), dim3(ceil_div_u32(N, 256 * V), (uint32_t)std::min<int64_t>(M, 65535), (uint32_t)nb),
+                           dim3(256), 0, ctx->stream, R, M, N, Mp, Np, (int64_t)planeR, a_b ? 1 : 0, b_b ? 1 : 0, P.bd,
+                           cc, eA, fB, (double*)P.c, P.rsc, P.csc, alpha, beta, beta_zero ? 1 : 0, flag);
+    };
+    if (N % 4 == 0) crt(std::integral_constant<int, 4>{});
+    else crt(std::integral_constant<int, 1>{});
     check_launch("ozaki crt");
+}
+
+void ozaki_grouped_f64(rstsr_hip_ctx* ctx, const std::vector<GemmProblem>& probs, double alpha, double beta,
+                       bool beta_zero) {
+    // a K longer than one INT8 launch can sum goes through the single-problem path
+    std::vector<GemmProblem> keep;
+    for (const GemmProblem& P : probs) {
+        if ((P.K + I8_KPAD - 1) / I8_KPAD * I8_KPAD > I8_KCHUNK) ozaki_gemm_f64(ctx, P, alpha, beta, beta_zero);
+        else keep.push_back(P);
+    }
+    if (keep.size() != probs.size()) return ozaki_grouped_f64(ctx, keep, alpha, beta, beta_zero);
+    if (probs.empty()) return;
+    const OzSetup su = ozaki_setup();
+    const int n = su.n;
+    const int count = (int)probs.size();
+    if (probs.size() >= (1u << 30)) RSTSR_RAISE(UnImplemented, "too many problems for the grouped split GEMM");
+    std::vector<OzProb> op(probs.size());
+    std::vector<uint32_t> pre((size_t)4 * (count + 1));
+    uint32_t* pn = pre.data();
+    uint32_t* psp = pn + (count + 1);
+    uint32_t* pg = psp + (count + 1);
+    uint32_t* pc = pg + (count + 1);
+    uint64_t tn = 0, tsp = 0, tg = 0, tc = 0;
+    int64_t sA = 0, sB = 0, sR = 0, se = 0, sf = 0;
+    for (int i = 0; i < count; ++i) {
+        const GemmProblem& P = probs[i];
+        if (P.M <= 0 || P.N <= 0 || P.K <= 0 || P.nbatch != 1)
+            RSTSR_RAISE(InvalidValue, "grouped split GEMM needs non-empty unbatched problems");
+        if (P.K > (int64_t)1 << 40) RSTSR_RAISE(UnImplemented, "K too large for the split GEMM");
+        if ((uint64_t)P.M * (uint64_t)P.N + 255 >= (1ull << 32))
+            RSTSR_RAISE(UnImplemented, "grouped split GEMM problem too large; run it on its own");
+        OzProb& o = op[i];
+        o.fdN = FastDiv((uint32_t)P.N);
+        o.M = P.M;
+        o.N = P.N;
+        o.K = P.K;
+        o.Mp = (P.M + I8_MPAD - 1) / I8_MPAD * I8_MPAD;
+        o.Np = (P.N + I8Sq::BN - 1) / I8Sq::BN * I8Sq::BN;
+        o.Kp = (P.K + I8_KPAD - 1) / I8_KPAD * I8_KPAD;
+        o.a = (const double*)P.a;
+        o.rsa = P.rsa;
+        o.csa = P.csa;
+        o.b = (const double*)P.b;
+        o.rsb = P.rsb;
+        o.csb = P.csb;
+        o.c = (double*)P.c;
+        o.rsc = P.rsc;
+        o.csc = P.csc;
+        o.oA = sA;
+        o.oB = sB;
+        o.oR = sR;
+        o.oe = se;
+        o.of = sf;
+        sA += o.Mp * o.Kp;
+        sB += o.Np * o.Kp;
+        sR += o.Mp * o.N;
+        se += o.Mp;
+        sf += o.Np;
+        pn[i] = (uint32_t)tn;
+        tn += (uint64_t)((P.M + 63) / 64 + (P.N + 63) / 64);
+        psp[i] = (uint32_t)tsp;
+        tsp += (uint64_t)((o.Mp + o.Np) / 64) * (uint64_t)(o.Kp / 64);
+        pg[i] = (uint32_t)tg;
+        tg += (uint64_t)(o.Mp / I8Sq::BM) * (uint64_t)(o.Np / I8Sq::BN);
+        pc[i] = (uint32_t)tc;
+        tc += ((uint64_t)P.M * (uint64_t)P.N + 255) / 256;
+        if (tn >= (1ull << 31) || tsp >= (1ull << 31) || tg >= (1ull << 31) || tc >= (1ull << 31))
+            RSTSR_RAISE(UnImplemented, "grouped split GEMM too large for one launch");
+    }
+    pn[count] = (uint32_t)tn;
+    psp[count] = (uint32_t)tsp;
+    pg[count] = (uint32_t)tg;
+    pc[count] = (uint32_t)tc;
+    // workspace: flag | eA [se] | fB [sf] | problems | prefix tables | A8 [l][sA] | B8 [l][sB] | R [l][sR]
+    size_t off = sizeof(int);
+    const size_t o_eA = off;
+    off += (size_t)se * sizeof(int);
+    const size_t o_fB = off;
+    off += (size_t)sf * sizeof(int);
+    const size_t n_zero = off / sizeof(int);
+    auto take = [&](size_t bytes) {
+        off = (off + 255) & ~(size_t)255;
+        size_t o = off;
+        off += bytes;
+        return o;
+    };
+    const size_t o_pr = take(op.size() * sizeof(OzProb));
+    const size_t o_pre = take(pre.size() * sizeof(uint32_t));
+    const size_t o_A8 = take((size_t)n * sA);
+    const size_t o_B8 = take((size_t)n * sB);
+    const size_t o_R = take((size_t)n * sR);
+    char* ws = (char*)workspace(ctx, off);
+    int* flag = (int*)ws;
+    int* eA = (int*)(ws + o_eA);
+    int* fB = (int*)(ws + o_fB);
+    const OzProb* dprobs = (const OzProb*)(ws + o_pr);
+    const uint32_t* dpre = (const uint32_t*)(ws + o_pre);
+    int8_t* A8 = (int8_t*)(ws + o_A8);
+    int8_t* B8 = (int8_t*)(ws + o_B8);
+    uint8_t* R = (uint8_t*)(ws + o_R);
+    HIP_CHECK(hipMemcpyAsync(ws + o_pr, op.data(), op.size() * sizeof(OzProb), hipMemcpyHostToDevice, ctx->stream));
+    HIP_CHECK(hipMemcpyAsync(ws + o_pre, pre.data(), pre.size() * sizeof(uint32_t), hipMemcpyHostToDevice, ctx->stream));
+    hipLaunchKernelGGL(k_zero_i32, dim3(ceil_div_u32(n_zero, 256)), dim3(256), 0, ctx->stream, flag, (int64_t)n_zero);
+    check_launch("grouped ozaki zero");
+    hipLaunchKernelGGL(k_gnorm, dim3((uint32_t)tn), dim3(256), 0, ctx->stream, dprobs, count, dpre + 0 * (count + 1),
+                       su.g, eA, fB, flag);
+    check_launch("grouped ozaki norms");
+    hipLaunchKernelGGL(k_gsplit, dim3((uint32_t)tsp), dim3(256), 0, ctx->stream, dprobs, count, dpre + 1 * (count + 1),
+                       eA, fB, A8, B8, (int64_t)sA, (int64_t)sB, su.ms, flag);
+    check_launch("grouped ozaki split");
+    // non-finite inputs anywhere in the group: the plain grouped kernel computes every product
+    plain_grouped_f64(ctx, probs, alpha, beta, beta_zero, flag);
+    hipLaunchKernelGGL((k_gi8gemm<I8Sq, I8_ROW4G>), dim3((uint32_t)tg, (uint32_t)n), dim3(I8Sq::NT), 0, ctx->stream,
+                       dprobs, count,
+                       dpre + 2 * (count + 1), (uint32_t)tg, A8, B8, R, (int64_t)sA, (int64_t)sB, (int64_t)sR, su.ms,
+                       flag);
+    check_launch("grouped ozaki int8 gemm");
+    hipLaunchKernelGGL(k_gcrt, dim3((uint32_t)tc), dim3(256), 0, ctx->stream, dprobs, count, dpre + 3 * (count + 1), R,
+                       (int64_t)sR, su.cc, eA, fB, alpha, beta, beta_zero ? 1 : 0, flag);
+    check_launch("grouped ozaki crt");
+    // the host tables die with this call: wait for the copies
+    HIP_CHECK(hipStreamSynchronize(ctx->stream));
 }
 
 }  // namespace hip

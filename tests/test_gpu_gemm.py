@@ -258,6 +258,71 @@ def test_grouped_alpha_beta(dev, oracle):
     assert np.max(np.abs(tc.to_numpy() - ref)) < 1e-13 * np.max(np.abs(ref))
 
 
+def _grouped_views(rng, g, m, n, k):
+    """Operands of problem g in one of several layouts (row/column major, reversed, sliced)."""
+    kind = g % 5
+    if kind == 0:
+        A, B = wide_exponent(rng, (m, k)), wide_exponent(rng, (k, n))
+    elif kind == 1:
+        A, B = wide_exponent(rng, (k, m)).T, wide_exponent(rng, (n, k)).T
+    elif kind == 2:
+        A, B = wide_exponent(rng, (m, k))[:, ::-1], wide_exponent(rng, (k, n))[::-1]
+    elif kind == 3:
+        A, B = wide_exponent(rng, (2 * m, 3 * k))[::2, ::3], wide_exponent(rng, (k, 2 * n))[:, 1::2]
+    else:
+        A, B = wide_exponent(rng, (k, m))[::-1].T, wide_exponent(rng, (k, n))
+    return A, B
+
+
+def test_grouped_ozaki_layouts(dev, oracle):
+    """Grouped split GEMM (one launch per stage over the whole group): odd sizes including
+    1 and K = 0, wide exponents, every operand layout, alpha/beta; normwise 1e-14 per problem."""
+    rng = np.random.default_rng(21)
+    sizes = [(1, 1, 1), (1, 7, 3), (5, 1, 9), (33, 17, 0), (129, 65, 257), (8, 300, 40), (200, 3, 131)]
+    sizes += [tuple(int(x) for x in rng.integers(1, 300, size=3)) for _ in range(25)]
+    probs, hosts = [], []
+    for g, (m, n, k) in enumerate(sizes):
+        A, B = _grouped_views(rng, g, m, n, k)
+        C0 = rng.normal(size=(m, n))
+        ta, _ = from_np(dev, A)
+        tb, _ = from_np(dev, B)
+        probs.append((ta, tb, rt.asarray(C0, device=dev)))
+        hosts.append((A, B, C0))
+    rt.gemm_grouped(probs, mode="ozaki", alpha=1.5, beta=-0.5)
+    for (A, B, C0), (_, _, tc) in zip(hosts, probs):
+        got = tc.to_numpy()
+        if A.shape[1] == 0:
+            assert np.array_equal(got, -0.5 * C0)
+            continue
+        ab = oracle.gemm_dot2(np.ascontiguousarray(A), np.ascontiguousarray(B))
+        ref = 1.5 * ab - 0.5 * C0
+        # split-GEMM error of A B (normwise), plus the rounding of the alpha/beta update
+        bound = 1e-14 * 1.5 * np.max(np.abs(ab)) + 2.0 ** -52 * (np.abs(1.5 * ab) + np.abs(0.5 * C0))
+        assert np.all(np.abs(got - ref) <= bound), (A.shape, B.shape)
+
+
+def test_grouped_ozaki_nonfinite_falls_back(dev, oracle):
+    """A NaN or Inf anywhere in the group sends the whole group to the plain FP64 kernel: IEEE
+    special values propagate as in the plain product and finite problems stay accurate."""
+    rng = np.random.default_rng(22)
+    shapes = [(40, 24, 56), (17, 33, 9), (64, 64, 64)]
+    hosts = [(rng.normal(size=(m, k)), rng.normal(size=(k, n))) for m, n, k in shapes]
+    hosts[1][0][3, 4] = np.nan
+    hosts[2][1][5, 6] = np.inf
+    probs = [(rt.asarray(A, device=dev), rt.asarray(B, device=dev), rt.empty((A.shape[0], B.shape[1]), np.float64,
+                                                                             device=dev)) for A, B in hosts]
+    rt.gemm_grouped(probs, mode="ozaki")
+    for (A, B), (_, _, tc) in zip(hosts, probs):
+        got = tc.to_numpy()
+        with np.errstate(invalid="ignore"):
+            ref = A @ B
+        assert np.array_equal(np.isnan(got), np.isnan(ref))
+        assert np.array_equal(np.isinf(got), np.isinf(ref))
+        fin = np.isfinite(ref)
+        if fin.any():
+            assert np.max(np.abs(got[fin] - ref[fin])) <= 1e-13 * np.max(np.abs(ref[fin]))
+
+
 @pytest.mark.parametrize("shape", [(64, 96, 80), (300, 130, 257)])
 def test_zgemm_ozaki(dev, shape):
     m, k, n = shape
