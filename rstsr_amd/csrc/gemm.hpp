@@ -30,6 +30,25 @@ struct GemmProblem {
     uint64_t nbatch = 1;
 };
 
+__device__ __forceinline__ uint32_t xcd_remap8(uint32_t t, uint32_t ntiles) {
+    if (ntiles % 8 != 0) return t;
+    return (t % 8) * (ntiles / 8) + t / 8;
+}
+
+// Block id -> output tile. Ids are round-robined over the 8 XCDs, so each XCD
+// first gets a contiguous range of tiles; inside it, tiles run down groups of
+// GM tile rows, so the blocks resident on one XCD at a time share GM row panels
+// of A and a few column panels of B in that XCD's L2.
+template <int GM>
+__device__ __forceinline__ void tile_coords(uint32_t id, uint32_t tiles_m, uint32_t tiles_n, uint32_t& tm,
+                                            uint32_t& tn) {
+    const uint32_t t = xcd_remap8(id, tiles_m * tiles_n);
+    const uint32_t per = GM * tiles_n, grp = t / per, first = grp * GM;
+    const uint32_t gsz = min((uint32_t)GM, tiles_m - first), r = t - grp * per;
+    tm = first + r % gsz;
+    tn = r / gsz;
+}
+
 // Device offsets of batch `bi` (innermost batch axis first).
 __device__ __forceinline__ void batch_offsets(const BatchDesc& bd, uint32_t bi, int64_t& oa, int64_t& ob, int64_t& oc) {
     oa = ob = oc = 0;

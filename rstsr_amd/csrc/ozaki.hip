@@ -266,32 +266,19 @@ struct I8Cfg {
 };
 // large products: 64 x 128 per wave
 using I8Wide = I8Cfg<128, 256, 64, 2, 2, 1, 32>;
+// large products, eight waves of 64 x 64, two blocks per CU
+using I8Wide8 = I8Cfg<128, 256, 64, 2, 4, 2, 32>;
+// the square tile with 32-byte row padding
+using I8Sq32 = I8Cfg<128, 128, 128, 2, 2, 2, 32>;
 // small and grouped products: 64 x 64 per wave, two blocks per CU
 using I8Sq = I8Cfg<128, 128, 128, 2, 2, 2, 16>;
 // planes: rows padded to I8_MPAD (every BM divides it), K to I8_KPAD (every BK divides it)
 constexpr int I8_MPAD = 128, I8_KPAD = 128;
 // one launch sums at most I8_KCHUNK products per output: |sum| <= 2^16 * 2^14 < 2^31
 constexpr int64_t I8_KCHUNK = 65536;
-static_assert(I8Wide::BM == I8_MPAD && I8Sq::BM == I8_MPAD, "tiles_m = Mp / I8_MPAD");
-
-__device__ __forceinline__ uint32_t xcd_remap8(uint32_t t, uint32_t ntiles) {
-    if (ntiles % 8 != 0) return t;
-    return (t % 8) * (ntiles / 8) + t / 8;
-}
-
-// Block id -> output tile. Ids are round-robined over the 8 XCDs, so each XCD
-// first gets a contiguous range of tiles; inside it, tiles run down groups of
-// GM tile rows, so the blocks resident on one XCD at a time share GM row panels
-// of A and a few column panels of B in that XCD's L2.
-template <int GM>
-__device__ __forceinline__ void tile_coords(uint32_t id, uint32_t tiles_m, uint32_t tiles_n, uint32_t& tm,
-                                            uint32_t& tn) {
-    const uint32_t t = xcd_remap8(id, tiles_m * tiles_n);
-    const uint32_t per = GM * tiles_n, grp = t / per, first = grp * GM;
-    const uint32_t gsz = min((uint32_t)GM, tiles_m - first), r = t - grp * per;
-    tm = first + r % gsz;
-    tn = r / gsz;
-}
+static_assert(I8Wide::BM == I8_MPAD && I8Wide8::BM == I8_MPAD && I8Sq::BM == I8_MPAD && I8Sq32::BM == I8_MPAD,
+              "tiles_m = Mp / I8_MPAD");
+static_assert(I8Wide::BN == 256 && I8Wide8::BN == 256 && I8Sq::BN == 128 && I8Sq32::BN == 128, "column padding");
 
 // x mod m in [0, m) for |x| < 2^31, m <= 256, inv = 1/m: x * inv is within
 // 2^-21 of x / m, so its floor is the quotient or (x a multiple of m rounding
@@ -742,16 +729,23 @@ static OzSetup ozaki_setup() {
     return su;
 }
 
-// Tile shape of the residue products. The 128 x 128 tile at two blocks per CU
-// measured faster than the 128 x 256 tile at one block per CU on every product
-// size tried (8192^3, 64 x 1024^3, 60 x 400 x 24000 x 1600): its second block
-// hides the other's modular epilogue. RSTSR_HIP_OZAKI_TILE=wide selects the
-// larger tile.
-static bool use_wide_tile(int64_t M, int64_t N) {
+// Tile shape of the residue products. Measured at 8192^3, 64 x 1024^3 and
+// 60 x 400 x 24000 x 1600: 128 x 256 from eight 64 x 64 waves at two blocks
+// per CU is fastest (a second resident block hides the other's modular
+// epilogue and barrier waits); 128 x 128 from four waves is next, and the same
+// block from four 64 x 128 waves at one block per CU last. Outputs at most 128
+// wide keep the 128 x 128 tile (no padding of N to 256). RSTSR_HIP_OZAKI_TILE
+// (square, square32, wide, wide8) overrides the choice.
+enum class I8Tile { Sq, Sq32, Wide, Wide8 };
+
+static I8Tile pick_i8_tile(int64_t N) {
     const char* s = std::getenv("RSTSR_HIP_OZAKI_TILE");
-    if (s && std::strcmp(s, "wide") == 0) return true;
-    if (s && std::strcmp(s, "square") == 0) return false;
-    return false;
+    if (!s || !*s) return N > 128 ? I8Tile::Wide8 : I8Tile::Sq;
+    if (std::strcmp(s, "square") == 0) return I8Tile::Sq;
+    if (std::strcmp(s, "square32") == 0) return I8Tile::Sq32;
+    if (std::strcmp(s, "wide") == 0) return I8Tile::Wide;
+    if (std::strcmp(s, "wide8") == 0) return I8Tile::Wide8;
+    RSTSR_RAISE(InvalidValue, "RSTSR_HIP_OZAKI_TILE must be square, square32, wide or wide8");
 }
 
 void ozaki_gemm_f64(rstsr_hip_ctx* ctx, const GemmProblem& P, double alpha, double beta, bool beta_zero) {
@@ -762,8 +756,8 @@ void ozaki_gemm_f64(rstsr_hip_ctx* ctx, const GemmProblem& P, double alpha, doub
     const ModSet& ms = su.ms;
     const int64_t M = P.M, N = P.N, K = P.K;
     if (K > (int64_t)1 << 40) RSTSR_RAISE(UnImplemented, "K too large for the split GEMM");
-    const bool wide = use_wide_tile(M, N);
-    const int64_t bn = wide ? I8Wide::BN : I8Sq::BN;
+    const I8Tile tile = pick_i8_tile(N);
+    const int64_t bn = (tile == I8Tile::Wide || tile == I8Tile::Wide8) ? 256 : 128;
     const int64_t Mp = (M + I8_MPAD - 1) / I8_MPAD * I8_MPAD, Np = (N + bn - 1) / bn * bn;
     const int64_t Kp = (K + I8_KPAD - 1) / I8_KPAD * I8_KPAD;
     const uint64_t nb = P.nbatch;
@@ -876,8 +870,12 @@ void ozaki_gemm_f64(rstsr_hip_ctx* ctx, const GemmProblem& P, double alpha, doub
                                    (uint32_t)(Mp / I8_MPAD), tiles_n, a_b ? Mp * Kp : (int64_t)0,
                                    b_b ? Np * Kp : (int64_t)0, Mp * N, flag);
             };
-            if (wide) go(I8Wide{});
-            else go(I8Sq{});
+            switch (tile) {
+                case I8Tile::Wide: go(I8Wide{}); break;
+                case I8Tile::Wide8: go(I8Wide8{}); break;
+                case I8Tile::Sq32: go(I8Sq32{}); break;
+                default: go(I8Sq{}); break;
+            }
             check_launch("ozaki int8 gemm");
         }
     }

@@ -396,6 +396,64 @@ def test_ri_contraction_small(dev, oracle):
         assert np.max(np.abs(got - ref)) <= 1e-14 * np.max(np.abs(ref)), mode
 
 
+@pytest.mark.parametrize("dtype", [np.float64, np.complex128])
+@pytest.mark.parametrize("m,k", [(300, 77), (64, 64), (129, 200)])
+def test_syrk_shortcut_bit_identical(dev, oracle, dtype, m, k):
+    """A = X^T and B = X in the same buffer with beta = 0 take the symmetric path (lower tiles plus
+    a mirror, device_faer/matmul.rs:38-48). Its lower triangle equals, bit for bit, the same
+    product with B in a separate buffer (full tile grid); the result is exactly symmetric; for
+    real types the whole matrix is bit-identical (the mirrored sums are the same sums)."""
+    rng = np.random.default_rng(m + k)
+    X = rng.normal(size=(k, m))
+    if dtype == np.complex128:
+        X = X + 1j * rng.normal(size=(k, m))
+    tX = rt.asarray(X, device=dev)
+    tY = rt.asarray(X.copy(), device=dev)
+    sym = rt.matmul(tX.T, tX).to_numpy()
+    full = rt.matmul(tX.T, tY).to_numpy()
+    low = np.tril_indices(m)
+    assert np.array_equal(sym[low].view(np.uint8), full[low].view(np.uint8))
+    assert np.array_equal(sym, sym.T)
+    if dtype == np.float64:
+        assert np.array_equal(sym.view(np.uint8), full.view(np.uint8))
+    else:
+        assert np.max(np.abs(sym - full)) <= 4 * k * 2.0 ** -53 * np.max(np.abs(full))
+    if dtype == np.float64:
+        ref = oracle.gemm_dot2(np.ascontiguousarray(X.T), np.ascontiguousarray(X))
+        ok, worst = comp_bound_ok(sym, X.T, X, ref, 2.0 ** -53)
+        assert ok, worst
+
+
+def test_syrk_batched(dev):
+    """Batched symmetric products (equal batch strides) on a partial last tile."""
+    rng = np.random.default_rng(5)
+    X = rng.normal(size=(3, 50, 140))
+    tX = rt.asarray(X, device=dev)
+    tY = rt.asarray(X.copy(), device=dev)
+    sym = rt.matmul(tX.permute([0, 2, 1]), tX).to_numpy()
+    full = rt.matmul(tX.permute([0, 2, 1]), tY).to_numpy()
+    assert np.array_equal(sym, full)
+    ref = np.einsum("bki,bkj->bij", X.astype(np.longdouble), X.astype(np.longdouble))
+    assert np.max(np.abs(sym - ref)) <= 1e-14 * np.max(np.abs(ref))
+
+
+def test_batch_fold_matches_unfolded(dev, oracle):
+    """A broadcast B with A and C batches that chain into their rows runs as one taller GEMM;
+    the result must equal per-batch products (each batch computed alone)."""
+    rng = np.random.default_rng(8)
+    A = rng.normal(size=(5, 37, 64))
+    B = rng.normal(size=(64, 90))
+    tA = rt.asarray(A, device=dev)
+    tB = rt.asarray(B, device=dev)
+    got = rt.matmul(tA, tB).to_numpy()
+    for b in range(5):
+        one = rt.matmul(rt.asarray(A[b], device=dev), tB).to_numpy()
+        ref = oracle.gemm_dot2(A[b], B)
+        ok, worst = comp_bound_ok(got[b], A[b], B, ref, 2.0 ** -53)
+        assert ok, worst
+        assert np.max(np.abs(one - got[b])) <= 64 * 2.0 ** -52 * np.max(np.abs(ref))
+
+
 def test_free_during_capture_is_deferred(dev):
     """Dropping a pre-capture tensor inside a capture must not break the capture."""
     a = rt.asarray(np.arange(64.0).reshape(8, 8), device=dev)
