@@ -143,6 +143,50 @@ __device__ __forceinline__ uint32_t fdiv(uint32_t n, const FastDiv& f) {
     return (uint32_t)(((uint64_t)t + n) >> f.s);
 }
 
+/* ------------------------------ streaming loads ------------------------------ */
+
+// Load of an element the launch reads exactly once: the nontemporal hint lets it
+// stream past the caches. Measured on MI355X, 2 GiB read-only sweeps sustain 7.1
+// TB/s with it and 6.3 TB/s without. Operands that are re-read (broadcast axes)
+// keep ordinary loads.
+template <class T>
+__device__ __forceinline__ T ld_nt(const T* p) {
+    if constexpr (std::is_arithmetic<T>::value) {
+        return __builtin_nontemporal_load(p);
+    } else if constexpr (sizeof(T) == 16 && alignof(T) >= 16) {
+        typedef unsigned u4 __attribute__((ext_vector_type(4)));
+        u4 w = __builtin_nontemporal_load((const u4*)p);
+        T v;
+        __builtin_memcpy(&v, &w, 16);
+        return v;
+    } else if constexpr (sizeof(T) % 8 == 0 && alignof(T) >= 8) {
+        typedef unsigned u2 __attribute__((ext_vector_type(2)));
+        T v;
+        const u2* q = (const u2*)p;
+#pragma unroll
+        for (size_t i = 0; i < sizeof(T) / 8; ++i) {
+            u2 w = __builtin_nontemporal_load(q + i);
+            __builtin_memcpy((char*)&v + 8 * i, &w, 8);
+        }
+        return v;
+    } else {
+        static_assert(sizeof(T) % 4 == 0 && alignof(T) >= 4, "streaming load of an odd-sized element");
+        T v;
+        const unsigned* q = (const unsigned*)p;
+        unsigned* d = (unsigned*)&v;
+#pragma unroll
+        for (size_t i = 0; i < sizeof(T) / 4; ++i) d[i] = __builtin_nontemporal_load(q + i);
+        return v;
+    }
+}
+
+// Cached or streaming load chosen at compile time.
+template <bool NT, class T>
+__device__ __forceinline__ T ld(const T* p) {
+    if constexpr (NT) return ld_nt(p);
+    else return *p;
+}
+
 /* ------------------------------ launch helpers ------------------------------ */
 
 inline void check_launch(const char* what) {

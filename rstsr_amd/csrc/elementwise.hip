@@ -105,8 +105,10 @@ struct BinBody {
     const T* a;
     const T* b;
     T sa, sb;
-    __device__ __forceinline__ T la(int64_t o) const { return AS ? sa : a[o]; }
-    __device__ __forceinline__ T lb(int64_t o) const { return BS ? sb : b[o]; }
+    template <bool NT = false>
+    __device__ __forceinline__ T la(int64_t o) const { return AS ? sa : ld<NT>(a + o); }
+    template <bool NT = false>
+    __device__ __forceinline__ T lb(int64_t o) const { return BS ? sb : ld<NT>(b + o); }
     __device__ __forceinline__ void st(int64_t o, T x, T y) const { c[o] = apply_op<OP, T>(x, y); }
 };
 
@@ -116,7 +118,9 @@ struct CastBody {
     using TB = None;
     TCv* c;
     const TAv* a;
-    __device__ __forceinline__ TAv la(int64_t o) const { return a[o]; }
+    template <bool NT = false>
+    __device__ __forceinline__ TAv la(int64_t o) const { return ld<NT>(a + o); }
+    template <bool NT = false>
     __device__ __forceinline__ None lb(int64_t) const { return None{}; }
     __device__ __forceinline__ void st(int64_t o, TAv x, None) const { c[o] = cast_to<TCv, TAv>(x); }
 };
@@ -127,7 +131,9 @@ struct FillBody {
     using TB = None;
     T* c;
     T v;
+    template <bool NT = false>
     __device__ __forceinline__ None la(int64_t) const { return None{}; }
+    template <bool NT = false>
     __device__ __forceinline__ None lb(int64_t) const { return None{}; }
     __device__ __forceinline__ void st(int64_t o, None, None) const { c[o] = v; }
 };
@@ -163,8 +169,10 @@ struct CloseBody {
     int* flag;
     double rtol, atol;
     int equal_nan;
-    __device__ __forceinline__ T la(int64_t o) const { return a[o]; }
-    __device__ __forceinline__ T lb(int64_t o) const { return b[o]; }
+    template <bool NT = false>
+    __device__ __forceinline__ T la(int64_t o) const { return ld<NT>(a + o); }
+    template <bool NT = false>
+    __device__ __forceinline__ T lb(int64_t o) const { return ld<NT>(b + o); }
     __device__ __forceinline__ void st(int64_t, T x, T y) const {
         bool ok = abs_diff_d<T>(x, y) <= atol + rtol * abs_d<T>(y);
         if (equal_nan && is_nan_v<T>(x) && is_nan_v<T>(y)) ok = true;
@@ -180,6 +188,7 @@ struct EwPlan {
     FastDiv fd[MAXD];
     int64_t st[3][MAXD];  // element strides of c, a, b
     int64_t off[3];       // element offsets of c, a, b
+    int nt[3];            // operand read once per launch (no zero stride on a longer axis): streaming loads
 };
 
 // The index math of one row/tile/element: offsets of (c, a, b) for a linear
@@ -243,14 +252,22 @@ __global__ void __launch_bounds__(EW_BLOCK) k_ew_rows(Body body, EwPlan p, uint3
     uint32_t base = chunk * (EW_BLOCK * EW_VEC) + threadIdx.x;
     TA va[EW_VEC];
     TB vb[EW_VEC];
+    auto load = [&](auto nta, auto ntb) {
 #pragma unroll
-    for (int v = 0; v < EW_VEC; ++v) {
-        uint32_t i = base + v * EW_BLOCK;
-        if (i < L) {
-            va[v] = body.la(ra + (int64_t)i * sa);
-            vb[v] = body.lb(rb + (int64_t)i * sb);
+        for (int v = 0; v < EW_VEC; ++v) {
+            uint32_t i = base + v * EW_BLOCK;
+            if (i < L) {
+                va[v] = body.template la<decltype(nta)::value>(ra + (int64_t)i * sa);
+                vb[v] = body.template lb<decltype(ntb)::value>(rb + (int64_t)i * sb);
+            }
         }
-    }
+    };
+    using Y = std::true_type;
+    using N = std::false_type;
+    if (p.nt[1] && p.nt[2]) load(Y{}, Y{});
+    else if (p.nt[1]) load(Y{}, N{});
+    else if (p.nt[2]) load(N{}, Y{});
+    else load(N{}, N{});
 #pragma unroll
     for (int v = 0; v < EW_VEC; ++v) {
         uint32_t i = base + v * EW_BLOCK;
@@ -265,9 +282,11 @@ __global__ void __launch_bounds__(EW_BLOCK) k_ew_rows(Body body, EwPlan p, uint3
 // row position (phase 2) fixed, so per-element addressing is a constant stride
 // added to a per-thread base; interior tiles skip the bounds tests; an operand
 // that is broadcast along DI is loaded once per thread in phase 2.
-template <class Body, int TI, int TC, int STAGE>
-__global__ void __launch_bounds__(EW_BLOCK) k_ew_tiled(Body body, EwPlan p, int di, uint32_t nti, uint32_t ntc,
-                                                       FastDiv fti, FastDiv ftc) {
+// ONE: at most one axis besides 0 and di (od, or -1 for none), so the outer
+// offset is one product instead of an unravel over every axis.
+template <class Body, int TI, int TC, int STAGE, bool ONE>
+__global__ void __launch_bounds__(EW_BLOCK) k_ew_tiled(Body body, EwPlan p, int di, int od, uint32_t nti,
+                                                       uint32_t ntc, FastDiv fti, FastDiv ftc) {
     using TA = typename Body::TA;
     using TB = typename Body::TB;
     constexpr bool SA = (STAGE & 1) != 0;
@@ -285,7 +304,13 @@ __global__ void __launch_bounds__(EW_BLOCK) k_ew_tiled(Body body, EwPlan p, int 
 
     // outer offsets: dims other than 0 and di, in order
     int64_t oc = p.off[0], oa = p.off[1], ob = p.off[2];
-    {
+    if constexpr (ONE) {
+        if (od >= 0) {
+            oc += (int64_t)outer * p.st[0][od];
+            oa += (int64_t)outer * p.st[1][od];
+            ob += (int64_t)outer * p.st[2][od];
+        }
+    } else {
         uint32_t lin = outer;
 #pragma unroll
         for (int d = 1; d < MAXD; ++d) {
@@ -312,11 +337,22 @@ __global__ void __launch_bounds__(EW_BLOCK) k_ew_tiled(Body body, EwPlan p, int 
         const int64_t pa = oa + (int64_t)(i0 + ii) * sa_i + (int64_t)(c0 + cb) * sa_c, da = (int64_t)ROWS1 * sa_c;
         const int64_t pb = ob + (int64_t)(i0 + ii) * sb_i + (int64_t)(c0 + cb) * sb_c, db = (int64_t)ROWS1 * sb_c;
         if (full) {
+            // staged operands are read along their unit-stride axis, once: streaming
+            // loads unless the operand repeats along another axis
+            auto stage = [&](auto nta, auto ntb) {
 #pragma unroll
-            for (int k = 0; k < PER1; ++k) {
-                if constexpr (SA) lds_a[k * ROWS1 + cb][ii] = body.la(pa + k * da);
-                if constexpr (SB) lds_b[k * ROWS1 + cb][ii] = body.lb(pb + k * db);
-            }
+                for (int k = 0; k < PER1; ++k) {
+                    if constexpr (SA) lds_a[k * ROWS1 + cb][ii] = body.template la<decltype(nta)::value>(pa + k * da);
+                    if constexpr (SB) lds_b[k * ROWS1 + cb][ii] = body.template lb<decltype(ntb)::value>(pb + k * db);
+                }
+            };
+            using Y = std::true_type;
+            using N = std::false_type;
+            const bool ya = SA && p.nt[1], yb = SB && p.nt[2];
+            if (ya && yb) stage(Y{}, Y{});
+            else if (ya) stage(Y{}, N{});
+            else if (yb) stage(N{}, Y{});
+            else stage(N{}, N{});
         } else {
 #pragma unroll
             for (int k = 0; k < PER1; ++k) {
@@ -427,7 +463,12 @@ static EwPlan to_plan(const Dims& d) {
         p.fd[i] = FastDiv((uint32_t)d.shape[i]);
         for (int k = 0; k < 3; ++k) p.st[k][i] = d.st[i][k];
     }
-    for (int k = 0; k < 3; ++k) p.off[k] = d.off[k];
+    for (int k = 0; k < 3; ++k) {
+        p.off[k] = d.off[k];
+        p.nt[k] = 1;
+        for (int i = 0; i < d.nd; ++i)
+            if (d.shape[i] > 1 && d.st[i][k] == 0) p.nt[k] = 0;
+    }
     return p;
 }
 
@@ -521,20 +562,23 @@ static void launch_ew(rstsr_hip_ctx* ctx, const Body& body, Dims d, const EwChoi
             uint64_t nouter = total / (dd.shape[0] * dd.shape[ch.di]);
             uint64_t nb = nouter * nti * ntc;
             if (nb >= LIM) RSTSR_RAISE(UnImplemented, "grid too large");
-            switch (ch.stage) {
-                case 1:
-                    hipLaunchKernelGGL((k_ew_tiled<Body, TI, TC, 1>), dim3((uint32_t)nb), dim3(EW_BLOCK), 0, ctx->stream,
-                                       body, p, ch.di, nti, ntc, FastDiv(nti), FastDiv(ntc));
-                    break;
-                case 2:
-                    hipLaunchKernelGGL((k_ew_tiled<Body, TI, TC, 2>), dim3((uint32_t)nb), dim3(EW_BLOCK), 0, ctx->stream,
-                                       body, p, ch.di, nti, ntc, FastDiv(nti), FastDiv(ntc));
-                    break;
-                default:
-                    hipLaunchKernelGGL((k_ew_tiled<Body, TI, TC, 3>), dim3((uint32_t)nb), dim3(EW_BLOCK), 0, ctx->stream,
-                                       body, p, ch.di, nti, ntc, FastDiv(nti), FastDiv(ntc));
-                    break;
-            }
+            int od = -1;
+            for (int k = 1; k < dd.nd; ++k)
+                if (k != ch.di) od = k;
+            auto go = [&](auto stage, auto one) {
+                hipLaunchKernelGGL((k_ew_tiled<Body, TI, TC, decltype(stage)::value, decltype(one)::value>),
+                                   dim3((uint32_t)nb), dim3(EW_BLOCK), 0, ctx->stream, body, p, ch.di, od, nti, ntc,
+                                   FastDiv(nti), FastDiv(ntc));
+            };
+            auto by_stage = [&](auto one) {
+                switch (ch.stage) {
+                    case 1: go(std::integral_constant<int, 1>{}, one); break;
+                    case 2: go(std::integral_constant<int, 2>{}, one); break;
+                    default: go(std::integral_constant<int, 3>{}, one); break;
+                }
+            };
+            if (dd.nd <= 3) by_stage(std::true_type{});
+            else by_stage(std::false_type{});
         }
         check_launch("elementwise");
     }

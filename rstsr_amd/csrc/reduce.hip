@@ -147,6 +147,7 @@ struct Ext {  // max / min, NaN ignored
 
 template <class T, bool MAX>
 struct Arg {  // argmax / argmin on (value, logical index)
+    static constexpr bool ARG = true;
     using Val = T;
     using Out = uint64_t;
     struct S {
@@ -172,6 +173,17 @@ struct Arg {  // argmax / argmin on (value, logical index)
         return better(b.v, b.i, a) ? b : a;
     }
     __device__ static S shfl(S s, int m) { return {shfl_val<T>(s.v, m), shfl_u64(s.i, m)}; }
+    // Lane-local scan of a run whose logical indices increase with position:
+    // strictly better values replace, so ties keep the earliest position; NaN is
+    // skipped as in add.
+    __device__ static void scan(T x, uint32_t e, T& bv, uint32_t& be) {
+        if constexpr (std::is_floating_point<T>::value)
+            if (x != x) return;
+        if (be == ~0u || (MAX ? (x > bv) : (x < bv))) {
+            bv = x;
+            be = e;
+        }
+    }
     // x0: the element at logical index 0 of this output's reduced space
     __device__ static Out fin(S s, T x0) {
         if constexpr (std::is_floating_point<T>::value)
@@ -179,6 +191,15 @@ struct Arg {  // argmax / argmin on (value, logical index)
         return s.i == ~0ull ? 0 : s.i;
     }
 };
+
+template <class A, class = void>
+struct IsArg : std::false_type {};
+template <class A>
+struct IsArg<A, std::void_t<decltype(A::ARG)>> : std::true_type {};
+template <class A>
+__device__ constexpr bool is_arg() {
+    return IsArg<A>::value;
+}
 
 /* =============================== plan =============================== */
 
@@ -197,6 +218,7 @@ struct RedPlan {
     // row kernel segmentation of the reduced space: units = rows(rest) x nseg0
     uint32_t seg0 = 1, nseg0 = 1;
     uint64_t units = 1, upw = 1;  // units per split
+    int stream = 0;               // every input element is read once: streaming loads
 };
 
 __device__ __forceinline__ void kept_offsets(const RedPlan& p, uint32_t o, int64_t& oin, int64_t& oout) {
@@ -248,27 +270,51 @@ __global__ void __launch_bounds__(RB) k_red_row(const T* __restrict__ a, typenam
     const int64_t s0 = p.rin[0];
     const uint64_t l0 = (uint64_t)p.rlog[0];
     const uint32_t L0 = (uint32_t)p.rshape[0];
-    for (uint64_t u = u0; u < u1; ++u) {
-        uint64_t row = u / p.nseg0;
-        uint32_t seg = (uint32_t)(u - row * p.nseg0);
-        int64_t roff;
-        uint64_t rli;
-        row_offsets(p, row, roff, rli);
-        const T* base = a + oin + roff;
-        uint32_t e = seg * p.seg0 + lane;
-        uint32_t eend = min(L0, (seg + 1) * p.seg0);
-        for (; e + 192 < eend; e += 256) {
-            T x0 = base[(int64_t)e * s0];
-            T x1 = base[(int64_t)(e + 64) * s0];
-            T x2 = base[(int64_t)(e + 128) * s0];
-            T x3 = base[(int64_t)(e + 192) * s0];
-            A::add(acc, x0, rli + (uint64_t)e * l0);
-            A::add(acc, x1, rli + (uint64_t)(e + 64) * l0);
-            A::add(acc, x2, rli + (uint64_t)(e + 128) * l0);
-            A::add(acc, x3, rli + (uint64_t)(e + 192) * l0);
+    auto sweep = [&](auto nt) {
+        constexpr bool NT = decltype(nt)::value;
+        for (uint64_t u = u0; u < u1; ++u) {
+            uint64_t row = u / p.nseg0;
+            uint32_t seg = (uint32_t)(u - row * p.nseg0);
+            int64_t roff;
+            uint64_t rli;
+            row_offsets(p, row, roff, rli);
+            const T* base = a + oin + roff;
+            uint32_t e = seg * p.seg0 + lane;
+            uint32_t eend = min(L0, (seg + 1) * p.seg0);
+            if constexpr (is_arg<A>()) {
+                // positions e increase with the logical index inside a row (l0 > 0)
+                T bv{};
+                uint32_t be = ~0u;
+                for (; e + 192 < eend; e += 256) {
+                    T x0 = ld<NT>(base + (int64_t)e * s0);
+                    T x1 = ld<NT>(base + (int64_t)(e + 64) * s0);
+                    T x2 = ld<NT>(base + (int64_t)(e + 128) * s0);
+                    T x3 = ld<NT>(base + (int64_t)(e + 192) * s0);
+                    A::scan(x0, e, bv, be);
+                    A::scan(x1, e + 64, bv, be);
+                    A::scan(x2, e + 128, bv, be);
+                    A::scan(x3, e + 192, bv, be);
+                }
+                for (; e < eend; e += 64) A::scan(ld<NT>(base + (int64_t)e * s0), e, bv, be);
+                if (be != ~0u) A::add(acc, bv, rli + (uint64_t)be * l0);
+                continue;
+            }
+            for (; e + 192 < eend; e += 256) {
+                T x0 = ld<NT>(base + (int64_t)e * s0);
+                T x1 = ld<NT>(base + (int64_t)(e + 64) * s0);
+                T x2 = ld<NT>(base + (int64_t)(e + 128) * s0);
+                T x3 = ld<NT>(base + (int64_t)(e + 192) * s0);
+                A::add(acc, x0, rli + (uint64_t)e * l0);
+                A::add(acc, x1, rli + (uint64_t)(e + 64) * l0);
+                A::add(acc, x2, rli + (uint64_t)(e + 128) * l0);
+                A::add(acc, x3, rli + (uint64_t)(e + 192) * l0);
+            }
+            for (; e < eend; e += 64) A::add(acc, ld<NT>(base + (int64_t)e * s0), rli + (uint64_t)e * l0);
         }
-        for (; e < eend; e += 64) A::add(acc, base[(int64_t)e * s0], rli + (uint64_t)e * l0);
-    }
+    };
+    // streaming loads pay off for 8-byte and wider elements; narrower ones measured slower
+    if (sizeof(T) >= 8 && p.stream) sweep(std::true_type{});
+    else sweep(std::false_type{});
 #pragma unroll
     for (int m = 32; m >= 1; m >>= 1) {
         S other = A::shfl(acc, m);
@@ -329,15 +375,44 @@ __global__ void __launch_bounds__(RB) k_red_col(const T* __restrict__ a, typenam
         uint64_t remaining = r1 - r0;
         const int64_t s0 = p.rin[0];
         const uint64_t l0 = (uint64_t)p.rlog[0];
-        while (remaining > 0) {
-            uint64_t n = min(remaining, p.rshape[0] - coord[0]);
+        auto run_row = [&](auto nt, uint64_t n) {
+            constexpr bool NT = decltype(nt)::value;
             const T* rowp = base + off;
             uint64_t j = 0;
+            if constexpr (is_arg<A>()) {
+                // j increases with the logical index along reduced axis 0 (l0 > 0)
+                T bv[VEC];
+                uint32_t be[VEC];
+#pragma unroll
+                for (int v = 0; v < VEC; ++v) be[v] = ~0u;
+                for (; j + 4 <= n; j += 4) {
+                    vt x0 = ld<NT>((const vt*)(rowp + (int64_t)j * s0));
+                    vt x1 = ld<NT>((const vt*)(rowp + (int64_t)(j + 1) * s0));
+                    vt x2 = ld<NT>((const vt*)(rowp + (int64_t)(j + 2) * s0));
+                    vt x3 = ld<NT>((const vt*)(rowp + (int64_t)(j + 3) * s0));
+#pragma unroll
+                    for (int v = 0; v < VEC; ++v) {
+                        A::scan(x0.v[v], (uint32_t)j, bv[v], be[v]);
+                        A::scan(x1.v[v], (uint32_t)j + 1, bv[v], be[v]);
+                        A::scan(x2.v[v], (uint32_t)j + 2, bv[v], be[v]);
+                        A::scan(x3.v[v], (uint32_t)j + 3, bv[v], be[v]);
+                    }
+                }
+                for (; j < n; ++j) {
+                    vt x = ld<NT>((const vt*)(rowp + (int64_t)j * s0));
+#pragma unroll
+                    for (int v = 0; v < VEC; ++v) A::scan(x.v[v], (uint32_t)j, bv[v], be[v]);
+                }
+#pragma unroll
+                for (int v = 0; v < VEC; ++v)
+                    if (be[v] != ~0u) A::add(acc[v], bv[v], li + (uint64_t)be[v] * l0);
+                return;
+            }
             for (; j + 4 <= n; j += 4) {
-                vt x0 = *(const vt*)(rowp + (int64_t)j * s0);
-                vt x1 = *(const vt*)(rowp + (int64_t)(j + 1) * s0);
-                vt x2 = *(const vt*)(rowp + (int64_t)(j + 2) * s0);
-                vt x3 = *(const vt*)(rowp + (int64_t)(j + 3) * s0);
+                vt x0 = ld<NT>((const vt*)(rowp + (int64_t)j * s0));
+                vt x1 = ld<NT>((const vt*)(rowp + (int64_t)(j + 1) * s0));
+                vt x2 = ld<NT>((const vt*)(rowp + (int64_t)(j + 2) * s0));
+                vt x3 = ld<NT>((const vt*)(rowp + (int64_t)(j + 3) * s0));
 #pragma unroll
                 for (int v = 0; v < VEC; ++v) {
                     A::add(acc[v], x0.v[v], li + j * l0);
@@ -347,10 +422,15 @@ __global__ void __launch_bounds__(RB) k_red_col(const T* __restrict__ a, typenam
                 }
             }
             for (; j < n; ++j) {
-                vt x = *(const vt*)(rowp + (int64_t)j * s0);
+                vt x = ld<NT>((const vt*)(rowp + (int64_t)j * s0));
 #pragma unroll
                 for (int v = 0; v < VEC; ++v) A::add(acc[v], x.v[v], li + j * l0);
             }
+        };
+        while (remaining > 0) {
+            uint64_t n = min(remaining, p.rshape[0] - coord[0]);
+            if (sizeof(T) >= 8 && p.stream) run_row(std::true_type{}, n);
+            else run_row(std::false_type{}, n);
             remaining -= n;
             if (remaining == 0) break;
             // carry into dims >= 1
@@ -503,6 +583,12 @@ static RedSetup reduce_plan(rstsr_hip_ctx* ctx, const Layout& la, const std::vec
     p.R = R;
     p.in_off = (int64_t)la.offset;
     p.out_off = (int64_t)lo.offset;
+    // a zero stride on an axis longer than one means repeated reads: keep those cached
+    p.stream = 1;
+    for (int i = 0; i < p.nk; ++i)
+        if (p.kshape[i] > 1 && p.kin[i] == 0) p.stream = 0;
+    for (int i = 0; i < p.nr; ++i)
+        if (p.rshape[i] > 1 && p.rin[i] == 0) p.stream = 0;
 
     const uint64_t target_waves = (uint64_t)std::max(ctx->num_cu, 1) * 32;
     bool row_unit = std::llabs(p.rin[0]) == 1 && p.rshape[0] >= 16;
@@ -522,6 +608,8 @@ static RedSetup reduce_plan(rstsr_hip_ctx* ctx, const Layout& la, const std::vec
         p.nsplit = (uint32_t)nsplit;
     } else {
         s.kind = RedKind::Col;
+        // argmin/argmax track positions along reduced axis 0 in 32 bits
+        if (p.rshape[0] >= 0xffffffffull) RSTSR_RAISE(UnImplemented, "reduced extent too large");
         uint64_t want = std::max<uint64_t>(1, (target_waves * 64) / M);
         uint64_t nsplit = std::min<uint64_t>(want, std::max<uint64_t>(1, R / 64));
         nsplit = std::min<uint64_t>(nsplit, 65536);
