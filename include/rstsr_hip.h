@@ -58,7 +58,14 @@ typedef enum {
     RSTSR_RED_MAX = 1,
     RSTSR_RED_MIN = 2,
     RSTSR_RED_ARGMAX = 3,
-    RSTSR_RED_ARGMIN = 4
+    RSTSR_RED_ARGMIN = 4,
+    /* feature_rayon/auto_impl/reduction.rs: prod :131-165, mean :167-205, var :207-259,
+     * std :261-315, l2_norm :317-354. var/std/l2_norm of complex arrays are real. */
+    RSTSR_RED_PROD = 5,
+    RSTSR_RED_MEAN = 6,
+    RSTSR_RED_VAR = 7,
+    RSTSR_RED_STD = 8,
+    RSTSR_RED_L2NORM = 9
 } rstsr_redop_t;
 
 /* GEMM arithmetic: plain FP64 MFMA, or Ozaki-II split over INT8 MFMA with CRT (f64; c128 as

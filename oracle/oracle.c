@@ -40,7 +40,7 @@ typedef struct {
 
 enum { F32 = 0, F64 = 1, I32 = 2, I64 = 3, C64 = 4, C128 = 5, U64 = 6 };
 enum { OP_ADD = 0, OP_SUB = 1, OP_MUL = 2, OP_DIV = 3 };
-enum { R_SUM = 0, R_MAX = 1, R_MIN = 2, R_ARGMAX = 3, R_ARGMIN = 4 };
+enum { R_SUM = 0, R_MAX = 1, R_MIN = 2, R_ARGMAX = 3, R_ARGMIN = 4, R_PROD = 5, R_MEAN = 6, R_VAR = 7, R_STD = 8, R_L2 = 9 };
 
 typedef struct { float re, im; } c64_t;
 typedef struct { double re, im; } c128_t;
@@ -315,6 +315,67 @@ static void reduce_one(int op, int t, const void* a, const olayout* lr, void* ou
         }
         return;
     }
+    if (op == R_PROD) {
+        /* prod (feature_rayon/auto_impl/reduction.rs:131-165): serial fold acc * x from one,
+         * in the element type (complex (ac-bd, ad+bc) unfused); integers wrap */
+        unsigned char acc[16] = {0};
+        switch (t) {
+            case F32: *(float*)acc = 1.0f; break;
+            case F64: *(double*)acc = 1.0; break;
+            case I32: *(int32_t*)acc = 1; break;
+            case I64: *(int64_t*)acc = 1; break;
+            case C64: ((float*)acc)[0] = 1.0f; break;
+            case C128: ((double*)acc)[0] = 1.0; break;
+        }
+        for (size_t i = 0; i < n; ++i) {
+            const void* p = (const char*)a + o.off * (ptrdiff_t)es;
+            unsigned char r[16];
+            if (t == I32) *(int32_t*)r = (int32_t)((uint32_t) * (int32_t*)acc * (uint32_t) * (const int32_t*)p);
+            else if (t == I64) *(int64_t*)r = (int64_t)((uint64_t) * (int64_t*)acc * (uint64_t) * (const int64_t*)p);
+            else apply(OP_MUL, t, r, acc, p);
+            memcpy(acc, r, es);
+            odo_next(&o, lr);
+        }
+        memcpy(out, acc, es);
+        return;
+    }
+    if (op == R_MEAN || op == R_VAR || op == R_STD || op == R_L2) {
+        /* mean :167-205 = sum / n; var :207-259 = sum|x|^2/n - |sum x/n|^2; std :261-315 = sqrt(var);
+         * l2_norm :317-354 = sqrt(sum |x|^2). Sums in double-double, squares exact (fma). */
+        dd_t sr = {0, 0}, si = {0, 0}, q = {0, 0};
+        int cx = t == C64 || t == C128;
+        for (size_t i = 0; i < n; ++i) {
+            const char* p = (const char*)a + o.off * (ptrdiff_t)es;
+            double re, im = 0.0;
+            if (t == C128) { re = ((const double*)p)[0]; im = ((const double*)p)[1]; }
+            else if (t == C64) { re = ((const float*)p)[0]; im = ((const float*)p)[1]; }
+            else re = as_d(t, p);
+            dd_add(&sr, re);
+            dd_add(&si, im);
+            double pr = re * re, pi = im * im;
+            dd_add(&q, pr);
+            q.c += fma(re, re, -pr);
+            dd_add(&q, pi);
+            q.c += fma(im, im, -pi);
+            odo_next(&o, lr);
+        }
+        double dn = (double)n, qs = q.s + q.c, mr = (sr.s + sr.c) / dn, mi = (si.s + si.c) / dn;
+        double r = 0.0;
+        if (op == R_L2) r = sqrt(qs);
+        else if (op == R_VAR) r = qs / dn - (mr * mr + mi * mi);
+        else if (op == R_STD) r = sqrt(qs / dn - (mr * mr + mi * mi));
+        if (op == R_MEAN) {
+            if (t == F32) *(float*)out = (float)mr;
+            else if (t == F64) *(double*)out = mr;
+            else if (t == C64) { ((float*)out)[0] = (float)mr; ((float*)out)[1] = (float)mi; }
+            else { ((double*)out)[0] = mr; ((double*)out)[1] = mi; }
+        } else {
+            if (t == F32 || t == C64) *(float*)out = (float)r;
+            else *(double*)out = r;
+        }
+        (void)cx;
+        return;
+    }
     if (op == R_MAX || op == R_MIN) {
         int mx = op == R_MAX;
         unsigned char acc[16];
@@ -362,8 +423,9 @@ int o_reduce_axes(int op, int t, const void* a, const olayout* la, size_t naxes,
     olayout lk = {nk, ks, kt, la->offset};
     size_t m = lsize(&lk);
     size_t eo = (op == R_ARGMAX || op == R_ARGMIN) ? 8 : esize(t);
+    if ((op == R_VAR || op == R_STD || op == R_L2) && (t == C64 || t == C128)) eo = esize(t) / 2;
     olayout lr = {nr, rs, rt, 0};
-    if (lsize(&lr) == 0 && op != R_SUM) return 2;
+    if (lsize(&lr) == 0 && (op == R_MAX || op == R_MIN || op == R_ARGMAX || op == R_ARGMIN)) return 2;
     odo_t ok, oo;
     odo_init(&ok, &lk);
     odo_init(&oo, lo);
@@ -377,7 +439,7 @@ int o_reduce_axes(int op, int t, const void* a, const olayout* la, size_t naxes,
 }
 
 int o_reduce_all(int op, int t, const void* a, const olayout* la, void* out) {
-    if (lsize(la) == 0 && op != R_SUM) return 2;
+    if (lsize(la) == 0 && (op == R_MAX || op == R_MIN || op == R_ARGMAX || op == R_ARGMIN)) return 2;
     reduce_one(op, t, a, la, out);
     return 0;
 }

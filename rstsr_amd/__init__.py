@@ -43,6 +43,8 @@ _CODE2NP = {v: k for k, v in _NP2CODE.items()}
 
 OP_ADD, OP_SUB, OP_MUL, OP_DIV = 0, 1, 2, 3
 RED_SUM, RED_MAX, RED_MIN, RED_ARGMAX, RED_ARGMIN = 0, 1, 2, 3, 4
+RED_PROD, RED_MEAN, RED_VAR, RED_STD, RED_L2NORM = 5, 6, 7, 8, 9
+_REAL_OF = {np.dtype(np.complex64): np.dtype(np.float32), np.dtype(np.complex128): np.dtype(np.float64)}
 GEMM_PLAIN, GEMM_OZAKI = 0, 1
 
 _default_device = None
@@ -307,7 +309,10 @@ class Tensor:
         raw = self._t.reduce_all(op)
         if op in (RED_ARGMAX, RED_ARGMIN):
             return int(np.frombuffer(raw, dtype=np.uint64)[0])
-        return np.frombuffer(raw, dtype=self.dtype)[0].item()
+        dt = self.dtype
+        if op in (RED_VAR, RED_STD, RED_L2NORM):
+            dt = _REAL_OF.get(np.dtype(dt), dt)
+        return np.frombuffer(raw, dtype=dt)[0].item()
 
     def _red_axes(self, op, axes):
         return Tensor(self._t.reduce_axes(op, _axes(axes, self.ndim)))
@@ -341,6 +346,38 @@ class Tensor:
 
     def argmin_axes(self, axes=None):
         return self._red_axes(RED_ARGMIN, axes)
+
+    # tensor/reduction.rs:122-126 (prod, mean, var, std, l2_norm); var/std/l2_norm of
+    # complex tensors are real
+    def prod_all(self):
+        return self._red_all(RED_PROD)
+
+    def mean_all(self):
+        return self._red_all(RED_MEAN)
+
+    def var_all(self):
+        return self._red_all(RED_VAR)
+
+    def std_all(self):
+        return self._red_all(RED_STD)
+
+    def l2_norm_all(self):
+        return self._red_all(RED_L2NORM)
+
+    def prod_axes(self, axes=None):
+        return self._red_axes(RED_PROD, axes)
+
+    def mean_axes(self, axes=None):
+        return self._red_axes(RED_MEAN, axes)
+
+    def var_axes(self, axes=None):
+        return self._red_axes(RED_VAR, axes)
+
+    def std_axes(self, axes=None):
+        return self._red_axes(RED_STD, axes)
+
+    def l2_norm_axes(self, axes=None):
+        return self._red_axes(RED_L2NORM, axes)
 
     sum = sum_axes
 

@@ -198,10 +198,20 @@ class Tensor {
         auto sp = layout.dim_split_axes(axes);
         return layout_for_array_copy(sp.second, IterOrder::K);
     }
+    // Element type of a reduction's result: u64 for argmax/argmin, the real part
+    // type for var/std/l2_norm of complex (T::Real), else T.
+    rstsr_dtype_t reduce_out_dtype(rstsr_redop_t op) const {
+        if (op == RSTSR_RED_ARGMAX || op == RSTSR_RED_ARGMIN) return RSTSR_U64;
+        if (op == RSTSR_RED_VAR || op == RSTSR_RED_STD || op == RSTSR_RED_L2NORM) {
+            if (dtype == RSTSR_C64) return RSTSR_F32;
+            if (dtype == RSTSR_C128) return RSTSR_F64;
+        }
+        return dtype;
+    }
     Tensor reduce_axes(rstsr_redop_t op, const std::vector<ptrdiff_t>& axes) const {
         Layout lo = reduce_out_layout(axes);
         bool arg = op == RSTSR_RED_ARGMAX || op == RSTSR_RED_ARGMIN;
-        rstsr_dtype_t to = arg ? RSTSR_U64 : dtype;
+        rstsr_dtype_t to = reduce_out_dtype(op);
         Tensor out = uninit(device(), lo, to);
         AbiLayout xa(layout), xo(lo);
         if (arg)
@@ -215,7 +225,7 @@ class Tensor {
     std::string reduce_all(rstsr_redop_t op) const {
         AbiLayout xa(layout);
         bool arg = op == RSTSR_RED_ARGMAX || op == RSTSR_RED_ARGMIN;
-        std::string out(arg ? 8 : esize(), '\0');
+        std::string out(dtype_size(reduce_out_dtype(op)), '\0');
         if (arg)
             check(rstsr_hip_argreduce_all(device().raw(), op, dtype, ptr(), &xa, (uint64_t*)&out[0]));
         else

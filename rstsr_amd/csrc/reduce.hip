@@ -28,6 +28,8 @@
 #include <array>
 #include <cfloat>
 #include <climits>
+#include <cmath>
+#include <cstring>
 #include <type_traits>
 
 #include "common.hpp"
@@ -67,7 +69,7 @@ struct SumF {  // f32, f64
     __device__ static void add(S& s, T x, uint64_t) { dd_add(s, (double)x); }
     __device__ static S combine(S a, S b) { return dd_combine(a, b); }
     __device__ static S shfl(S s, int m) { return {shfl_d(s.s, m), shfl_d(s.c, m)}; }
-    __device__ static Out fin(S s, T) { return (T)(s.s + s.c); }
+    __device__ static Out fin(S s, T, uint64_t) { return (T)(s.s + s.c); }
 };
 
 template <class T, class R>  // complex: T = c64/c128, R = float/double
@@ -86,7 +88,7 @@ struct SumC {
     __device__ static S shfl(S s, int m) {
         return {{shfl_d(s.re.s, m), shfl_d(s.re.c, m)}, {shfl_d(s.im.s, m), shfl_d(s.im.c, m)}};
     }
-    __device__ static Out fin(S s, T) { return T{(R)(s.re.s + s.re.c), (R)(s.im.s + s.im.c)}; }
+    __device__ static Out fin(S s, T, uint64_t) { return T{(R)(s.re.s + s.re.c), (R)(s.im.s + s.im.c)}; }
 };
 
 template <class T>
@@ -102,7 +104,7 @@ struct SumI {  // i32, i64: wrapping
         if constexpr (sizeof(T) == 8) return (U)shfl_u64((uint64_t)s, m);
         else return (U)__shfl_xor((int)s, m, 64);
     }
-    __device__ static Out fin(S s, T) { return (T)s; }
+    __device__ static Out fin(S s, T, uint64_t) { return (T)s; }
 };
 
 template <class T>
@@ -142,7 +144,7 @@ struct Ext {  // max / min, NaN ignored
     }
     __device__ static S combine(S a, S b) { return (MAX ? (b > a) : (b < a)) ? b : a; }
     __device__ static S shfl(S s, int m) { return shfl_val<T>(s, m); }
-    __device__ static Out fin(S s, T) { return s; }
+    __device__ static Out fin(S s, T, uint64_t) { return s; }
 };
 
 template <class T, bool MAX>
@@ -185,10 +187,122 @@ struct Arg {  // argmax / argmin on (value, logical index)
         }
     }
     // x0: the element at logical index 0 of this output's reduced space
-    __device__ static Out fin(S s, T x0) {
+    __device__ static Out fin(S s, T x0, uint64_t) {
         if constexpr (std::is_floating_point<T>::value)
             if (x0 != x0) return 0;
         return s.i == ~0ull ? 0 : s.i;
+    }
+};
+
+// prod (auto_impl/reduction.rs:131-165): init one, acc * x. Floating products
+// run in f64 (complex: (ac-bd, ad+bc) in f64); integers wrap.
+template <class T>
+struct ProdF {
+    using Val = T;
+    using Out = T;
+    using S = double;
+    __device__ static S init() { return 1.0; }
+    __device__ static void add(S& s, T x, uint64_t) { s *= (double)x; }
+    __device__ static S combine(S a, S b) { return a * b; }
+    __device__ static S shfl(S s, int m) { return shfl_d(s, m); }
+    __device__ static Out fin(S s, T, uint64_t) { return (T)s; }
+};
+
+template <class T, class R>
+struct ProdC {
+    using Val = T;
+    using Out = T;
+    struct S {
+        double re, im;
+    };
+    __device__ static S init() { return {1.0, 0.0}; }
+    __device__ static S mul(S a, S b) { return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re}; }
+    __device__ static void add(S& s, T x, uint64_t) { s = mul(s, S{(double)x.re, (double)x.im}); }
+    __device__ static S combine(S a, S b) { return mul(a, b); }
+    __device__ static S shfl(S s, int m) { return {shfl_d(s.re, m), shfl_d(s.im, m)}; }
+    __device__ static Out fin(S s, T, uint64_t) { return T{(R)s.re, (R)s.im}; }
+};
+
+template <class T>
+struct ProdI {
+    using Val = T;
+    using Out = T;
+    using U = typename std::make_unsigned<T>::type;
+    using S = U;
+    __device__ static S init() { return 1; }
+    __device__ static void add(S& s, T x, uint64_t) { s = (U)(s * (U)x); }
+    __device__ static S combine(S a, S b) { return (U)(a * b); }
+    __device__ static S shfl(S s, int m) {
+        if constexpr (sizeof(T) == 8) return (U)shfl_u64((uint64_t)s, m);
+        else return (U)__shfl_xor((int)s, m, 64);
+    }
+    __device__ static Out fin(S s, T, uint64_t) { return (T)s; }
+};
+
+// mean (auto_impl/reduction.rs:167-205): the compensated sum divided by the
+// number of reduced elements.
+template <class T>
+struct MeanF : SumF<T> {
+    using S = typename SumF<T>::S;
+    __device__ static T fin(S s, T, uint64_t n) { return (T)((s.s + s.c) / (double)n); }
+};
+
+template <class T, class R>
+struct MeanC : SumC<T, R> {
+    using S = typename SumC<T, R>::S;
+    __device__ static T fin(S s, T, uint64_t n) {
+        return T{(R)((s.re.s + s.re.c) / (double)n), (R)((s.im.s + s.im.c) / (double)n)};
+    }
+};
+
+// Exact square into a double-double: x*x plus its rounding error.
+__device__ __forceinline__ void dd_add_sq(DD& a, double x) {
+    const double p = x * x;
+    dd_add(a, p);
+    a.c += fma(x, x, -p);
+}
+
+// var / std (auto_impl/reduction.rs:207-315) and l2_norm (:317-354) on the
+// real type R: compensated sums of x and |x|^2, then the reference's formulas
+// var = sum|x|^2 / n - |sum x / n|^2, std = sqrt(var), l2 = sqrt(sum |x|^2).
+// KIND 0: var, 1: std, 2: l2_norm. T real (R = T) or complex (R its part type).
+template <class T, class R, int KIND>
+struct Moments {
+    static constexpr bool CX = !std::is_same<T, R>::value;
+    using Val = T;
+    using Out = R;
+    struct S {
+        DD re, im, q;
+    };
+    __device__ static S init() { return {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}}; }
+    __device__ static void add(S& s, T x, uint64_t) {
+        if constexpr (CX) {
+            if constexpr (KIND != 2) {
+                dd_add(s.re, (double)x.re);
+                dd_add(s.im, (double)x.im);
+            }
+            dd_add_sq(s.q, (double)x.re);
+            dd_add_sq(s.q, (double)x.im);
+        } else {
+            if constexpr (KIND != 2) dd_add(s.re, (double)x);
+            dd_add_sq(s.q, (double)x);
+        }
+    }
+    __device__ static S combine(S a, S b) { return {dd_combine(a.re, b.re), dd_combine(a.im, b.im), dd_combine(a.q, b.q)}; }
+    __device__ static S shfl(S s, int m) {
+        return {{shfl_d(s.re.s, m), shfl_d(s.re.c, m)}, {shfl_d(s.im.s, m), shfl_d(s.im.c, m)},
+                {shfl_d(s.q.s, m), shfl_d(s.q.c, m)}};
+    }
+    __device__ static Out fin(S s, T, uint64_t n) {
+        const double q = s.q.s + s.q.c;
+        if constexpr (KIND == 2) {
+            return (R)sqrt(q);
+        } else {
+            const double dn = (double)n;
+            const double mr = (s.re.s + s.re.c) / dn, mi = (s.im.s + s.im.c) / dn;
+            const double var = q / dn - (mr * mr + mi * mi);
+            return KIND == 0 ? (R)var : (R)sqrt(var);
+        }
     }
 };
 
@@ -322,7 +436,7 @@ __global__ void __launch_bounds__(RB) k_red_row(const T* __restrict__ a, typenam
         acc = (lane & m) ? A::combine(other, acc) : A::combine(acc, other);
     }
     if (lane == 0) {
-        if (p.nsplit == 1) out[oout] = A::fin(acc, a[oin]);
+        if (p.nsplit == 1) out[oout] = A::fin(acc, a[oin], p.R);
         else part[(uint64_t)o * p.nsplit + sp] = acc;
     }
 }
@@ -453,7 +567,7 @@ __global__ void __launch_bounds__(RB) k_red_col(const T* __restrict__ a, typenam
 #pragma unroll
     for (int v = 0; v < VEC; ++v) {
         // kept axis 0 has input stride +1 when VEC > 1
-        if (p.nsplit == 1) out[oout + v * p.kout[0]] = A::fin(acc[v], a[oin + v]);
+        if (p.nsplit == 1) out[oout + v * p.kout[0]] = A::fin(acc[v], a[oin + v], p.R);
         else part[(uint64_t)(o + v) * p.nsplit + sp] = acc[v];
     }
 }
@@ -478,7 +592,7 @@ __global__ void __launch_bounds__(RB) k_red_fin(const T* __restrict__ a, typenam
     if (lane == 0) {
         int64_t oin, oout;
         kept_offsets(p, o, oin, oout);
-        out[oout] = A::fin(acc, a[oin]);
+        out[oout] = A::fin(acc, a[oin], p.R);
     }
 }
 
@@ -698,6 +812,45 @@ static void reduce_dispatch(rstsr_hip_ctx* ctx, rstsr_redop_t op, rstsr_dtype_t 
             }
             break;
         }
+        case RSTSR_RED_PROD:
+            switch (t) {
+                case RSTSR_F32: go(ProdF<float>{}, float{}); return;
+                case RSTSR_F64: go(ProdF<double>{}, double{}); return;
+                case RSTSR_I32: go(ProdI<int32_t>{}, int32_t{}); return;
+                case RSTSR_I64: go(ProdI<int64_t>{}, int64_t{}); return;
+                case RSTSR_C64: go(ProdC<c64, float>{}, c64{}); return;
+                case RSTSR_C128: go(ProdC<c128, double>{}, c128{}); return;
+                default: break;
+            }
+            break;
+        case RSTSR_RED_MEAN:
+            switch (t) {
+                case RSTSR_F32: go(MeanF<float>{}, float{}); return;
+                case RSTSR_F64: go(MeanF<double>{}, double{}); return;
+                case RSTSR_C64: go(MeanC<c64, float>{}, c64{}); return;
+                case RSTSR_C128: go(MeanC<c128, double>{}, c128{}); return;
+                default: break;
+            }
+            break;
+        case RSTSR_RED_VAR:
+        case RSTSR_RED_STD:
+        case RSTSR_RED_L2NORM: {
+            auto mom = [&](auto kind) {
+                constexpr int K = decltype(kind)::value;
+                switch (t) {
+                    case RSTSR_F32: go(Moments<float, float, K>{}, float{}); return true;
+                    case RSTSR_F64: go(Moments<double, double, K>{}, double{}); return true;
+                    case RSTSR_C64: go(Moments<c64, float, K>{}, c64{}); return true;
+                    case RSTSR_C128: go(Moments<c128, double, K>{}, c128{}); return true;
+                    default: return false;
+                }
+            };
+            bool ok = op == RSTSR_RED_VAR   ? mom(std::integral_constant<int, 0>{})
+                      : op == RSTSR_RED_STD ? mom(std::integral_constant<int, 1>{})
+                                            : mom(std::integral_constant<int, 2>{});
+            if (ok) return;
+            break;
+        }
         case RSTSR_RED_ARGMAX:
         case RSTSR_RED_ARGMIN: {
             bool mx = op == RSTSR_RED_ARGMAX;
@@ -716,19 +869,53 @@ static void reduce_dispatch(rstsr_hip_ctx* ctx, rstsr_redop_t op, rstsr_dtype_t 
 
 static size_t out_size_of(rstsr_redop_t op, rstsr_dtype_t t) {
     if (op == RSTSR_RED_ARGMAX || op == RSTSR_RED_ARGMIN) return 8;
+    // var/std/l2_norm of a complex array are real (T::Real)
+    if ((op == RSTSR_RED_VAR || op == RSTSR_RED_STD || op == RSTSR_RED_L2NORM) && (t == RSTSR_C64 || t == RSTSR_C128))
+        return dtype_size(t) / 2;
     return dtype_size(t);
 }
 
 static void check_op(rstsr_redop_t op, bool arg) {
-    if ((int)op < 0 || (int)op > 4) RSTSR_RAISE(InvalidValue, "unknown reduction op");
+    if ((int)op < 0 || (int)op > 9) RSTSR_RAISE(InvalidValue, "unknown reduction op");
     bool is_arg = op == RSTSR_RED_ARGMAX || op == RSTSR_RED_ARGMIN;
     if (is_arg != arg) RSTSR_RAISE(InvalidValue, arg ? "expected argmax/argmin" : "argmax/argmin need the arg entry point");
 }
 
 // Common path of *_axes entry points; la, lo already validated.
+// Value of a reduction over zero elements: the accumulator's f_out(f_init())
+// in the reference (sum 0, prod 1, l2_norm 0, mean/var/std 0/0 = NaN); max/min
+// and argmax/argmin over nothing are errors there (auto_impl/reduction.rs:95-97).
+static bool empty_value_op(rstsr_redop_t op) {
+    return op == RSTSR_RED_SUM || op == RSTSR_RED_PROD || op == RSTSR_RED_MEAN || op == RSTSR_RED_VAR ||
+           op == RSTSR_RED_STD || op == RSTSR_RED_L2NORM;
+}
+
+static void empty_value(rstsr_redop_t op, rstsr_dtype_t t, void* v) {
+    const size_t es = out_size_of(op, t);
+    std::memset(v, 0, 16);
+    const bool nan = op == RSTSR_RED_MEAN || op == RSTSR_RED_VAR || op == RSTSR_RED_STD;
+    auto put = [&](double re, double im) {
+        if (t == RSTSR_F32 || ((t == RSTSR_C64) && es == 4)) *(float*)v = (float)re;
+        else if (t == RSTSR_F64 || (t == RSTSR_C128 && es == 8)) *(double*)v = re;
+        else if (t == RSTSR_C64) {
+            ((float*)v)[0] = (float)re;
+            ((float*)v)[1] = (float)im;
+        } else if (t == RSTSR_C128) {
+            ((double*)v)[0] = re;
+            ((double*)v)[1] = im;
+        } else if (t == RSTSR_I32) *(int32_t*)v = (int32_t)re;
+        else if (t == RSTSR_I64) *(int64_t*)v = (int64_t)re;
+    };
+    if (op == RSTSR_RED_PROD) put(1.0, 0.0);
+    else if (nan) {
+        if (t == RSTSR_I32 || t == RSTSR_I64) RSTSR_RAISE(UnImplemented, "mean/var/std of integers");
+        put(std::nan(""), (t == RSTSR_C64 || t == RSTSR_C128) && op == RSTSR_RED_MEAN ? std::nan("") : 0.0);
+    }
+}
+
 static void reduce_axes_impl(rstsr_hip_ctx* ctx, rstsr_redop_t op, rstsr_dtype_t t, const void* a, const Layout& la,
                              const std::vector<ptrdiff_t>& axes, void* out, const Layout& lo) {
-    if (op != RSTSR_RED_SUM && la.size() == 0) {
+    if (!empty_value_op(op) && la.size() == 0) {
         if (op == RSTSR_RED_MAX || op == RSTSR_RED_MIN)
             RSTSR_RAISE(InvalidValue, "zero-size array is not supported for max/min");
         RSTSR_RAISE(InvalidLayout, "empty sequence is not allowed for reduce_arg.");
@@ -736,8 +923,10 @@ static void reduce_axes_impl(rstsr_hip_ctx* ctx, rstsr_redop_t op, rstsr_dtype_t
     RedSetup s = reduce_plan(ctx, la, axes, lo);
     if (s.empty_out) return;
     if (s.empty_reduce) {
-        // only sums reach here (la.size() == 0 but outputs exist): T::zero
-        if (op != RSTSR_RED_SUM) RSTSR_RAISE(InvalidValue, "zero-size reduction");
+        // outputs exist but each reduces zero elements: every output is the empty value
+        if (!empty_value_op(op)) RSTSR_RAISE(InvalidValue, "zero-size reduction");
+        alignas(16) unsigned char ev[16];
+        empty_value(op, t, ev);
         // build a plan over the kept axes only to address outputs
         Layout lm = la.dim_split_axes(axes).second;
         std::vector<ptrdiff_t> none;
@@ -766,16 +955,20 @@ static void reduce_axes_impl(rstsr_hip_ctx* ctx, rstsr_redop_t op, rstsr_dtype_t
         p.out_off = (int64_t)lo.offset;
         auto z = [&](auto tag) {
             using T = decltype(tag);
-            hipLaunchKernelGGL((k_fill_value<T>), dim3(ceil_div_u32(M, RB)), dim3(RB), 0, ctx->stream, (T*)out, p, T{});
+            T v;
+            std::memcpy(&v, ev, sizeof(T));
+            hipLaunchKernelGGL((k_fill_value<T>), dim3(ceil_div_u32(M, RB)), dim3(RB), 0, ctx->stream, (T*)out, p, v);
             check_launch("fill");
         };
+        // the output element type: T, or its real part type for var/std/l2_norm of complex
+        const size_t es = out_size_of(op, t);
         switch (t) {
             case RSTSR_F32: z(float{}); break;
             case RSTSR_F64: z(double{}); break;
             case RSTSR_I32: z(int32_t{}); break;
             case RSTSR_I64: z(int64_t{}); break;
-            case RSTSR_C64: z(c64{}); break;
-            case RSTSR_C128: z(c128{}); break;
+            case RSTSR_C64: es == 4 ? z(float{}) : z(c64{}); break;
+            case RSTSR_C128: es == 8 ? z(double{}) : z(c128{}); break;
             default: RSTSR_RAISE(UnImplemented, "dtype");
         }
         return;
@@ -843,11 +1036,13 @@ static void reduce_all_common(rstsr_hip_ctx_t* ctx, rstsr_redop_t op, rstsr_dtyp
     if (la.size() == 0) {
         if (op == RSTSR_RED_MAX || op == RSTSR_RED_MIN)
             RSTSR_RAISE(InvalidValue, "zero-size array is not supported for max/min");
-        if (op != RSTSR_RED_SUM) RSTSR_RAISE(InvalidLayout, "empty sequence is not allowed for reduce_arg.");
+        if (!empty_value_op(op)) RSTSR_RAISE(InvalidLayout, "empty sequence is not allowed for reduce_arg.");
         trivial_sum = true;
     }
     if (trivial_sum) {
-        std::memset(host_out, 0, out_bytes);
+        alignas(16) unsigned char ev[16];
+        empty_value(op, t, ev);
+        std::memcpy(host_out, ev, out_bytes);
         return;
     }
     Layout lo({}, {}, 0);
@@ -867,7 +1062,7 @@ int rstsr_hip_reduce_all(rstsr_hip_ctx_t* ctx, rstsr_redop_t op, rstsr_dtype_t t
                          const rstsr_layout_t* la, void* host_out) {
     return guarded([&] {
         check_op(op, false);
-        reduce_all_common(ctx, op, t, a, la, host_out, dtype_size(t));
+        reduce_all_common(ctx, op, t, a, la, host_out, out_size_of(op, t));
     });
 }
 

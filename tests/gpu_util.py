@@ -36,7 +36,10 @@ def oracle_reduce(oracle, op, va, axes, dtype):
     nd = len(va.shape)
     ax = [a % nd for a in axes]
     shape = [va.shape[i] for i in range(nd) if i not in ax]
-    vo = c_view(shape, np.uint64 if op in (3, 4) else dtype)
+    odt = np.uint64 if op in (3, 4) else dtype
+    if op in (7, 8, 9) and np.dtype(dtype).kind == "c":  # var/std/l2_norm of complex are real
+        odt = np.float32 if np.dtype(dtype) == np.complex64 else np.float64
+    vo = c_view(shape, odt)
     r = oracle.reduce_axes(op, va, list(axes), vo)
     assert r == 0
     return vo.array()

@@ -128,6 +128,8 @@ class Oracle:
 
     def reduce_all(self, op, a):
         outdt = np.uint64 if op in (3, 4) else a.base.dtype
+        if op in (7, 8, 9) and np.issubdtype(a.base.dtype, np.complexfloating):
+            outdt = np.float32 if a.base.dtype == np.complex64 else np.float64
         out = np.zeros(1, dtype=outdt)
         r = self.lib.o_reduce_all(op, CODES[a.base.dtype], a.ptr, C.byref(a.c), out.ctypes.data_as(C.c_void_p))
         if r != 0:

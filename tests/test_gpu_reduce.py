@@ -139,3 +139,128 @@ def test_reduce_errors(dev):
     assert z.tolist() == [0.0, 0.0, 0.0]
     with pytest.raises(rt.RSTSRError):
         rt.zeros((3, 4), device=dev).sum_axes([0, 0])
+
+
+# ---------------- prod / mean / var / std / l2_norm (auto_impl/reduction.rs:131-354) ----------------
+PROD, MEAN, VAR, STD, L2 = 5, 6, 7, 8, 9
+MOM = {PROD: "prod", MEAN: "mean", VAR: "var", STD: "std", L2: "l2_norm"}
+
+
+def check_moment(op, got, ref, x, axes, dtype):
+    """Stated tolerances. mean/l2: both sides are the compensated sum rounded once, then divided or
+    square-rooted: 4 ulp of the result. var/std: E|x|^2 - |Ex|^2 cancels; the bound is 8 ulp of
+    E|x|^2 (absolute), and for std that bound propagated through the square root. prod: the
+    device multiplies in f64 in another order than the oracle's sequential fold; relative error
+    <= 2 n u of the element type."""
+    kind = np.dtype(dtype).kind
+    rdt = np.dtype(dtype).type(0).real.dtype if kind == "c" else np.dtype(dtype)
+    if kind == "i":
+        assert bits_equal(got, ref)
+        return
+    eps = float(np.finfo(rdt).eps)
+    g = np.asarray(got, dtype=np.complex128 if (kind == "c" and op in (PROD, MEAN)) else np.float64)
+    r = np.asarray(ref, dtype=g.dtype)
+    nd = x.ndim
+    ax = tuple(a % nd for a in axes)
+    n = int(np.prod([x.shape[a] for a in ax])) if ax else 1
+    if op in (MEAN, L2):
+        tol = 4 * eps * np.abs(r) + 1e-300
+    elif op == VAR:
+        msq = np.mean(np.abs(x.astype(np.complex128 if kind == "c" else np.float64)) ** 2, axis=ax)
+        tol = 8 * eps * msq + 1e-300
+    elif op == STD:
+        msq = np.mean(np.abs(x.astype(np.complex128 if kind == "c" else np.float64)) ** 2, axis=ax)
+        tol = 8 * eps * msq / np.maximum(np.abs(r), 1e-300) + 4 * eps * np.abs(r) + 1e-300
+    else:
+        tol = 2 * n * eps * np.abs(r) + 1e-300
+    err = np.abs(g - r)
+    assert np.all(err <= tol), float(np.max(err / tol))
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32, np.complex128, np.complex64, np.int32, np.int64])
+@pytest.mark.parametrize("op", [PROD, MEAN, VAR, STD, L2])
+@pytest.mark.parametrize("ci", [0, 1, 5, 6, 8, 9])
+def test_moment_axes(dev, oracle, dtype, op, ci):
+    if np.dtype(dtype).kind == "i" and op != PROD:
+        pytest.skip("mean/var/std/l2_norm are defined for floating and complex types (ComplexFloat)")
+    X, perm, step, axes = CASES[ci]
+    rng = np.random.default_rng(zlib.crc32(repr(("mom", ci, op, str(dtype))).encode()))
+    n = int(np.prod(X))
+    if op == PROD:
+        # factors near one keep long products finite; integers in [-3, 3] wrap exactly
+        if np.dtype(dtype).kind == "i":
+            base = rng.integers(-3, 4, size=n).astype(dtype)
+        elif np.dtype(dtype).kind == "c":
+            base = (np.exp(rng.normal(0, 1e-3, size=n)) * np.exp(1j * rng.normal(0, 1e-3, size=n))).astype(dtype)
+        else:
+            base = np.exp(rng.normal(0, 1e-3, size=n)).astype(dtype)
+    else:
+        base = rand(rng, n, dtype)
+        if op != L2:
+            base = base + np.asarray(0.25, dtype=base.dtype)  # a nonzero mean exercises the cancellation
+    x = base.reshape(X).transpose(perm)[..., ::step]
+    tx, vx = from_np(dev, x)
+    out = getattr(tx, MOM[op] + "_axes")(axes)
+    ref = oracle_reduce(oracle, op, vx, axes, dtype)
+    got = out.to_numpy()
+    assert got.shape == ref.shape and got.dtype == ref.dtype
+    check_moment(op, got, ref, x, axes, dtype)
+    ls, lm = rt._rstsr.dim_split_axes(tx.layout, axes)
+    assert out.layout == rt._rstsr.layout_for_array_copy(lm, "K")
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.complex128, np.float32])
+@pytest.mark.parametrize("op", [PROD, MEAN, VAR, STD, L2])
+@pytest.mark.parametrize("n", [1, 4097, 1 << 20])
+def test_moment_all(dev, oracle, dtype, op, n):
+    rng = np.random.default_rng(n * 10 + op)
+    if op == PROD:
+        x = np.exp(rng.normal(0, 1e-4, size=2 * n)).astype(dtype)[::2]
+    else:
+        x = rand(rng, 2 * n, dtype)[::2]
+    tx, vx = from_np(dev, x)
+    got = getattr(tx, MOM[op] + "_all")()
+    ref = oracle.reduce_all(op, vx)
+    check_moment(op, np.asarray(got), np.asarray(ref), x, [0], dtype)
+
+
+def test_moment_known_answers(dev):
+    """tensor/reduction.rs:557-743: mean (arange, flipped slices), var/std (4x3), complex std and
+    l2_norm, with the reference's own tolerances."""
+    import json
+    import os
+
+    ref = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "reference_cases.json")))
+    r = ref["mean_arange24"]
+    a = rt.asarray(np.arange(24.0).reshape(2, 3, 4), device=dev)
+    assert a.mean_all() == r["mean_all"]
+    assert list(a.mean_axes([0, 2]).to_numpy()) == r["mean_axes_0_2"]
+    assert list(a[::-1, :, ::-2].mean_axes([-1, 1]).to_numpy()) == r["flipped_mean"]
+    r = ref["moments_4x3"]
+    m = rt.asarray(np.array(r["data"], dtype=np.float64).reshape(r["shape"]), device=dev)
+
+    def allclose_f64(g, want):
+        g, want = np.asarray(g), np.asarray(want)
+        return g.shape == want.shape and bool(np.all(np.abs(g - want) <= 1e-8 + 1e-5 * np.abs(want)))
+
+    assert abs(m.var_all() - r["var_all"]) < 1e-10
+    assert allclose_f64(m.var_axes([0]).to_numpy(), r["var_axes_0"])
+    assert allclose_f64(m.var_axes([1]).to_numpy(), r["var_axes_1"])
+    assert abs(m.std_all() - r["std_all"]) < 1e-10
+    assert allclose_f64(m.std_axes([0]).to_numpy(), r["std_axes_0"])
+    assert allclose_f64(m.std_axes([1]).to_numpy(), r["std_axes_1"])
+    z = rt.asarray((np.array(r["data"], dtype=np.float64) + 1j * np.array(r["imag"], dtype=np.float64)).reshape(4, 3),
+                   device=dev)
+    assert abs(z.std_all() - r["std_all_complex"]) < 1e-10
+    assert abs(z.l2_norm_all() - r["l2_norm_all_complex"]) < 1e-10
+    assert z.std_axes([0]).to_numpy().dtype == np.float64
+
+
+def test_moment_empty_values(dev):
+    """Zero reduced elements: prod 1, l2_norm 0, mean/var/std NaN (f_out(f_init()) in the reference)."""
+    e = rt.asarray(np.zeros((3, 0)), device=dev)
+    assert np.all(e.prod_axes([1]).to_numpy() == 1.0)
+    assert np.all(e.l2_norm_axes([1]).to_numpy() == 0.0)
+    assert np.all(np.isnan(e.mean_axes([1]).to_numpy()))
+    assert np.isnan(e.std_all())
+    assert e.prod_all() == 1.0

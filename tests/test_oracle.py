@@ -22,6 +22,7 @@ REF = json.load(open(os.path.join(HERE, "golden", "reference_cases.json")))
 
 ADD, SUB, MUL, DIV = 0, 1, 2, 3
 SUM, MAX, MIN, ARGMAX, ARGMIN = 0, 1, 2, 3, 4
+PROD, MEAN, VAR, STD, L2 = 5, 6, 7, 8, 9
 
 
 def view_of_numpy_slice(base, build):
@@ -46,7 +47,10 @@ def o_reduce(oracle, op, a_np, axes):
     va = View.of(a_np)
     kept = [i for i in range(a_np.ndim) if i not in [x % a_np.ndim for x in axes]]
     shape = [a_np.shape[i] for i in kept]
-    vo = c_view(shape, np.uint64 if op in (ARGMAX, ARGMIN) else a_np.dtype)
+    odt = np.uint64 if op in (ARGMAX, ARGMIN) else a_np.dtype
+    if op in (VAR, STD, L2) and np.issubdtype(a_np.dtype, np.complexfloating):
+        odt = np.float32 if a_np.dtype == np.complex64 else np.float64
+    vo = c_view(shape, odt)
     assert oracle.reduce_axes(op, va, list(axes), vo) == 0
     return vo.array()
 
@@ -305,3 +309,101 @@ def test_pins_catch_planted_mistakes(oracle):
     # last-occurrence tie rule would fail the argmax known answer
     d = np.array([3, 3, 3, 3, 2, 2, 2, 2], dtype=np.int32)
     assert len(d) - 1 - int(np.argmax(d[::-1])) != REF["argmax_core"]["all_cases"][1][1]
+
+
+# ---------------- prod / mean / var / std / l2_norm ----------------
+
+
+def _allclose_f64(a, b):
+    """dev_utilities/mod.rs:11-36: |a - b| <= 1e-8 + 1e-5 |b| elementwise."""
+    a, b = np.asarray(a, dtype=np.float64), np.asarray(b, dtype=np.float64)
+    return a.shape == b.shape and bool(np.all(np.abs(a - b) <= 1e-8 + 1e-5 * np.abs(b)))
+
+
+def test_known_moments(oracle):
+    r = REF["moments_4x3"]
+    a = np.array(r["data"], dtype=np.float64).reshape(r["shape"])
+    assert abs(oracle.reduce_all(VAR, View.of(a)) - r["var_all"]) < 1e-10
+    assert _allclose_f64(o_reduce(oracle, VAR, a, [0]), r["var_axes_0"])
+    assert _allclose_f64(o_reduce(oracle, VAR, a, [1]), r["var_axes_1"])
+    assert abs(oracle.reduce_all(STD, View.of(a)) - r["std_all"]) < 1e-10
+    assert _allclose_f64(o_reduce(oracle, STD, a, [0]), r["std_axes_0"])
+    assert _allclose_f64(o_reduce(oracle, STD, a, [1]), r["std_axes_1"])
+    z = (np.array(r["data"], dtype=np.float64) + 1j * np.array(r["imag"], dtype=np.float64)).reshape(r["shape"])
+    assert abs(oracle.reduce_all(STD, View.of(z)) - r["std_all_complex"]) < 1e-10
+    assert abs(oracle.reduce_all(L2, View.of(z)) - r["l2_norm_all_complex"]) < 1e-10
+
+
+def test_known_mean_direct(oracle):
+    r = REF["mean_arange24"]
+    a = np.arange(24.0).reshape(2, 3, 4)
+    assert oracle.reduce_all(MEAN, View.of(a)) == r["mean_all"]
+    assert list(o_reduce(oracle, MEAN, a, [0, 2])) == r["mean_axes_0_2"]
+    assert list(o_reduce(oracle, MEAN, a[::-1, :, ::-2], [-1, 1])) == r["flipped_mean"]
+
+
+def test_var_std_are_population_statistics(oracle):
+    """statistics.pvariance works in exact rational arithmetic, so on data with a small mean the
+    reference formula (E|x|^2 - |E x|^2 with compensated sums) must agree to a few ulp."""
+    import statistics
+
+    rng = np.random.default_rng(3)
+    for n in (1, 2, 7, 100, 1001):
+        x = rng.normal(size=n)
+        pv = statistics.pvariance(x.tolist())
+        assert abs(oracle.reduce_all(VAR, View.of(x)) - pv) <= 1e-13 * max(pv, 1e-300) + 1e-300
+        assert abs(oracle.reduce_all(STD, View.of(x)) - math.sqrt(pv)) <= 1e-13 * math.sqrt(pv) + 1e-300
+    # complex: population variance of x is var(re) + var(im)
+    z = rng.normal(size=257) + 1j * rng.normal(size=257)
+    pv = statistics.pvariance(z.real.tolist()) + statistics.pvariance(z.imag.tolist())
+    assert abs(oracle.reduce_all(VAR, View.of(z)) - pv) <= 1e-13 * pv
+
+
+def test_l2_mean_prod_closed_forms(oracle):
+    rng = np.random.default_rng(4)
+    x = rng.normal(size=513)
+    assert abs(oracle.reduce_all(L2, View.of(x)) - math.sqrt(math.fsum(x * x))) <= 2 ** -52 * math.sqrt(math.fsum(x * x))
+    assert abs(oracle.reduce_all(MEAN, View.of(x)) - math.fsum(x) / 513) <= 2 ** -52 * abs(math.fsum(x) / 513)
+    # prod: the sequential fold in the element type, bit for bit (Python floats are f64)
+    y = rng.uniform(0.5, 1.5, size=200)
+    acc = 1.0
+    for v in y:
+        acc *= float(v)
+    assert oracle.reduce_all(PROD, View.of(y)) == acc
+    # integers wrap (i32): exact modular product
+    k = rng.integers(-1000, 1000, size=50).astype(np.int32)
+    ref = 1
+    for v in k.tolist():
+        ref = (ref * v) % (1 << 32)
+    ref = ref - (1 << 32) if ref >= (1 << 31) else ref
+    assert int(oracle.reduce_all(PROD, View.of(k))) == ref
+    # complex: Python's complex product on the same order
+    z = rng.uniform(0.5, 1.2, size=40) + 1j * rng.uniform(-0.3, 0.3, size=40)
+    accz = complex(1, 0)
+    for v in z:
+        accz = complex(accz.real * v.real - accz.imag * v.imag, accz.real * v.imag + accz.imag * v.real)
+    assert oracle.reduce_all(PROD, View.of(z)) == accz
+
+
+def test_empty_reduction_values(oracle):
+    e = np.zeros((0,))
+    assert oracle.reduce_all(PROD, View.of(e)) == 1.0
+    assert math.isnan(oracle.reduce_all(MEAN, View.of(e)))
+    assert math.isnan(oracle.reduce_all(VAR, View.of(e)))
+    assert oracle.reduce_all(L2, View.of(e)) == 0.0
+
+
+def test_planted_mistakes_in_moments_are_caught(oracle):
+    """A sample variance (divide by n-1), a mean over the wrong count and a missing square root
+    each fail the pins above."""
+    import statistics
+
+    x = np.random.default_rng(5).normal(size=100)
+    pv = statistics.pvariance(x.tolist())
+    sample = oracle.reduce_all(VAR, View.of(x)) * 100 / 99
+    assert abs(sample - pv) > 1e-13 * pv
+    wrong_mean = oracle.reduce_all(SUM, View.of(x)) / 99
+    assert abs(wrong_mean - math.fsum(x) / 100) > 2 ** -52 * abs(math.fsum(x) / 100)
+    r = REF["moments_4x3"]
+    a = np.array(r["data"], dtype=np.float64).reshape(r["shape"])
+    assert abs(oracle.reduce_all(VAR, View.of(a)) - r["std_all"]) > 1e-10
