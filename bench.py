@@ -130,6 +130,10 @@ def main():
                             ("max_ax0", V, [0], np.float64), ("max_ax12", V, [1, 2], np.float64),
                             ("i_ax0", Vi, [0], np.uint64), ("i_ax12", Vi, [1, 2], np.uint64)]:
         red[nm] = red_out(t, axes, dt)
+    # unary outputs laid out as the tensor API allocates them: layout_for_array_copy(V, K)
+    lUo = RL.layout_for_array_copy(V.layout, "K")
+    U64 = rt.Tensor(RL.empty(dev, lUo, rt._code(np.float64)))
+    lU = abi.make_layout(*lUo)
     P = tX.transpose([2, 0, 1])
     Pout = rt.empty(list(P.shape), np.float64, device=dev)
     Cflat = rt.empty([256, 512, 1024], np.float64, device=dev)
@@ -155,6 +159,11 @@ def main():
         f = L.rstsr_hip_argreduce_axes if op == ARGMAX else L.rstsr_hip_reduce_axes
         return lambda: abi.check(f(ctx, op, t, ptr(a), C.byref(la), na, axes, ptr(o), C.byref(lo)))
 
+    UN_EXP, UN_SIN, UN_SQRT = 10, 24, 27
+
+    def mk_un(op, c, lc, a, la):
+        return lambda: abi.check(L.rstsr_hip_unary(ctx, op, F64, ptr(c), C.byref(lc), F64, ptr(a), C.byref(la)))
+
     ops = [
         Op("mul_f64", mk_bin(MUL, F64, Y64, lY, tX, lV, tb, lbV), nV * 16 + 512 * 8),
         Op("mul_f32", mk_bin(MUL, F32, Y32, lY, tX32, lV32, tb32, lbV32), nV * 8 + 512 * 4),
@@ -169,6 +178,11 @@ def main():
         Op("argmax_ax12_i32", mk_red(ARGMAX, I32, tXi, lVi, ax12, 2, red["i_ax12"]), nV * 4 + 1024 * 8),
         Op("to_contig_f64", lambda: abi.check(L.rstsr_hip_assign(ctx, F64, ptr(Pout), C.byref(lPout), F64, ptr(tX),
                                                                  C.byref(lP))), 2 * 8 * 256 * 512 * 1024),
+        # unary ops on the strided view (not part of the step): sqrt is a single IEEE op, exp and
+        # sin are the device's f64 math library
+        Op("sqrt_f64", mk_un(UN_SQRT, U64, lU, tX, lV), nV * 16, in_step=False),
+        Op("exp_f64", mk_un(UN_EXP, U64, lU, tX, lV), nV * 16, in_step=False),
+        Op("sin_f64", mk_un(UN_SIN, U64, lU, tX, lV), nV * 16, in_step=False),
         # calibration: contiguous copy of the same 1.07 GB (not part of the step)
         Op("contig_copy_f64", lambda: abi.check(L.rstsr_hip_assign(ctx, F64, ptr(Cflat), C.byref(lC), F64, ptr(tX),
                                                                    C.byref(lX))), 2 * 8 * 256 * 512 * 1024,

@@ -68,6 +68,57 @@ typedef enum {
     RSTSR_RED_L2NORM = 9
 } rstsr_redop_t;
 
+/* Unary elementwise operations: the 35 traits of rstsr-core/src/operators/ops/op_binary_common.rs:3-39
+ * (OpAcosAPI ... OpSignAPI, in that order), then OpNegAPI and OpNotAPI
+ * (operators/ops/op_binary_arithmetic.rs:90-109). Output dtypes follow the traits' TOut:
+ *  - acos .. trunc except the classifiers, square and signbit: T::FloatType
+ *    (rstsr-dtype-traits DTypeIntoFloatAPI: i32/i64 -> f64, floats and complex unchanged);
+ *    ceil, expm1, floor, log1p, round and trunc need a real float (num::Float bound);
+ *  - is_finite, is_inf, is_nan (floats, complex) and signbit (integers, floats): bool;
+ *  - square, sign, neg: T; not: integers and bool;
+ *  - abs, imag, real: ExtNum::AbsOut (complex -> its real type, otherwise T).
+ * log1p has a trait but no implementation in the reference devices; it is provided here
+ * for real floats. */
+typedef enum {
+    RSTSR_UN_ACOS = 0,
+    RSTSR_UN_ACOSH = 1,
+    RSTSR_UN_ASIN = 2,
+    RSTSR_UN_ASINH = 3,
+    RSTSR_UN_ATAN = 4,
+    RSTSR_UN_ATANH = 5,
+    RSTSR_UN_CEIL = 6,
+    RSTSR_UN_CONJ = 7,
+    RSTSR_UN_COS = 8,
+    RSTSR_UN_COSH = 9,
+    RSTSR_UN_EXP = 10,
+    RSTSR_UN_EXPM1 = 11,
+    RSTSR_UN_FLOOR = 12,
+    RSTSR_UN_INV = 13,
+    RSTSR_UN_IS_FINITE = 14,
+    RSTSR_UN_IS_INF = 15,
+    RSTSR_UN_IS_NAN = 16,
+    RSTSR_UN_LOG = 17,
+    RSTSR_UN_LOG1P = 18,
+    RSTSR_UN_LOG2 = 19,
+    RSTSR_UN_LOG10 = 20,
+    RSTSR_UN_RECIPROCAL = 21,
+    RSTSR_UN_ROUND = 22,
+    RSTSR_UN_SIGNBIT = 23,
+    RSTSR_UN_SIN = 24,
+    RSTSR_UN_SINH = 25,
+    RSTSR_UN_SQUARE = 26,
+    RSTSR_UN_SQRT = 27,
+    RSTSR_UN_TAN = 28,
+    RSTSR_UN_TANH = 29,
+    RSTSR_UN_TRUNC = 30,
+    RSTSR_UN_ABS = 31,
+    RSTSR_UN_IMAG = 32,
+    RSTSR_UN_REAL = 33,
+    RSTSR_UN_SIGN = 34,
+    RSTSR_UN_NEG = 35,
+    RSTSR_UN_NOT = 36
+} rstsr_unop_t;
+
 /* GEMM arithmetic: plain FP64 MFMA, or Ozaki-II split over INT8 MFMA with CRT (f64; c128 as
  * four real products with real alpha/beta). */
 typedef enum { RSTSR_GEMM_PLAIN = 0, RSTSR_GEMM_OZAKI_INT8 = 1 } rstsr_gemm_mode_t;
@@ -160,6 +211,20 @@ int rstsr_hip_argreduce_axes(rstsr_hip_ctx_t* ctx, rstsr_redop_t op, rstsr_dtype
 int rstsr_hip_allclose(rstsr_hip_ctx_t* ctx, rstsr_dtype_t t, const void* a, const rstsr_layout_t* la,
                        const void* b, const rstsr_layout_t* lb, double rtol, double atol, int equal_nan,
                        int* host_out);
+
+/* ---- unary elementwise (op_binary_common.rs:41-55, op_binary_arithmetic.rs:95-109) ---- */
+/* Output dtype of `op` on input dtype `ta` (the trait's TOut); RSTSR_ERR_UNIMPLEMENTED when the
+ * reference has no implementation for that input type. */
+int rstsr_hip_unary_out_dtype(rstsr_unop_t op, rstsr_dtype_t ta, rstsr_dtype_t* tc);
+/* OpXxxAPI::op_muta_refb: c = op(a) elementwise, layouts of one shape (the tensor API passes
+ * lc = layout_for_array_copy(la, K), tensor/operators/op_unary_common.rs:150). tc must equal
+ * rstsr_hip_unary_out_dtype(op, ta). c may alias a only with an identical layout. */
+int rstsr_hip_unary(rstsr_hip_ctx_t* ctx, rstsr_unop_t op, rstsr_dtype_t tc, void* c, const rstsr_layout_t* lc,
+                    rstsr_dtype_t ta, const void* a, const rstsr_layout_t* la);
+/* OpXxxAPI::op_muta: c = op(c) in place; only ops whose output dtype equals t (the owned-tensor
+ * path of op_unary_common.rs:175-190). */
+int rstsr_hip_unary_inplace(rstsr_hip_ctx_t* ctx, rstsr_unop_t op, rstsr_dtype_t t, void* c,
+                            const rstsr_layout_t* lc);
 
 /* ---- matmul (DeviceMatMulAPI, operators/matmul.rs:7-25; device_faer/matmul.rs:204-239) ---- */
 /* c = alpha * a % b + beta * c with rstsr's row-major matmul rules 1-7 (layout/matmul.rs:9-29).

@@ -29,7 +29,7 @@ DeviceHIP = _rstsr.DeviceHIP
 device_count = _rstsr.device_count
 
 # dtype codes of include/rstsr_hip.h
-F32, F64, I32, I64, C64, C128, U64 = 0, 1, 2, 3, 4, 5, 6
+F32, F64, I32, I64, C64, C128, U64, BOOL = 0, 1, 2, 3, 4, 5, 6, 7
 _NP2CODE = {
     np.dtype(np.float32): F32,
     np.dtype(np.float64): F64,
@@ -38,6 +38,7 @@ _NP2CODE = {
     np.dtype(np.complex64): C64,
     np.dtype(np.complex128): C128,
     np.dtype(np.uint64): U64,
+    np.dtype(np.bool_): BOOL,
 }
 _CODE2NP = {v: k for k, v in _NP2CODE.items()}
 
@@ -46,6 +47,15 @@ RED_SUM, RED_MAX, RED_MIN, RED_ARGMAX, RED_ARGMIN = 0, 1, 2, 3, 4
 RED_PROD, RED_MEAN, RED_VAR, RED_STD, RED_L2NORM = 5, 6, 7, 8, 9
 _REAL_OF = {np.dtype(np.complex64): np.dtype(np.float32), np.dtype(np.complex128): np.dtype(np.float64)}
 GEMM_PLAIN, GEMM_OZAKI = 0, 1
+
+# unary ops (rstsr_unop_t): operators/ops/op_binary_common.rs:3-39 in order, then neg and not
+UNARY_OPS = (
+    "acos", "acosh", "asin", "asinh", "atan", "atanh", "ceil", "conj", "cos", "cosh", "exp", "expm1",
+    "floor", "inv", "is_finite", "is_inf", "is_nan", "log", "log1p", "log2", "log10", "reciprocal",
+    "round", "signbit", "sin", "sinh", "square", "sqrt", "tan", "tanh", "trunc", "abs", "imag", "real",
+    "sign", "neg", "not_",
+)
+UN = {name: i for i, name in enumerate(UNARY_OPS)}
 
 _default_device = None
 
@@ -380,6 +390,54 @@ class Tensor:
         return self._red_axes(RED_L2NORM, axes)
 
     sum = sum_axes
+
+    # ---------------- unary (tensor/operators/op_unary_common.rs:96-190) ----------------
+    def unary(self, op):
+        """op(&tensor): a new tensor laid out as layout_for_array_copy(self, K)."""
+        return Tensor(self._t.unary(UN[op] if isinstance(op, str) else int(op)))
+
+    def unary_to(self, op, out):
+        """op_muta_refb into an existing output view `out` of the same shape."""
+        self._t.unary_to(UN[op] if isinstance(op, str) else int(op), out._t)
+        return out
+
+    def unary_(self, op):
+        """op(tensor) on an owned tensor: in place (op_muta), same-dtype ops only."""
+        self._t.unary_inplace(UN[op] if isinstance(op, str) else int(op))
+        return self
+
+    def __neg__(self):
+        return self.unary("neg")
+
+    def __invert__(self):
+        return self.unary("not_")
+
+    def __abs__(self):
+        return self.unary("abs")
+
+
+def _unary_method(name):
+    def f(self):
+        return self.unary(name)
+
+    f.__name__ = name
+    f.__doc__ = f"rt::{name.rstrip('_')}(&tensor)"
+    return f
+
+
+def _unary_func(name):
+    def f(x):
+        return x.unary(name)
+
+    f.__name__ = name
+    f.__doc__ = f"rt::{name.rstrip('_')}(&tensor)"
+    return f
+
+
+for _name in UNARY_OPS:
+    setattr(Tensor, _name, _unary_method(_name))
+    globals()[_name] = _unary_func(_name)
+del _name
 
 
 def asarray(x, device=None, dtype=None):

@@ -54,6 +54,9 @@ def lib():
         "rstsr_hip_reduce_axes": [vp, C.c_int, C.c_int, vp, P, C.c_size_t, C.POINTER(C.c_ssize_t), vp, P],
         "rstsr_hip_argreduce_axes": [vp, C.c_int, C.c_int, vp, P, C.c_size_t, C.POINTER(C.c_ssize_t), vp, P],
         "rstsr_hip_reduce_all": [vp, C.c_int, C.c_int, vp, P, vp],
+        "rstsr_hip_unary": [vp, C.c_int, C.c_int, vp, P, C.c_int, vp, P],
+        "rstsr_hip_unary_inplace": [vp, C.c_int, C.c_int, vp, P],
+        "rstsr_hip_unary_out_dtype": [C.c_int, C.c_int, C.POINTER(C.c_int)],
         "rstsr_hip_matmul": [vp, C.c_int, vp, P, vp, P, vp, P, vp, vp, C.c_int],
         "rstsr_hip_gemm": [vp, C.c_int, C.c_int, C.c_size_t, C.c_size_t, C.c_size_t, vp, vp, C.c_ssize_t,
                            C.c_ssize_t, vp, C.c_ssize_t, C.c_ssize_t, vp, vp, C.c_ssize_t, C.c_ssize_t],

@@ -27,6 +27,7 @@ DEVICE_SOURCES = [
     ("api.cpp", []),
     ("elementwise.hip", ["-ffp-contract=off"]),
     ("reduce.hip", ["-ffp-contract=off"]),
+    ("unary.hip", ["-ffp-contract=off"]),
     ("gemm.hip", []),
     ("ozaki.hip", []),
 ]

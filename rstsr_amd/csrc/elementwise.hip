@@ -1,23 +1,14 @@
-// Strided n-d elementwise kernels for gfx950: binary arithmetic, cast/copy,
-// fill and allclose, over layouts already broadcast to one shape.
+// Strided n-d elementwise operations for gfx950: binary arithmetic, cast/copy,
+// fill and allclose, over layouts already broadcast to one shape. The kernels
+// and the planner are in ew.hpp.
 //
 // Replaces the layout-iterator loops of rstsr-native-impl/src/cpu_rayon/
 // op_with_func.rs:13-390 (binary ops), assignment.rs:14-225 (assign, fill) and
-// reduction.rs:334 (allclose). Results are independent of iteration order, so
-// the planner below is free to pick the order that coalesces HBM traffic:
-//   1. drop size-1 axes, sort axes by the output's |stride| (innermost first)
-//      and merge axes that are contiguous for every operand;
-//   2. if an input's unit-stride axis differs from the output's, run the
-//      LDS-tiled kernel: 64x64 tiles read along the input's unit axis and
-//      written along the output's unit axis (both sides coalesced);
-//   3. else if the innermost extent is long, run the row kernel (one block per
-//      row chunk, index math once per block);
-//   4. else the generic kernel (per-element fast-division unravel).
+// reduction.rs:334 (allclose).
 // IEEE add/sub/mul/div are single correctly rounded operations; complex
 // multiply is (ac - bd, ad + bc) as num-complex computes it. This file is built
 // with -ffp-contract=off so that no product is fused into an FMA.
-#include "common.hpp"
-#include "registry.hpp"
+#include "ew.hpp"
 
 #include <array>
 #include <type_traits>
@@ -26,8 +17,6 @@ namespace rstsr {
 namespace hip {
 
 /* =============================== element ops =============================== */
-
-struct None {};
 
 template <class T>
 __device__ __forceinline__ T op_add(T a, T b) { return a + b; }
@@ -180,424 +169,6 @@ struct CloseBody {
     }
 };
 
-/* =============================== plan =============================== */
-
-struct EwPlan {
-    int nd = 0;
-    uint32_t shape[MAXD];
-    FastDiv fd[MAXD];
-    int64_t st[3][MAXD];  // element strides of c, a, b
-    int64_t off[3];       // element offsets of c, a, b
-    int nt[3];            // operand read once per launch (no zero stride on a longer axis): streaming loads
-};
-
-// The index math of one row/tile/element: offsets of (c, a, b) for a linear
-// index over dims [d0, nd).
-__device__ __forceinline__ void unravel3(const EwPlan& p, int d0, uint32_t lin, int64_t& oc, int64_t& oa, int64_t& ob) {
-    oc = p.off[0];
-    oa = p.off[1];
-    ob = p.off[2];
-#pragma unroll
-    for (int d = 0; d < MAXD; ++d) {
-        if (d < d0 || d >= p.nd) continue;
-        uint32_t q = fdiv(lin, p.fd[d]);
-        uint32_t r = lin - q * p.shape[d];
-        oc += (int64_t)r * p.st[0][d];
-        oa += (int64_t)r * p.st[1][d];
-        ob += (int64_t)r * p.st[2][d];
-        lin = q;
-    }
-}
-
-constexpr int EW_BLOCK = 256;
-constexpr int EW_VEC = 4;
-
-// generic: one linear element index per (thread, v), unravel per element
-template <class Body>
-__global__ void __launch_bounds__(EW_BLOCK) k_ew_generic(Body body, EwPlan p, uint32_t total) {
-    using TA = typename Body::TA;
-    using TB = typename Body::TB;
-    uint32_t base = blockIdx.x * (EW_BLOCK * EW_VEC) + threadIdx.x;
-    TA va[EW_VEC];
-    TB vb[EW_VEC];
-    int64_t oc[EW_VEC];
-#pragma unroll
-    for (int v = 0; v < EW_VEC; ++v) {
-        uint32_t i = base + v * EW_BLOCK;
-        if (i < total) {
-            int64_t oa, ob;
-            unravel3(p, 0, i, oc[v], oa, ob);
-            va[v] = body.la(oa);
-            vb[v] = body.lb(ob);
-        }
-    }
-#pragma unroll
-    for (int v = 0; v < EW_VEC; ++v) {
-        uint32_t i = base + v * EW_BLOCK;
-        if (i < total) body.st(oc[v], va[v], vb[v]);
-    }
-}
-
-// rows: dim 0 is a long row; blockIdx -> (row, chunk); unravel once per block
-template <class Body>
-__global__ void __launch_bounds__(EW_BLOCK) k_ew_rows(Body body, EwPlan p, uint32_t nchunk, FastDiv fchunk) {
-    using TA = typename Body::TA;
-    using TB = typename Body::TB;
-    uint32_t row = fdiv(blockIdx.x, fchunk);
-    uint32_t chunk = blockIdx.x - row * nchunk;
-    int64_t rc, ra, rb;
-    unravel3(p, 1, row, rc, ra, rb);
-    const uint32_t L = p.shape[0];
-    const int64_t sc = p.st[0][0], sa = p.st[1][0], sb = p.st[2][0];
-    uint32_t base = chunk * (EW_BLOCK * EW_VEC) + threadIdx.x;
-    TA va[EW_VEC];
-    TB vb[EW_VEC];
-    auto load = [&](auto nta, auto ntb) {
-#pragma unroll
-        for (int v = 0; v < EW_VEC; ++v) {
-            uint32_t i = base + v * EW_BLOCK;
-            if (i < L) {
-                va[v] = body.template la<decltype(nta)::value>(ra + (int64_t)i * sa);
-                vb[v] = body.template lb<decltype(ntb)::value>(rb + (int64_t)i * sb);
-            }
-        }
-    };
-    using Y = std::true_type;
-    using N = std::false_type;
-    if (p.nt[1] && p.nt[2]) load(Y{}, Y{});
-    else if (p.nt[1]) load(Y{}, N{});
-    else if (p.nt[2]) load(N{}, Y{});
-    else load(N{}, N{});
-#pragma unroll
-    for (int v = 0; v < EW_VEC; ++v) {
-        uint32_t i = base + v * EW_BLOCK;
-        if (i < L) body.st(rc + (int64_t)i * sc, va[v], vb[v]);
-    }
-}
-
-// tiled transpose: dims 0 (= dc, output unit axis) and DI (input unit axis)
-// form a TI (along DI) x TC (along dc) tile; remaining dims are unravelled from
-// the block index. Operands flagged in STAGE (bit0 = A, bit1 = B) are read
-// along DI into LDS. Each thread keeps one tile column (phase 1) / one tile
-// row position (phase 2) fixed, so per-element addressing is a constant stride
-// added to a per-thread base; interior tiles skip the bounds tests; an operand
-// that is broadcast along DI is loaded once per thread in phase 2.
-// ONE: at most one axis besides 0 and di (od, or -1 for none), so the outer
-// offset is one product instead of an unravel over every axis.
-template <class Body, int TI, int TC, int STAGE, bool ONE>
-__global__ void __launch_bounds__(EW_BLOCK) k_ew_tiled(Body body, EwPlan p, int di, int od, uint32_t nti,
-                                                       uint32_t ntc, FastDiv fti, FastDiv ftc) {
-    using TA = typename Body::TA;
-    using TB = typename Body::TB;
-    constexpr bool SA = (STAGE & 1) != 0;
-    constexpr bool SB = (STAGE & 2) != 0;
-    constexpr int ROWS1 = EW_BLOCK / TI, PER1 = TC / ROWS1;  // phase 1: TI lanes along DI
-    constexpr int ROWS2 = EW_BLOCK / TC, PER2 = TI / ROWS2;  // phase 2: TC lanes along dc
-    __shared__ TA lds_a[SA ? TC : 1][SA ? TI + 1 : 1];
-    __shared__ TB lds_b[SB ? TC : 1][SB ? TI + 1 : 1];
-
-    uint32_t b = blockIdx.x;
-    uint32_t q1 = fdiv(b, ftc);
-    uint32_t tc = b - q1 * ntc;
-    uint32_t outer = fdiv(q1, fti);
-    uint32_t ti = q1 - outer * nti;
-
-    // outer offsets: dims other than 0 and di, in order
-    int64_t oc = p.off[0], oa = p.off[1], ob = p.off[2];
-    if constexpr (ONE) {
-        if (od >= 0) {
-            oc += (int64_t)outer * p.st[0][od];
-            oa += (int64_t)outer * p.st[1][od];
-            ob += (int64_t)outer * p.st[2][od];
-        }
-    } else {
-        uint32_t lin = outer;
-#pragma unroll
-        for (int d = 1; d < MAXD; ++d) {
-            if (d >= p.nd || d == di) continue;
-            uint32_t q = fdiv(lin, p.fd[d]);
-            uint32_t r = lin - q * p.shape[d];
-            oc += (int64_t)r * p.st[0][d];
-            oa += (int64_t)r * p.st[1][d];
-            ob += (int64_t)r * p.st[2][d];
-            lin = q;
-        }
-    }
-    const uint32_t Nc = p.shape[0], Ni = p.shape[di];
-    const uint32_t c0 = tc * TC, i0 = ti * TI;
-    const int64_t sc_c = p.st[0][0], sc_i = p.st[0][di];
-    const int64_t sa_c = p.st[1][0], sa_i = p.st[1][di];
-    const int64_t sb_c = p.st[2][0], sb_i = p.st[2][di];
-    const bool full = (i0 + TI <= Ni) && (c0 + TC <= Nc);
-    const int tid = threadIdx.x;
-
-    if constexpr (SA || SB) {
-        // phase 1: ii = tid % TI fixed, cc = k*ROWS1 + tid/TI -> coalesced along the input axis
-        const int ii = tid % TI, cb = tid / TI;
-        const int64_t pa = oa + (int64_t)(i0 + ii) * sa_i + (int64_t)(c0 + cb) * sa_c, da = (int64_t)ROWS1 * sa_c;
-        const int64_t pb = ob + (int64_t)(i0 + ii) * sb_i + (int64_t)(c0 + cb) * sb_c, db = (int64_t)ROWS1 * sb_c;
-        if (full) {
-            // staged operands are read along their unit-stride axis, once: streaming
-            // loads unless the operand repeats along another axis
-            auto stage = [&](auto nta, auto ntb) {
-#pragma unroll
-                for (int k = 0; k < PER1; ++k) {
-                    if constexpr (SA) lds_a[k * ROWS1 + cb][ii] = body.template la<decltype(nta)::value>(pa + k * da);
-                    if constexpr (SB) lds_b[k * ROWS1 + cb][ii] = body.template lb<decltype(ntb)::value>(pb + k * db);
-                }
-            };
-            using Y = std::true_type;
-            using N = std::false_type;
-            const bool ya = SA && p.nt[1], yb = SB && p.nt[2];
-            if (ya && yb) stage(Y{}, Y{});
-            else if (ya) stage(Y{}, N{});
-            else if (yb) stage(N{}, Y{});
-            else stage(N{}, N{});
-        } else {
-#pragma unroll
-            for (int k = 0; k < PER1; ++k) {
-                if (i0 + ii < Ni && c0 + k * ROWS1 + cb < Nc) {
-                    if constexpr (SA) lds_a[k * ROWS1 + cb][ii] = body.la(pa + k * da);
-                    if constexpr (SB) lds_b[k * ROWS1 + cb][ii] = body.lb(pb + k * db);
-                }
-            }
-        }
-        __syncthreads();
-    }
-    // phase 2: cc = tid % TC fixed, ii = k*ROWS2 + tid/TC -> coalesced along the output axis
-    const int cc = tid % TC, ib = tid / TC;
-    const int64_t pc = oc + (int64_t)(c0 + cc) * sc_c + (int64_t)(i0 + ib) * sc_i, dc2 = (int64_t)ROWS2 * sc_i;
-    const int64_t qa = oa + (int64_t)(c0 + cc) * sa_c + (int64_t)(i0 + ib) * sa_i, da2 = (int64_t)ROWS2 * sa_i;
-    const int64_t qb = ob + (int64_t)(c0 + cc) * sb_c + (int64_t)(i0 + ib) * sb_i, db2 = (int64_t)ROWS2 * sb_i;
-    const bool col_ok = c0 + cc < Nc;
-    auto run = [&](auto hoist_b) {
-        constexpr bool HB = decltype(hoist_b)::value;
-        TB yb{};
-        if constexpr (HB) {
-            if (col_ok) yb = body.lb(qb);
-        }
-        auto one = [&](int k) {
-            TA x;
-            TB y;
-            if constexpr (SA) x = lds_a[cc][k * ROWS2 + ib];
-            else x = body.la(qa + k * da2);
-            if constexpr (SB) y = lds_b[cc][k * ROWS2 + ib];
-            else if constexpr (HB) y = yb;
-            else y = body.lb(qb + k * db2);
-            body.st(pc + k * dc2, x, y);
-        };
-        if (full) {
-#pragma unroll
-            for (int k = 0; k < PER2; ++k) one(k);
-        } else {
-#pragma unroll
-            for (int k = 0; k < PER2; ++k)
-                if (col_ok && i0 + k * ROWS2 + ib < Ni) one(k);
-        }
-    };
-    if (!SB && sb_i == 0) run(std::true_type{});
-    else run(std::false_type{});
-}
-
-/* =============================== host planning =============================== */
-
-struct Dims {
-    int nd = 0;  // -1: empty (no elements)
-    std::vector<uint64_t> shape;
-    std::vector<std::array<int64_t, 3>> st;  // per dim, strides of c, a, b
-    int64_t off[3] = {0, 0, 0};
-};
-
-// Collapse the common shape of up to three operands (missing operands carry
-// stride 0). Order: output |stride| ascending; merge axes contiguous in all.
-static Dims collapse3(const Shape& shape, const Stride* s[3], const size_t off[3]) {
-    Dims d;
-    for (int k = 0; k < 3; ++k) d.off[k] = (int64_t)off[k];
-    std::vector<size_t> ax;
-    for (size_t i = 0; i < shape.size(); ++i) {
-        if (shape[i] == 0) {
-            d.nd = -1;
-            return d;
-        }
-        if (shape[i] != 1) ax.push_back(i);
-    }
-    auto S = [&](int k, size_t i) -> int64_t { return s[k] ? (int64_t)(*s[k])[i] : 0; };
-    std::stable_sort(ax.begin(), ax.end(), [&](size_t x, size_t y) {
-        for (int k = 0; k < 3; ++k) {
-            int64_t ax_ = std::llabs(S(k, x)), ay = std::llabs(S(k, y));
-            if (ax_ != ay) return ax_ < ay;
-        }
-        return false;
-    });
-    for (size_t i : ax) {
-        if (!d.shape.empty()) {
-            auto& ps = d.st.back();
-            uint64_t pn = d.shape.back();
-            bool ok = true;
-            for (int k = 0; k < 3; ++k)
-                if (S(k, i) != ps[k] * (int64_t)pn) ok = false;
-            if (ok) {
-                d.shape.back() *= shape[i];
-                continue;
-            }
-        }
-        d.shape.push_back(shape[i]);
-        d.st.push_back({S(0, i), S(1, i), S(2, i)});
-    }
-    d.nd = (int)d.shape.size();
-    return d;
-}
-
-static EwPlan to_plan(const Dims& d) {
-    EwPlan p;
-    p.nd = d.nd;
-    if (d.nd > MAXD)
-        RSTSR_RAISE(UnImplemented, "layouts with more than 8 non-mergeable axes are not supported by the HIP device");
-    for (int i = 0; i < MAXD; ++i) {
-        p.shape[i] = 1;
-        p.fd[i] = FastDiv(1);
-        for (int k = 0; k < 3; ++k) p.st[k][i] = 0;
-    }
-    for (int i = 0; i < d.nd; ++i) {
-        p.shape[i] = (uint32_t)d.shape[i];
-        p.fd[i] = FastDiv((uint32_t)d.shape[i]);
-        for (int k = 0; k < 3; ++k) p.st[k][i] = d.st[i][k];
-    }
-    for (int k = 0; k < 3; ++k) {
-        p.off[k] = d.off[k];
-        p.nt[k] = 1;
-        for (int i = 0; i < d.nd; ++i)
-            if (d.shape[i] > 1 && d.st[i][k] == 0) p.nt[k] = 0;
-    }
-    return p;
-}
-
-enum class EwKind { Generic, Rows, Tiled };
-
-struct EwChoice {
-    EwKind kind = EwKind::Generic;
-    int di = -1;
-    int stage = 0;  // bit0 A, bit1 B
-};
-
-// has_a / has_b: operand present as an array (not scalar / absent)
-static EwChoice choose(const Dims& d, bool has_a, bool has_b) {
-    EwChoice ch;
-    uint64_t total = 1;
-    for (auto n : d.shape) total *= n;
-    if (d.nd <= 0 || total < 4096) return ch;
-    // input-preferred axis: a unit-stride axis of some input that is not the
-    // output's innermost axis
-    int best_di = -1;
-    for (int k = 1; k <= 2; ++k) {
-        if ((k == 1 && !has_a) || (k == 2 && !has_b)) continue;
-        if (std::llabs(d.st[0][k]) <= 1) continue;  // already contiguous along dim 0
-        for (int i = 1; i < d.nd; ++i) {
-            if (std::llabs(d.st[i][k]) == 1) {
-                if (best_di < 0) best_di = i;
-                break;
-            }
-        }
-    }
-    if (best_di > 0 && d.shape[0] >= 16 && d.shape[best_di] >= 16) {
-        ch.kind = EwKind::Tiled;
-        ch.di = best_di;
-        for (int k = 1; k <= 2; ++k) {
-            if ((k == 1 && !has_a) || (k == 2 && !has_b)) continue;
-            // stage only operands that walk the input axis (a broadcast along it reads one
-            // element per tile column, which phase 2 fetches directly)
-            int64_t si = std::llabs(d.st[best_di][k]), sc = std::llabs(d.st[0][k]);
-            if (si > 0 && si < sc) ch.stage |= (k == 1 ? 1 : 2);
-        }
-        return ch;
-    }
-    if (d.shape[0] >= (uint64_t)(EW_BLOCK * EW_VEC)) ch.kind = EwKind::Rows;
-    return ch;
-}
-
-// Launch `body` over the plan, splitting the outermost axis so that every
-// launch indexes fewer than 2^31 elements with 32-bit arithmetic.
-template <class Body>
-static void launch_ew(rstsr_hip_ctx* ctx, const Body& body, Dims d, const EwChoice& ch) {
-    if (d.nd < 0) return;
-    if (d.nd == 0) {
-        d.nd = 1;
-        d.shape.push_back(1);
-        d.st.push_back({0, 0, 0});
-    }
-    const uint64_t LIM = 1ull << 31;
-    for (auto n : d.shape)
-        if (n >= LIM) RSTSR_RAISE(UnImplemented, "an axis of 2^31 or more elements after merging is not supported");
-    uint64_t inner = 1;
-    for (int i = 0; i + 1 < d.nd; ++i) inner *= d.shape[i];
-    int last = d.nd - 1;
-    uint64_t nlast = d.shape[last];
-    uint64_t per = std::max<uint64_t>(1, (LIM / 2) / std::max<uint64_t>(inner, 1));
-    if (ch.kind == EwKind::Tiled && ch.di == last) per = nlast;  // tile axis itself is small enough
-    for (uint64_t s0 = 0; s0 < nlast; s0 += per) {
-        uint64_t cnt = std::min(per, nlast - s0);
-        Dims dd = d;
-        dd.shape[last] = cnt;
-        for (int k = 0; k < 3; ++k) dd.off[k] += (int64_t)s0 * d.st[last][k];
-        EwPlan p = to_plan(dd);
-        uint64_t total = inner * cnt;
-        if (ch.kind == EwKind::Generic) {
-            uint32_t nb = ceil_div_u32(total, EW_BLOCK * EW_VEC);
-            hipLaunchKernelGGL(k_ew_generic<Body>, dim3(nb), dim3(EW_BLOCK), 0, ctx->stream, body, p, (uint32_t)total);
-        } else if (ch.kind == EwKind::Rows) {
-            uint32_t nchunk = ceil_div_u32(dd.shape[0], EW_BLOCK * EW_VEC);
-            uint64_t nrows = total / dd.shape[0];
-            uint64_t nb = nrows * nchunk;
-            if (nb >= LIM) RSTSR_RAISE(UnImplemented, "grid too large");
-            hipLaunchKernelGGL(k_ew_rows<Body>, dim3((uint32_t)nb), dim3(EW_BLOCK), 0, ctx->stream, body, p, nchunk,
-                               FastDiv(nchunk));
-        } else {
-            // tiles: 64x64 for 8-byte elements, 128 (input axis) x 64 for 4-byte and smaller, 32x32 for 16-byte
-            constexpr size_t ES = sizeof(typename Body::TA) > sizeof(typename Body::TB) ? sizeof(typename Body::TA)
-                                                                                          : sizeof(typename Body::TB);
-            constexpr int TI = ES > 8 ? 32 : (ES > 4 ? 64 : 128);
-            constexpr int TC = ES > 8 ? 32 : 64;
-            uint32_t ntc = ceil_div_u32(dd.shape[0], TC);
-            uint32_t nti = ceil_div_u32(dd.shape[ch.di], TI);
-            uint64_t nouter = total / (dd.shape[0] * dd.shape[ch.di]);
-            uint64_t nb = nouter * nti * ntc;
-            if (nb >= LIM) RSTSR_RAISE(UnImplemented, "grid too large");
-            int od = -1;
-            for (int k = 1; k < dd.nd; ++k)
-                if (k != ch.di) od = k;
-            auto go = [&](auto stage, auto one) {
-                hipLaunchKernelGGL((k_ew_tiled<Body, TI, TC, decltype(stage)::value, decltype(one)::value>),
-                                   dim3((uint32_t)nb), dim3(EW_BLOCK), 0, ctx->stream, body, p, ch.di, od, nti, ntc,
-                                   FastDiv(nti), FastDiv(ntc));
-            };
-            auto by_stage = [&](auto one) {
-                switch (ch.stage) {
-                    case 1: go(std::integral_constant<int, 1>{}, one); break;
-                    case 2: go(std::integral_constant<int, 2>{}, one); break;
-                    default: go(std::integral_constant<int, 3>{}, one); break;
-                }
-            };
-            if (dd.nd <= 3) by_stage(std::true_type{});
-            else by_stage(std::false_type{});
-        }
-        check_launch("elementwise");
-    }
-}
-
-/* =============================== dispatch =============================== */
-
-template <class F>
-static void dispatch_dtype(rstsr_dtype_t t, F&& f) {
-    switch (t) {
-        case RSTSR_F32: f(float{}); break;
-        case RSTSR_F64: f(double{}); break;
-        case RSTSR_I32: f(int32_t{}); break;
-        case RSTSR_I64: f(int64_t{}); break;
-        case RSTSR_C64: f(c64{}); break;
-        case RSTSR_C128: f(c128{}); break;
-        default: RSTSR_RAISE(UnImplemented, "dtype not supported for this operation");
-    }
-}
 
 // Binary op with each of a / b an array or a host scalar (nullptr layout).
 static void binary_impl(rstsr_hip_ctx* ctx, rstsr_binop_t op, rstsr_dtype_t t, void* c, const rstsr_layout_t* lc_,
@@ -843,7 +414,7 @@ int rstsr_hip_fill(rstsr_hip_ctx_t* ctx, rstsr_dtype_t t, void* c, const rstsr_l
             using T = decltype(tag);
             FillBody<T> body{(T*)c, T{}};
             std::memcpy(&body.v, value, sizeof(T));
-            launch_ew(ctx, body, d, ch);
+            launch_ew<FillBody<T>, false>(ctx, body, d, ch);
         });
     });
 }
@@ -868,11 +439,10 @@ int rstsr_hip_allclose(rstsr_hip_ctx_t* ctx, rstsr_dtype_t t, const void* a, con
         size_t off[3] = {la.offset, la.offset, lb.offset};
         Dims d = collapse3(la.shape, s, off);
         EwChoice ch = choose(d, true, true);
-        if (ch.kind == EwKind::Tiled) ch.kind = EwKind::Generic;
         dispatch_dtype(t, [&](auto tag) {
             using T = decltype(tag);
             CloseBody<T> body{(const T*)a, (const T*)b, flag, rtol, atol, equal_nan};
-            launch_ew(ctx, body, d, ch);
+            launch_ew<CloseBody<T>, false>(ctx, body, d, ch);
         });
         HIP_CHECK(hipMemcpyAsync(ctx->host_pinned, flag, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
         HIP_CHECK(hipStreamSynchronize(ctx->stream));

@@ -103,6 +103,9 @@ PYBIND11_MODULE(_rstsr, m) {
                  return a.reduce_axes((rstsr_redop_t)op, axes);
              })
         .def("reduce_all", [](const Tensor& a, int op) { return py::bytes(a.reduce_all((rstsr_redop_t)op)); })
+        .def("unary", [](const Tensor& a, int op) { return a.unary((rstsr_unop_t)op); })
+        .def("unary_inplace", [](Tensor& a, int op) { a.unary_inplace((rstsr_unop_t)op); })
+        .def("unary_to", [](const Tensor& a, int op, Tensor& out) { a.unary_to((rstsr_unop_t)op, out); })
         .def("allclose", &Tensor::allclose)
         .def("matmul", [](const Tensor& a, const Tensor& b, int mode) { return a.matmul(b, (rstsr_gemm_mode_t)mode); });
 

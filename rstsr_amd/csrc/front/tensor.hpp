@@ -193,6 +193,29 @@ class Tensor {
         check(rstsr_hip_op_muta_numb(device().raw(), op, dtype, ptr(), &xc, s));
     }
 
+    /* ---------- unary (tensor/operators/op_unary_common.rs:141-190) ---------- */
+    // &tensor path: a new output laid out as layout_for_array_copy(self, K)
+    Tensor unary(rstsr_unop_t op) const {
+        rstsr_dtype_t to;
+        check(rstsr_hip_unary_out_dtype(op, dtype, &to));
+        Layout lc = layout_for_array_copy(layout, IterOrder::K);
+        Tensor c = uninit(device(), lc, to);
+        AbiLayout xc(lc), xa(layout);
+        check(rstsr_hip_unary(device().raw(), op, to, c.ptr(), &xc, dtype, ptr(), &xa));
+        return c;
+    }
+    // op_muta_refb into a caller-given output view (any layout of the same shape)
+    void unary_to(rstsr_unop_t op, Tensor& out) const {
+        if (!device().same_device(out.device())) RSTSR_RAISE(DeviceMismatch, "operands live on different devices");
+        AbiLayout xc(out.layout), xa(layout);
+        check(rstsr_hip_unary(device().raw(), op, out.dtype, out.ptr(), &xc, dtype, ptr(), &xa));
+    }
+    // owned-tensor path: op_muta on the tensor's own storage (same-type ops only)
+    void unary_inplace(rstsr_unop_t op) {
+        AbiLayout xc(layout);
+        check(rstsr_hip_unary_inplace(device().raw(), op, dtype, ptr(), &xc));
+    }
+
     /* ---------- reductions ---------- */
     Layout reduce_out_layout(const std::vector<ptrdiff_t>& axes) const {
         auto sp = layout.dim_split_axes(axes);

@@ -19,7 +19,18 @@ CODES = {
     np.dtype(np.complex64): 4,
     np.dtype(np.complex128): 5,
     np.dtype(np.uint64): 6,
+    np.dtype(np.bool_): 7,
 }
+DTYPES = {v: k for k, v in CODES.items()}
+
+# unary op codes (rstsr_unop_t order)
+UNARY_OPS = (
+    "acos", "acosh", "asin", "asinh", "atan", "atanh", "ceil", "conj", "cos", "cosh", "exp", "expm1",
+    "floor", "inv", "is_finite", "is_inf", "is_nan", "log", "log1p", "log2", "log10", "reciprocal",
+    "round", "signbit", "sin", "sinh", "square", "sqrt", "tan", "tanh", "trunc", "abs", "imag", "real",
+    "sign", "neg", "not_",
+)
+UN = {name: i for i, name in enumerate(UNARY_OPS)}
 
 
 class OLayout(C.Structure):
@@ -106,6 +117,8 @@ class Oracle:
         self.lib.b_reduce_axes.argtypes = [C.c_int, C.c_int, vp, L, C.c_size_t, C.POINTER(C.c_ssize_t), vp, L,
                                            C.c_int]
         self.lib.o_max_threads.restype = C.c_int
+        self.lib.o_unary.argtypes = [C.c_int, C.c_int, vp, L, C.c_int, vp, L]
+        self.lib.o_unary_out_dtype.argtypes = [C.c_int, C.c_int]
 
     # ---- wrappers taking View objects; outputs written into View bases ----
     def binary(self, op, c, a, b, scalar_b=None):
@@ -135,6 +148,28 @@ class Oracle:
         if r != 0:
             raise ValueError("oracle reduce_all error %d" % r)
         return out[0]
+
+    def unary_out_dtype(self, op, dtype):
+        """Output numpy dtype of unary `op` on `dtype`, or None when the reference has none."""
+        r = self.lib.o_unary_out_dtype(UN[op] if isinstance(op, str) else op, CODES[np.dtype(dtype)])
+        return None if r < 0 else DTYPES[r]
+
+    def unary(self, op, c, a):
+        """c = op(a) over two Views of one shape (c's dtype is the op's output dtype)."""
+        code = UN[op] if isinstance(op, str) else op
+        r = self.lib.o_unary(code, CODES[c.base.dtype], c.ptr, C.byref(c.c), CODES[a.base.dtype], a.ptr,
+                             C.byref(a.c))
+        if r != 0:
+            raise ValueError("oracle unary: no implementation for this op/dtype")
+
+    def unary_array(self, op, x):
+        """op(x) for a numpy array x (any strides); returns a new c-contiguous array."""
+        dt = self.unary_out_dtype(op, x.dtype)
+        if dt is None:
+            raise ValueError("oracle unary: no implementation for this op/dtype")
+        out = c_view(list(x.shape), dt)
+        self.unary(op, out, View.of(x))
+        return out.array().copy()
 
     def gemm_naive(self, A, B, alpha=1, beta=0, C0=None):
         """C = alpha*A@B + beta*C0 with the naive sequential-k order; A, B 2-D numpy (any strides)."""
