@@ -180,11 +180,50 @@ __device__ __forceinline__ T ld_nt(const T* p) {
     }
 }
 
-// Cached or streaming load chosen at compile time.
+// Cached or streaming load chosen at compile time. The choice must stay a
+// template argument: two runtime branches that differ only in the hint are
+// merged by the compiler, and the merged load keeps no hint.
 template <bool NT, class T>
 __device__ __forceinline__ T ld(const T* p) {
     if constexpr (NT) return ld_nt(p);
     else return *p;
+}
+
+// Store of an output written once and not read back soon (large outputs): the
+// nontemporal hint keeps it from displacing cached inputs.
+template <class T>
+__device__ __forceinline__ void st_nt(T* p, T v) {
+    if constexpr (std::is_arithmetic<T>::value) {
+        __builtin_nontemporal_store(v, p);
+    } else if constexpr (sizeof(T) == 16 && alignof(T) >= 16) {
+        typedef unsigned u4 __attribute__((ext_vector_type(4)));
+        u4 w;
+        __builtin_memcpy(&w, &v, 16);
+        __builtin_nontemporal_store(w, (u4*)p);
+    } else if constexpr (sizeof(T) % 8 == 0 && alignof(T) >= 8) {
+        typedef unsigned u2 __attribute__((ext_vector_type(2)));
+        u2* q = (u2*)p;
+#pragma unroll
+        for (size_t i = 0; i < sizeof(T) / 8; ++i) {
+            u2 w;
+            __builtin_memcpy(&w, (const char*)&v + 8 * i, 8);
+            __builtin_nontemporal_store(w, q + i);
+        }
+    } else if constexpr (sizeof(T) % 4 == 0 && alignof(T) >= 4) {
+        unsigned* q = (unsigned*)p;
+        const unsigned* s = (const unsigned*)&v;
+#pragma unroll
+        for (size_t i = 0; i < sizeof(T) / 4; ++i) __builtin_nontemporal_store(s[i], q + i);
+    } else {
+        *p = v;  // byte-sized elements: plain store
+    }
+}
+
+// Cached or streaming store chosen at compile time.
+template <bool NT, class T>
+__device__ __forceinline__ void sto(T* p, T v) {
+    if constexpr (NT) st_nt(p, v);
+    else *p = v;
 }
 
 /* ------------------------------ launch helpers ------------------------------ */

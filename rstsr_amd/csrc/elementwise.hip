@@ -94,11 +94,15 @@ struct BinBody {
     const T* a;
     const T* b;
     T sa, sb;
+    static constexpr int OPS = (AS ? 0 : 1) | (BS ? 0 : 2);
+    static constexpr size_t OUT_SIZE = sizeof(T);
+    static constexpr bool NT_JOINT = false;
     template <bool NT = false>
     __device__ __forceinline__ T la(int64_t o) const { return AS ? sa : ld<NT>(a + o); }
     template <bool NT = false>
     __device__ __forceinline__ T lb(int64_t o) const { return BS ? sb : ld<NT>(b + o); }
-    __device__ __forceinline__ void st(int64_t o, T x, T y) const { c[o] = apply_op<OP, T>(x, y); }
+    template <bool NT = false>
+    __device__ __forceinline__ void st(int64_t o, T x, T y) const { sto<NT>(c + o, apply_op<OP, T>(x, y)); }
 };
 
 template <class TCv, class TAv>
@@ -107,11 +111,15 @@ struct CastBody {
     using TB = None;
     TCv* c;
     const TAv* a;
+    static constexpr int OPS = 1;
+    static constexpr size_t OUT_SIZE = sizeof(TCv);
+    static constexpr bool NT_JOINT = false;
     template <bool NT = false>
     __device__ __forceinline__ TAv la(int64_t o) const { return ld<NT>(a + o); }
     template <bool NT = false>
     __device__ __forceinline__ None lb(int64_t) const { return None{}; }
-    __device__ __forceinline__ void st(int64_t o, TAv x, None) const { c[o] = cast_to<TCv, TAv>(x); }
+    template <bool NT = false>
+    __device__ __forceinline__ void st(int64_t o, TAv x, None) const { sto<NT>(c + o, cast_to<TCv, TAv>(x)); }
 };
 
 template <class T>
@@ -120,11 +128,15 @@ struct FillBody {
     using TB = None;
     T* c;
     T v;
+    static constexpr int OPS = 0;
+    static constexpr size_t OUT_SIZE = sizeof(T);
+    static constexpr bool NT_JOINT = false;
     template <bool NT = false>
     __device__ __forceinline__ None la(int64_t) const { return None{}; }
     template <bool NT = false>
     __device__ __forceinline__ None lb(int64_t) const { return None{}; }
-    __device__ __forceinline__ void st(int64_t o, None, None) const { c[o] = v; }
+    template <bool NT = false>
+    __device__ __forceinline__ void st(int64_t o, None, None) const { sto<NT>(c + o, v); }
 };
 
 template <class T>
@@ -158,10 +170,14 @@ struct CloseBody {
     int* flag;
     double rtol, atol;
     int equal_nan;
+    static constexpr int OPS = 3;
+    static constexpr size_t OUT_SIZE = 0;
+    static constexpr bool NT_JOINT = false;
     template <bool NT = false>
     __device__ __forceinline__ T la(int64_t o) const { return ld<NT>(a + o); }
     template <bool NT = false>
     __device__ __forceinline__ T lb(int64_t o) const { return ld<NT>(b + o); }
+    template <bool NT = false>
     __device__ __forceinline__ void st(int64_t, T x, T y) const {
         bool ok = abs_diff_d<T>(x, y) <= atol + rtol * abs_d<T>(y);
         if (equal_nan && is_nan_v<T>(x) && is_nan_v<T>(y)) ok = true;

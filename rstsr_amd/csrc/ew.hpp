@@ -37,7 +37,6 @@ struct EwPlan {
     FastDiv fd[MAXD];
     int64_t st[3][MAXD];  // element strides of c, a, b
     int64_t off[3];       // element offsets of c, a, b
-    int nt[3];            // operand read once per launch (no zero stride on a longer axis): streaming loads
 };
 
 // The index math of one row/tile/element: offsets of (c, a, b) for a linear
@@ -87,8 +86,10 @@ __global__ void __launch_bounds__(EW_BLOCK) k_ew_generic(Body body, EwPlan p, ui
     }
 }
 
-// rows: dim 0 is a long row; blockIdx -> (row, chunk); unravel once per block
-template <class Body>
+// rows: dim 0 is a long row; blockIdx -> (row, chunk); unravel once per block.
+// NTM: bit0 / bit1 streaming loads of A / B (operands read once), bit2 streaming
+// stores (large outputs).
+template <class Body, int NTM>
 __global__ void __launch_bounds__(EW_BLOCK) k_ew_rows(Body body, EwPlan p, uint32_t nchunk, FastDiv fchunk) {
     using TA = typename Body::TA;
     using TB = typename Body::TB;
@@ -101,26 +102,18 @@ __global__ void __launch_bounds__(EW_BLOCK) k_ew_rows(Body body, EwPlan p, uint3
     uint32_t base = chunk * (EW_BLOCK * EW_VEC) + threadIdx.x;
     TA va[EW_VEC];
     TB vb[EW_VEC];
-    auto load = [&](auto nta, auto ntb) {
-#pragma unroll
-        for (int v = 0; v < EW_VEC; ++v) {
-            uint32_t i = base + v * EW_BLOCK;
-            if (i < L) {
-                va[v] = body.template la<decltype(nta)::value>(ra + (int64_t)i * sa);
-                vb[v] = body.template lb<decltype(ntb)::value>(rb + (int64_t)i * sb);
-            }
-        }
-    };
-    using Y = std::true_type;
-    using N = std::false_type;
-    if (p.nt[1] && p.nt[2]) load(Y{}, Y{});
-    else if (p.nt[1]) load(Y{}, N{});
-    else if (p.nt[2]) load(N{}, Y{});
-    else load(N{}, N{});
 #pragma unroll
     for (int v = 0; v < EW_VEC; ++v) {
         uint32_t i = base + v * EW_BLOCK;
-        if (i < L) body.st(rc + (int64_t)i * sc, va[v], vb[v]);
+        if (i < L) {
+            va[v] = body.template la<(NTM & 1) != 0>(ra + (int64_t)i * sa);
+            vb[v] = body.template lb<(NTM & 2) != 0>(rb + (int64_t)i * sb);
+        }
+    }
+#pragma unroll
+    for (int v = 0; v < EW_VEC; ++v) {
+        uint32_t i = base + v * EW_BLOCK;
+        if (i < L) body.template st<(NTM & 4) != 0>(rc + (int64_t)i * sc, va[v], vb[v]);
     }
 }
 
@@ -133,7 +126,9 @@ __global__ void __launch_bounds__(EW_BLOCK) k_ew_rows(Body body, EwPlan p, uint3
 // that is broadcast along DI is loaded once per thread in phase 2.
 // ONE: at most one axis besides 0 and di (od, or -1 for none), so the outer
 // offset is one product instead of an unravel over every axis.
-template <class Body, int TI, int TC, int STAGE, bool ONE>
+// NTM: bit0 / bit1 streaming loads of the staged A / B in full tiles, bit2
+// streaming stores.
+template <class Body, int TI, int TC, int STAGE, bool ONE, int NTM>
 __global__ void __launch_bounds__(EW_BLOCK) k_ew_tiled(Body body, EwPlan p, int di, int od, uint32_t nti,
                                                        uint32_t ntc, FastDiv fti, FastDiv ftc) {
     using TA = typename Body::TA;
@@ -186,22 +181,11 @@ __global__ void __launch_bounds__(EW_BLOCK) k_ew_tiled(Body body, EwPlan p, int 
         const int64_t pa = oa + (int64_t)(i0 + ii) * sa_i + (int64_t)(c0 + cb) * sa_c, da = (int64_t)ROWS1 * sa_c;
         const int64_t pb = ob + (int64_t)(i0 + ii) * sb_i + (int64_t)(c0 + cb) * sb_c, db = (int64_t)ROWS1 * sb_c;
         if (full) {
-            // staged operands are read along their unit-stride axis, once: streaming
-            // loads unless the operand repeats along another axis
-            auto stage = [&](auto nta, auto ntb) {
 #pragma unroll
-                for (int k = 0; k < PER1; ++k) {
-                    if constexpr (SA) lds_a[k * ROWS1 + cb][ii] = body.template la<decltype(nta)::value>(pa + k * da);
-                    if constexpr (SB) lds_b[k * ROWS1 + cb][ii] = body.template lb<decltype(ntb)::value>(pb + k * db);
-                }
-            };
-            using Y = std::true_type;
-            using N = std::false_type;
-            const bool ya = SA && p.nt[1], yb = SB && p.nt[2];
-            if (ya && yb) stage(Y{}, Y{});
-            else if (ya) stage(Y{}, N{});
-            else if (yb) stage(N{}, Y{});
-            else stage(N{}, N{});
+            for (int k = 0; k < PER1; ++k) {
+                if constexpr (SA) lds_a[k * ROWS1 + cb][ii] = body.template la<(NTM & 1) != 0>(pa + k * da);
+                if constexpr (SB) lds_b[k * ROWS1 + cb][ii] = body.template lb<(NTM & 2) != 0>(pb + k * db);
+            }
         } else {
 #pragma unroll
             for (int k = 0; k < PER1; ++k) {
@@ -233,7 +217,7 @@ __global__ void __launch_bounds__(EW_BLOCK) k_ew_tiled(Body body, EwPlan p, int 
             if constexpr (SB) y = lds_b[cc][k * ROWS2 + ib];
             else if constexpr (HB) y = yb;
             else y = body.lb(qb + k * db2);
-            body.st(pc + k * dc2, x, y);
+            body.template st<(NTM & 4) != 0>(pc + k * dc2, x, y);
         };
         if (full) {
 #pragma unroll
@@ -312,12 +296,7 @@ static EwPlan to_plan(const Dims& d) {
         p.fd[i] = FastDiv((uint32_t)d.shape[i]);
         for (int k = 0; k < 3; ++k) p.st[k][i] = d.st[i][k];
     }
-    for (int k = 0; k < 3; ++k) {
-        p.off[k] = d.off[k];
-        p.nt[k] = 1;
-        for (int i = 0; i < d.nd; ++i)
-            if (d.shape[i] > 1 && d.st[i][k] == 0) p.nt[k] = 0;
-    }
+    for (int k = 0; k < 3; ++k) p.off[k] = d.off[k];
     return p;
 }
 
@@ -364,13 +343,42 @@ static EwChoice choose(const Dims& d, bool has_a, bool has_b) {
     return ch;
 }
 
+// Outputs at least this large are written with streaming stores: they cannot
+// stay in the 256 MB last-level cache for a following kernel anyway.
+constexpr uint64_t EW_NT_STORE_MIN = 128ull << 20;
+
+// Call f(std::integral_constant<int, v>) for the runtime value v in [0, 8),
+// instantiating only masks whose load bits are operands the body has.
+template <int OPS, class F>
+static void with_ntm(int v, F&& f) {
+    auto one = [&](auto c) {
+        constexpr int V = decltype(c)::value;
+        if constexpr (((V & 3) & ~OPS) == 0) f(c);
+    };
+    switch (v & 7) {
+        case 0: one(std::integral_constant<int, 0>{}); break;
+        case 1: one(std::integral_constant<int, 1>{}); break;
+        case 2: one(std::integral_constant<int, 2>{}); break;
+        case 3: one(std::integral_constant<int, 3>{}); break;
+        case 4: one(std::integral_constant<int, 4>{}); break;
+        case 5: one(std::integral_constant<int, 5>{}); break;
+        case 6: one(std::integral_constant<int, 6>{}); break;
+        default: one(std::integral_constant<int, 7>{}); break;
+    }
+}
+
 // Launch `body` over the plan, splitting the outermost axis so that every
 // launch indexes fewer than 2^31 elements with 32-bit arithmetic.
 // TILED = false: the LDS-tiled kernel is not instantiated for this body, and a
 // tiled choice runs the generic kernel instead.
+// Streaming hints (compile-time kernel variants): loads of operands read once,
+// stores of outputs of at least EW_NT_STORE_MIN bytes. Bodies with NT_JOINT
+// take both hints or neither (fewer variants).
 template <class Body, bool TILED = true>
 static void launch_ew(rstsr_hip_ctx* ctx, const Body& body, Dims d, EwChoice ch) {
+    constexpr int OPS = Body::OPS;
     if (!TILED && ch.kind == EwKind::Tiled) ch.kind = EwKind::Generic;
+    if (OPS == 0 && ch.kind == EwKind::Tiled) ch.kind = EwKind::Generic;
     if (d.nd < 0) return;
     if (d.nd == 0) {
         d.nd = 1;
@@ -380,6 +388,17 @@ static void launch_ew(rstsr_hip_ctx* ctx, const Body& body, Dims d, EwChoice ch)
     const uint64_t LIM = 1ull << 31;
     for (auto n : d.shape)
         if (n >= LIM) RSTSR_RAISE(UnImplemented, "an axis of 2^31 or more elements after merging is not supported");
+    uint64_t all = 1;
+    for (auto n : d.shape) all *= n;
+    bool once[3] = {true, true, true};
+    for (int k = 1; k < 3; ++k)
+        for (int i = 0; i < d.nd; ++i)
+            if (d.shape[i] > 1 && d.st[i][k] == 0) once[k] = false;
+    const bool big = Body::OUT_SIZE > 0 && all * Body::OUT_SIZE >= EW_NT_STORE_MIN;
+    int lmask = ((OPS & 1) && once[1] ? 1 : 0) | ((OPS & 2) && once[2] ? 2 : 0);
+    int rows_ntm = lmask | (big ? 4 : 0);
+    if (Body::NT_JOINT) rows_ntm = (lmask == OPS && big) ? (OPS | 4) : 0;
+
     uint64_t inner = 1;
     for (int i = 0; i + 1 < d.nd; ++i) inner *= d.shape[i];
     int last = d.nd - 1;
@@ -401,9 +420,17 @@ static void launch_ew(rstsr_hip_ctx* ctx, const Body& body, Dims d, EwChoice ch)
             uint64_t nrows = total / dd.shape[0];
             uint64_t nb = nrows * nchunk;
             if (nb >= LIM) RSTSR_RAISE(UnImplemented, "grid too large");
-            hipLaunchKernelGGL(k_ew_rows<Body>, dim3((uint32_t)nb), dim3(EW_BLOCK), 0, ctx->stream, body, p, nchunk,
-                               FastDiv(nchunk));
-        } else if constexpr (TILED) {
+            auto go = [&](auto ntm) {
+                hipLaunchKernelGGL((k_ew_rows<Body, decltype(ntm)::value>), dim3((uint32_t)nb), dim3(EW_BLOCK), 0,
+                                   ctx->stream, body, p, nchunk, FastDiv(nchunk));
+            };
+            if constexpr (Body::NT_JOINT) {
+                if (rows_ntm) go(std::integral_constant<int, OPS | 4>{});
+                else go(std::integral_constant<int, 0>{});
+            } else {
+                with_ntm<OPS>(rows_ntm, go);
+            }
+        } else if constexpr (TILED && OPS != 0) {
             // tiles: 64x64 for 8-byte elements, 128 (input axis) x 64 for 4-byte and smaller, 32x32 for 16-byte
             constexpr size_t ES = sizeof(typename Body::TA) > sizeof(typename Body::TB) ? sizeof(typename Body::TA)
                                                                                           : sizeof(typename Body::TB);
@@ -417,25 +444,41 @@ static void launch_ew(rstsr_hip_ctx* ctx, const Body& body, Dims d, EwChoice ch)
             int od = -1;
             for (int k = 1; k < dd.nd; ++k)
                 if (k != ch.di) od = k;
-            auto go = [&](auto stage, auto one) {
-                hipLaunchKernelGGL((k_ew_tiled<Body, TI, TC, decltype(stage)::value, decltype(one)::value>),
+            auto go = [&](auto stage, auto one, auto ntm) {
+                hipLaunchKernelGGL((k_ew_tiled<Body, TI, TC, decltype(stage)::value, decltype(one)::value,
+                                               decltype(ntm)::value>),
                                    dim3((uint32_t)nb), dim3(EW_BLOCK), 0, ctx->stream, body, p, ch.di, od, nti, ntc,
                                    FastDiv(nti), FastDiv(ntc));
             };
-            auto by_stage = [&](auto one) {
-                // a body without operand B only ever stages A (choose() tiles along an input's unit axis)
-                if constexpr (std::is_same<typename Body::TB, None>::value) {
-                    go(std::integral_constant<int, 1>{}, one);
-                } else {
-                    switch (ch.stage) {
-                        case 1: go(std::integral_constant<int, 1>{}, one); break;
-                        case 2: go(std::integral_constant<int, 2>{}, one); break;
-                        default: go(std::integral_constant<int, 3>{}, one); break;
+            // ONE (at most three axes): streaming variants, staged loads all-or-none; more axes: plain
+            auto by_nt = [&](auto stage) {
+                constexpr int S = decltype(stage)::value;
+                if (dd.nd <= 3) {
+                    const bool lo = ((S & 1) == 0 || once[1]) && ((S & 2) == 0 || once[2]);
+                    using Y = std::true_type;
+                    switch ((lo ? 1 : 0) | (big ? 2 : 0)) {
+                        case 0: go(stage, Y{}, std::integral_constant<int, 0>{}); break;
+                        case 1: go(stage, Y{}, std::integral_constant<int, S>{}); break;
+                        case 2: go(stage, Y{}, std::integral_constant<int, 4>{}); break;
+                        default: go(stage, Y{}, std::integral_constant<int, S | 4>{}); break;
                     }
+                } else {
+                    go(stage, std::false_type{}, std::integral_constant<int, 0>{});
                 }
             };
-            if (dd.nd <= 3) by_stage(std::true_type{});
-            else by_stage(std::false_type{});
+            // choose() tiles along an input's unit axis, so that input is always staged:
+            // a body with one array operand stages exactly that one
+            if constexpr (OPS == 1) {
+                by_nt(std::integral_constant<int, 1>{});
+            } else if constexpr (OPS == 2) {
+                by_nt(std::integral_constant<int, 2>{});
+            } else {
+                switch (ch.stage) {
+                    case 1: by_nt(std::integral_constant<int, 1>{}); break;
+                    case 2: by_nt(std::integral_constant<int, 2>{}); break;
+                    default: by_nt(std::integral_constant<int, 3>{}); break;
+                }
+            }
         }
         check_launch("elementwise");
     }

@@ -314,11 +314,15 @@ struct UnBody {
     using TB = None;
     TCv* c;
     const TAv* a;
+    static constexpr int OPS = 1;
+    static constexpr size_t OUT_SIZE = sizeof(TCv);
+    static constexpr bool NT_JOINT = true;
     template <bool NT = false>
     __device__ __forceinline__ TAv la(int64_t o) const { return ld<NT>(a + o); }
     template <bool NT = false>
     __device__ __forceinline__ None lb(int64_t) const { return None{}; }
-    __device__ __forceinline__ void st(int64_t o, TAv x, None) const { c[o] = un_apply<OP, TCv, TAv>(x); }
+    template <bool NT = false>
+    __device__ __forceinline__ void st(int64_t o, TAv x, None) const { sto<NT>(c + o, un_apply<OP, TCv, TAv>(x)); }
 };
 
 // integer -> f64 conversion (DTypeIntoFloatAPI::into_float = `as f64`)
@@ -328,11 +332,15 @@ struct ToF64Body {
     using TB = None;
     double* c;
     const TAv* a;
+    static constexpr int OPS = 1;
+    static constexpr size_t OUT_SIZE = sizeof(double);
+    static constexpr bool NT_JOINT = true;
     template <bool NT = false>
     __device__ __forceinline__ TAv la(int64_t o) const { return ld<NT>(a + o); }
     template <bool NT = false>
     __device__ __forceinline__ None lb(int64_t) const { return None{}; }
-    __device__ __forceinline__ void st(int64_t o, TAv x, None) const { c[o] = (double)x; }
+    template <bool NT = false>
+    __device__ __forceinline__ void st(int64_t o, TAv x, None) const { sto<NT>(c + o, (double)x); }
 };
 
 /* =============================== host side =============================== */
