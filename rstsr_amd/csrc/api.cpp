@@ -5,6 +5,8 @@
 // Every allocation made here is registered with its extent so that each kernel
 // entry point can prove, on the host, that the layouts it is given stay inside
 // the buffers they index before anything is launched.
+#include <cstdio>
+#include <cstdlib>
 #include <map>
 
 #include "common.hpp"
@@ -42,6 +44,12 @@ std::pair<uintptr_t, size_t> registry_find(const void* p) {
     --it;
     if ((uintptr_t)p >= it->first && (uintptr_t)p < it->first + std::max<size_t>(it->second, 1)) return *it;
     return {0, 0};
+}
+
+bool same_buffer(const void* x, const void* y) {
+    auto a = registry_find(x), b = registry_find(y);
+    if (a.second == 0 || b.second == 0) return true;
+    return a.first == b.first;
 }
 
 void check_extent(const void* ptr, const Layout& l, size_t elem, const char* name) {
@@ -108,6 +116,73 @@ static void* grow(rstsr_hip_ctx* ctx, Workspace& w, size_t bytes) {
     w.ptr = p;
     w.bytes = nb;
     return p;
+}
+
+bool stream_capturing(rstsr_hip_ctx* ctx) {
+    hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+    HIP_CHECK(hipStreamIsCapturing(ctx->stream, &st));
+    return st != hipStreamCaptureStatusNone;
+}
+
+static std::mutex g_tune_mu;
+static std::map<std::string, int>& tune_table() {
+    static std::map<std::string, int> t;
+    static bool loaded = false;
+    if (!loaded) {
+        loaded = true;
+        if (const char* f = std::getenv("RSTSR_HIP_TUNE_CACHE")) {
+            if (FILE* fp = std::fopen(f, "r")) {
+                char key[256];
+                int v;
+                while (std::fscanf(fp, "%255s %d", key, &v) == 2) t[key] = v;
+                std::fclose(fp);
+            }
+        }
+    }
+    return t;
+}
+
+int tune_select(rstsr_hip_ctx* ctx, const std::string& key, int ncand, int heuristic, bool can_time,
+                const std::function<void(int)>& run) {
+    {
+        std::lock_guard<std::mutex> g(g_tune_mu);
+        auto& t = tune_table();
+        auto it = t.find(key);
+        if (it != t.end() && it->second < ncand) {
+            run(it->second);
+            return it->second;
+        }
+    }
+    if (!can_time || ncand == 1 || stream_capturing(ctx) || std::getenv("RSTSR_HIP_NO_TUNE")) {
+        run(heuristic);
+        return heuristic;
+    }
+    // one untimed run first: the first touch of the operands (TLB and cache
+    // warm-up) would otherwise be charged to whichever candidate runs first
+    run(heuristic);
+    float best = 1e30f;
+    int bi = heuristic;
+    for (int c = 0; c < ncand; ++c) {
+        HIP_CHECK(hipEventRecord(ctx->events[14], ctx->stream));
+        run(c);
+        HIP_CHECK(hipEventRecord(ctx->events[15], ctx->stream));
+        HIP_CHECK(hipEventSynchronize(ctx->events[15]));
+        float ms = 0;
+        HIP_CHECK(hipEventElapsedTime(&ms, ctx->events[14], ctx->events[15]));
+        if (ms < best) {
+            best = ms;
+            bi = c;
+        }
+    }
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    tune_table()[key] = bi;
+    if (const char* f = std::getenv("RSTSR_HIP_TUNE_CACHE")) {
+        if (FILE* fp = std::fopen(f, "a")) {
+            std::fprintf(fp, "%s %d\n", key.c_str(), bi);
+            std::fclose(fp);
+        }
+    }
+    return bi;
 }
 
 void* workspace(rstsr_hip_ctx* ctx, size_t bytes) { return grow(ctx, ctx->ws, bytes); }

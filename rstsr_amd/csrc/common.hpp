@@ -6,6 +6,7 @@
 
 #include <cstdint>
 #include <cstring>
+#include <functional>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -89,6 +90,19 @@ inline void bind(rstsr_hip_ctx* ctx) {
 // captured graph keeps valid addresses.
 void* workspace(rstsr_hip_ctx* ctx, size_t bytes);
 void* aux_buffer(rstsr_hip_ctx* ctx, size_t bytes);
+
+// True while the context stream is being captured into a graph.
+bool stream_capturing(rstsr_hip_ctx* ctx);
+
+// Autotuner: run(c) performs the whole call with candidate c (all candidates give
+// identical results). The first call for `key` with can_time runs the heuristic
+// once untimed (operand warm-up), then times every candidate with events on the
+// context stream and keeps the fastest; later calls use the cached choice. No
+// timing while the stream is captured, when can_time is false (the output is
+// also an input) or under RSTSR_HIP_NO_TUNE: then the cached choice or the
+// heuristic runs. RSTSR_HIP_TUNE_CACHE=<file> loads and appends choices.
+int tune_select(rstsr_hip_ctx* ctx, const std::string& key, int ncand, int heuristic, bool can_time,
+                const std::function<void(int)>& run);
 
 /* ------------------------------ element types ------------------------------ */
 
@@ -224,6 +238,14 @@ template <bool NT, class T>
 __device__ __forceinline__ void sto(T* p, T v) {
     if constexpr (NT) st_nt(p, v);
     else *p = v;
+}
+
+// Block id -> work index such that each of the 8 XCDs (blocks are dispatched to
+// them round-robin) runs one contiguous range of work items: neighbouring tiles
+// then share that XCD's L2 and DRAM pages. A bijection on [0, n) for any n.
+__device__ __forceinline__ uint32_t xcd_chunk(uint32_t t, uint32_t n) {
+    const uint32_t q = n / 8, r = n % 8, x = t % 8, i = t / 8;
+    return x < r ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
 }
 
 /* ------------------------------ launch helpers ------------------------------ */

@@ -208,6 +208,8 @@ static void binary_impl(rstsr_hip_ctx* ctx, rstsr_binop_t op, rstsr_dtype_t t, v
     size_t off[3] = {lc.offset, has_a ? la.offset : 0, has_b ? lb.offset : 0};
     Dims d = collapse3(lc.shape, s, off);
     EwChoice ch = choose(d, has_a, has_b);
+    // tile-order tuning reruns the op: only when the output is not also an input
+    const bool can_time = (!has_a || !same_buffer(c, a)) && (!has_b || !same_buffer(c, b));
     dispatch_dtype(t, [&](auto tag) {
         using T = decltype(tag);
         auto run = [&](auto opc) {
@@ -220,13 +222,13 @@ static void binary_impl(rstsr_hip_ctx* ctx, rstsr_binop_t op, rstsr_dtype_t t, v
                 if (!has_b) std::memcpy(&sb, b, sizeof(T));
                 if (has_a && has_b) {
                     BinBody<OP, T, false, false> body{(T*)c, (const T*)a, (const T*)b, sa, sb};
-                    launch_ew(ctx, body, d, ch);
+                    launch_ew(ctx, body, d, ch, can_time);
                 } else if (has_a) {
                     BinBody<OP, T, false, true> body{(T*)c, (const T*)a, nullptr, sa, sb};
-                    launch_ew(ctx, body, d, ch);
+                    launch_ew(ctx, body, d, ch, can_time);
                 } else if (has_b) {
                     BinBody<OP, T, true, false> body{(T*)c, nullptr, (const T*)b, sa, sb};
-                    launch_ew(ctx, body, d, ch);
+                    launch_ew(ctx, body, d, ch, can_time);
                 } else {
                     BinBody<OP, T, true, true> body{(T*)c, nullptr, nullptr, sa, sb};
                     launch_ew(ctx, body, d, ch);
@@ -330,11 +332,12 @@ static void assign_impl(rstsr_hip_ctx* ctx, rstsr_dtype_t tc, void* c, const Lay
     size_t off[3] = {lc.offset, la.offset, 0};
     Dims d = collapse3(lc.shape, s, off);
     EwChoice ch = choose(d, true, false);
+    const bool can_time = !same_buffer(c, a);
     dispatch_pair(tc, ta, [&](auto ctag, auto atag) {
         using TC = decltype(ctag);
         using TA = decltype(atag);
         CastBody<TC, TA> body{(TC*)c, (const TA*)a};
-        launch_ew(ctx, body, d, ch);
+        launch_ew(ctx, body, d, ch, can_time);
     });
 }
 

@@ -21,6 +21,7 @@
 #include <algorithm>
 #include <array>
 #include <cstdlib>
+#include <string>
 #include <type_traits>
 #include <vector>
 
@@ -130,7 +131,7 @@ __global__ void __launch_bounds__(EW_BLOCK) k_ew_rows(Body body, EwPlan p, uint3
 // streaming stores.
 template <class Body, int TI, int TC, int STAGE, bool ONE, int NTM>
 __global__ void __launch_bounds__(EW_BLOCK) k_ew_tiled(Body body, EwPlan p, int di, int od, uint32_t nti,
-                                                       uint32_t ntc, FastDiv fti, FastDiv ftc) {
+                                                       uint32_t ntc, FastDiv fti, FastDiv ftc, int xmap) {
     using TA = typename Body::TA;
     using TB = typename Body::TB;
     constexpr bool SA = (STAGE & 1) != 0;
@@ -140,11 +141,21 @@ __global__ void __launch_bounds__(EW_BLOCK) k_ew_tiled(Body body, EwPlan p, int 
     __shared__ TA lds_a[SA ? TC : 1][SA ? TI + 1 : 1];
     __shared__ TB lds_b[SB ? TC : 1][SB ? TI + 1 : 1];
 
-    uint32_t b = blockIdx.x;
-    uint32_t q1 = fdiv(b, ftc);
-    uint32_t tc = b - q1 * ntc;
-    uint32_t outer = fdiv(q1, fti);
-    uint32_t ti = q1 - outer * nti;
+    // tile order (autotuned per problem class): 0 output-axis tiles fastest,
+    // 1 the same in one contiguous range per XCD, 2 input-axis tiles fastest
+    uint32_t b = xmap == 1 ? xcd_chunk(blockIdx.x, gridDim.x) : blockIdx.x;
+    uint32_t tc, ti, outer;
+    if (xmap == 2) {
+        uint32_t q1 = fdiv(b, fti);
+        ti = b - q1 * nti;
+        outer = fdiv(q1, ftc);
+        tc = q1 - outer * ntc;
+    } else {
+        uint32_t q1 = fdiv(b, ftc);
+        tc = b - q1 * ntc;
+        outer = fdiv(q1, fti);
+        ti = q1 - outer * nti;
+    }
 
     // outer offsets: dims other than 0 and di, in order
     int64_t oc = p.off[0], oa = p.off[1], ob = p.off[2];
@@ -181,10 +192,14 @@ __global__ void __launch_bounds__(EW_BLOCK) k_ew_tiled(Body body, EwPlan p, int 
         const int64_t pa = oa + (int64_t)(i0 + ii) * sa_i + (int64_t)(c0 + cb) * sa_c, da = (int64_t)ROWS1 * sa_c;
         const int64_t pb = ob + (int64_t)(i0 + ii) * sb_i + (int64_t)(c0 + cb) * sb_c, db = (int64_t)ROWS1 * sb_c;
         if (full) {
+            // offsets advance by a uniform stride: one 64-bit add per load
+            int64_t ua = pa, ub = pb;
 #pragma unroll
             for (int k = 0; k < PER1; ++k) {
-                if constexpr (SA) lds_a[k * ROWS1 + cb][ii] = body.template la<(NTM & 1) != 0>(pa + k * da);
-                if constexpr (SB) lds_b[k * ROWS1 + cb][ii] = body.template lb<(NTM & 2) != 0>(pb + k * db);
+                if constexpr (SA) lds_a[k * ROWS1 + cb][ii] = body.template la<(NTM & 1) != 0>(ua);
+                if constexpr (SB) lds_b[k * ROWS1 + cb][ii] = body.template lb<(NTM & 2) != 0>(ub);
+                ua += da;
+                ub += db;
             }
         } else {
 #pragma unroll
@@ -220,8 +235,21 @@ __global__ void __launch_bounds__(EW_BLOCK) k_ew_tiled(Body body, EwPlan p, int 
             body.template st<(NTM & 4) != 0>(pc + k * dc2, x, y);
         };
         if (full) {
+            int64_t uc = pc, ua = qa, ub = qb;
 #pragma unroll
-            for (int k = 0; k < PER2; ++k) one(k);
+            for (int k = 0; k < PER2; ++k) {
+                TA x;
+                TB y;
+                if constexpr (SA) x = lds_a[cc][k * ROWS2 + ib];
+                else x = body.la(ua);
+                if constexpr (SB) y = lds_b[cc][k * ROWS2 + ib];
+                else if constexpr (HB) y = yb;
+                else y = body.lb(ub);
+                body.template st<(NTM & 4) != 0>(uc, x, y);
+                uc += dc2;
+                ua += da2;
+                ub += db2;
+            }
         } else {
 #pragma unroll
             for (int k = 0; k < PER2; ++k)
@@ -346,6 +374,8 @@ static EwChoice choose(const Dims& d, bool has_a, bool has_b) {
 // Outputs at least this large are written with streaming stores: they cannot
 // stay in the 256 MB last-level cache for a following kernel anyway.
 constexpr uint64_t EW_NT_STORE_MIN = 128ull << 20;
+// Tiled launches moving at least this many bytes have their tile order tuned.
+constexpr uint64_t EW_TUNE_MIN = 32ull << 20;
 
 // Call f(std::integral_constant<int, v>) for the runtime value v in [0, 8),
 // instantiating only masks whose load bits are operands the body has.
@@ -375,7 +405,7 @@ static void with_ntm(int v, F&& f) {
 // stores of outputs of at least EW_NT_STORE_MIN bytes. Bodies with NT_JOINT
 // take both hints or neither (fewer variants).
 template <class Body, bool TILED = true>
-static void launch_ew(rstsr_hip_ctx* ctx, const Body& body, Dims d, EwChoice ch) {
+static void launch_ew(rstsr_hip_ctx* ctx, const Body& body, Dims d, EwChoice ch, bool can_time = false) {
     constexpr int OPS = Body::OPS;
     if (!TILED && ch.kind == EwKind::Tiled) ch.kind = EwKind::Generic;
     if (OPS == 0 && ch.kind == EwKind::Tiled) ch.kind = EwKind::Generic;
@@ -444,11 +474,13 @@ static void launch_ew(rstsr_hip_ctx* ctx, const Body& body, Dims d, EwChoice ch)
             int od = -1;
             for (int k = 1; k < dd.nd; ++k)
                 if (k != ch.di) od = k;
+
+            int xmap = 0;
             auto go = [&](auto stage, auto one, auto ntm) {
                 hipLaunchKernelGGL((k_ew_tiled<Body, TI, TC, decltype(stage)::value, decltype(one)::value,
                                                decltype(ntm)::value>),
                                    dim3((uint32_t)nb), dim3(EW_BLOCK), 0, ctx->stream, body, p, ch.di, od, nti, ntc,
-                                   FastDiv(nti), FastDiv(ntc));
+                                   FastDiv(nti), FastDiv(ntc), xmap);
             };
             // ONE (at most three axes): streaming variants, staged loads all-or-none; more axes: plain
             auto by_nt = [&](auto stage) {
@@ -468,16 +500,36 @@ static void launch_ew(rstsr_hip_ctx* ctx, const Body& body, Dims d, EwChoice ch)
             };
             // choose() tiles along an input's unit axis, so that input is always staged:
             // a body with one array operand stages exactly that one
-            if constexpr (OPS == 1) {
-                by_nt(std::integral_constant<int, 1>{});
-            } else if constexpr (OPS == 2) {
-                by_nt(std::integral_constant<int, 2>{});
-            } else {
-                switch (ch.stage) {
-                    case 1: by_nt(std::integral_constant<int, 1>{}); break;
-                    case 2: by_nt(std::integral_constant<int, 2>{}); break;
-                    default: by_nt(std::integral_constant<int, 3>{}); break;
+            auto launch = [&]() {
+                if constexpr (OPS == 1) {
+                    by_nt(std::integral_constant<int, 1>{});
+                } else if constexpr (OPS == 2) {
+                    by_nt(std::integral_constant<int, 2>{});
+                } else {
+                    switch (ch.stage) {
+                        case 1: by_nt(std::integral_constant<int, 1>{}); break;
+                        case 2: by_nt(std::integral_constant<int, 2>{}); break;
+                        default: by_nt(std::integral_constant<int, 3>{}); break;
+                    }
                 }
+            };
+            // The fastest tile order depends on how the shapes and strides meet the
+            // HBM channel interleave (measured: either order can win by 5-10 %), so it
+            // is tuned per problem class on launches of at least EW_TUNE_MIN bytes.
+            const uint64_t bytes = total * (sizeof(typename Body::TA) + Body::OUT_SIZE);
+            if (can_time && per >= nlast && bytes >= EW_TUNE_MIN) {
+                std::string key = "ew|" + std::to_string(sizeof(typename Body::TA)) + "|" +
+                                  std::to_string(Body::OUT_SIZE) + "|" + std::to_string(ch.stage) + "|" +
+                                  std::to_string(ntc) + "|" + std::to_string(nti) + "|" + std::to_string(nouter) +
+                                  "|" + std::to_string(dd.st[0][1] + dd.st[0][2]) + "|" +
+                                  std::to_string(dd.st[ch.di][0]);
+                tune_select(ctx, key, 3, 0, true, [&](int c) {
+                    xmap = c;
+                    launch();
+                    check_launch("elementwise");
+                });
+            } else {
+                launch();
             }
         }
         check_launch("elementwise");

@@ -25,6 +25,8 @@
 #include "registry.hpp"
 #include "gemm.hpp"
 
+#include <functional>
+
 namespace rstsr {
 namespace hip {
 
@@ -451,85 +453,21 @@ using Z1 = MCfg<2, 2, 2, 2>;  // c128 64 x 64
 using Z2 = MCfg<2, 4, 2, 2>;  // c128 64 x 128, 8 waves of 32 x 32
 constexpr int NCAND_D = 4, NCAND_Z = 3;
 
-// The autotuner times every candidate configuration once per problem class
-// (sizes bucketed to powers of two, layout flags, batch bucket) on the real
-// call and keeps the fastest. Timing runs only when C is write-only (beta == 0)
-// and the stream is not being captured; otherwise the cached choice or the
-// size heuristic is used. RSTSR_HIP_TUNE_CACHE=<file> persists choices.
-static std::mutex g_tune_mu;
-static std::map<std::string, int>& tune_table() {
-    static std::map<std::string, int> t;
-    static bool loaded = false;
-    if (!loaded) {
-        loaded = true;
-        if (const char* f = std::getenv("RSTSR_HIP_TUNE_CACHE")) {
-            if (FILE* fp = std::fopen(f, "r")) {
-                char key[256];
-                int v;
-                while (std::fscanf(fp, "%255s %d", key, &v) == 2) t[key] = v;
-                std::fclose(fp);
-            }
-        }
-    }
-    return t;
-}
-
+// The autotuner (tune_select, api.cpp) times every candidate configuration once
+// per problem class (sizes bucketed to powers of two, layout flags, batch bucket)
+// on the real call and keeps the fastest. Timing runs only when C is write-only
+// (beta == 0) and the stream is not being captured; otherwise the cached choice
+// or the size heuristic is used. RSTSR_HIP_TUNE_CACHE=<file> persists choices.
 static int bucket(int64_t x) {
     int b = 0;
     while ((1ll << b) < x) ++b;
     return b;
 }
 
-static bool stream_capturing(rstsr_hip_ctx* ctx) {
-    hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
-    HIP_CHECK(hipStreamIsCapturing(ctx->stream, &st));
-    return st != hipStreamCaptureStatusNone;
-}
-
 // run(c) performs the whole call with configuration c. Returns the chosen index.
 template <class F>
 static int tuned_run(rstsr_hip_ctx* ctx, const std::string& key, int ncand, int heuristic, bool can_time, F&& run) {
-    {
-        std::lock_guard<std::mutex> g(g_tune_mu);
-        auto& t = tune_table();
-        auto it = t.find(key);
-        if (it != t.end() && it->second < ncand) {
-            run(it->second);
-            return it->second;
-        }
-    }
-    if (!can_time || ncand == 1 || stream_capturing(ctx) || std::getenv("RSTSR_HIP_NO_TUNE")) {
-        run(heuristic);
-        return heuristic;
-    }
-    // one untimed run first: the first touch of the operands (TLB and cache
-    // warm-up) would otherwise be charged to whichever candidate runs first
-    run(heuristic);
-    float best = 1e30f;
-    int bi = heuristic;
-    for (int c = 0; c < ncand; ++c) {
-        HIP_CHECK(hipEventRecord(ctx->events[14], ctx->stream));
-        run(c);
-        HIP_CHECK(hipEventRecord(ctx->events[15], ctx->stream));
-        HIP_CHECK(hipEventSynchronize(ctx->events[15]));
-        float ms = 0;
-        HIP_CHECK(hipEventElapsedTime(&ms, ctx->events[14], ctx->events[15]));
-        if (ms < best) {
-            best = ms;
-            bi = c;
-        }
-    }
-    // the last candidate's output is already in C; every configuration produces
-    // identical bits (the K order per element does not depend on the tile shape)
-    std::lock_guard<std::mutex> g(g_tune_mu);
-    tune_table()[key] = bi;
-    if (const char* f = std::getenv("RSTSR_HIP_TUNE_CACHE")) {
-        if (FILE* fp = std::fopen(f, "a")) {
-            std::fprintf(fp, "%s %d\n", key.c_str(), bi);
-            std::fclose(fp);
-        }
-    }
-    return bi;
+    return tune_select(ctx, key, ncand, heuristic, can_time, std::function<void(int)>(run));
 }
 
 template <class T, class CF>

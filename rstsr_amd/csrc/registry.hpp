@@ -13,6 +13,8 @@ namespace hip {
 void registry_add(void* p, size_t bytes);
 bool registry_remove(void* p);
 std::pair<uintptr_t, size_t> registry_find(const void* p);
+// True when x and y lie in one registered buffer, or when either is not found.
+bool same_buffer(const void* x, const void* y);
 // Raise unless every element `l` addresses (elements of `elem` bytes from ptr)
 // lies inside one buffer returned by rstsr_hip_malloc.
 void check_extent(const void* ptr, const Layout& l, size_t elem, const char* name);

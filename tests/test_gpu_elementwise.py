@@ -205,3 +205,49 @@ def test_errors_are_raised_on_host(dev):
     c = rt.asarray(np.zeros((3, 4), dtype=np.float32), device=dev)
     with pytest.raises(rt.RSTSRError):
         a + c
+
+
+_ORDER_CHILD = r"""
+import sys, numpy as np
+sys.path.insert(0, sys.argv[1])
+import rstsr_amd as rt
+dev = rt.DeviceHIP(0)
+rng = np.random.default_rng(int(sys.argv[2]))
+shape, perm = (4096, 2304), (1, 0)
+X = rng.standard_normal(shape)
+Xb = rng.standard_normal((1, shape[1]))
+t = rt.asarray(X, device=dev).transpose(list(perm))
+tb = rt.asarray(Xb, device=dev).transpose(list(perm))
+ok = True
+for _ in range(2):  # first call tunes (or reads the cache), second uses the choice
+    ok &= np.array_equal(t.to_contig().to_numpy(), np.ascontiguousarray(X.T))
+    ok &= np.array_equal((t * tb).to_numpy(), X.T * Xb.T)
+dev.synchronize()
+print("ORDER_OK" if ok else "ORDER_BAD", flush=True)
+"""
+
+
+def test_tiled_tile_orders_all_exact(tmp_path):
+    """Every tile order the tuner can pick gives the exact transposed copy and broadcast product:
+    a first process tunes and records its choices in a cache file; fresh processes then run with
+    each recorded choice forced to order 0, 1 and 2 (the cache file is the tuner's own format)."""
+    import os
+    import subprocess
+    import sys
+
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    cache = tmp_path / "tune.txt"
+    env = dict(os.environ, RSTSR_HIP_TUNE_CACHE=str(cache))
+
+    def child(seed):
+        r = subprocess.run([sys.executable, "-c", _ORDER_CHILD, root, str(seed)], env=env, capture_output=True,
+                           text=True, timeout=120)
+        assert r.returncode == 0, r.stderr[-2000:]
+        return r.stdout
+
+    assert "ORDER_OK" in child(1)
+    lines = [ln.split() for ln in cache.read_text().splitlines() if ln.startswith("ew|")]
+    assert len(lines) >= 1, "the tiled launch was not tuned"
+    for order in (0, 1, 2):
+        cache.write_text("".join(f"{k} {order}\n" for k, _ in lines))
+        assert "ORDER_OK" in child(10 + order), order
