@@ -162,16 +162,19 @@ int tune_select(rstsr_hip_ctx* ctx, const std::string& key, int ncand, int heuri
     run(heuristic);
     float best = 1e30f;
     int bi = heuristic;
-    for (int c = 0; c < ncand; ++c) {
-        HIP_CHECK(hipEventRecord(ctx->events[14], ctx->stream));
-        run(c);
-        HIP_CHECK(hipEventRecord(ctx->events[15], ctx->stream));
-        HIP_CHECK(hipEventSynchronize(ctx->events[15]));
-        float ms = 0;
-        HIP_CHECK(hipEventElapsedTime(&ms, ctx->events[14], ctx->events[15]));
-        if (ms < best) {
-            best = ms;
-            bi = c;
+    // two rounds over the candidates, each candidate's best time counts
+    for (int round = 0; round < 2; ++round) {
+        for (int c = 0; c < ncand; ++c) {
+            HIP_CHECK(hipEventRecord(ctx->events[14], ctx->stream));
+            run(c);
+            HIP_CHECK(hipEventRecord(ctx->events[15], ctx->stream));
+            HIP_CHECK(hipEventSynchronize(ctx->events[15]));
+            float ms = 0;
+            HIP_CHECK(hipEventElapsedTime(&ms, ctx->events[14], ctx->events[15]));
+            if (ms < best) {
+                best = ms;
+                bi = c;
+            }
         }
     }
     std::lock_guard<std::mutex> g(g_tune_mu);

@@ -96,8 +96,8 @@ bool stream_capturing(rstsr_hip_ctx* ctx);
 
 // Autotuner: run(c) performs the whole call with candidate c (all candidates give
 // identical results). The first call for `key` with can_time runs the heuristic
-// once untimed (operand warm-up), then times every candidate with events on the
-// context stream and keeps the fastest; later calls use the cached choice. No
+// once untimed (operand warm-up), then times every candidate twice with events on
+// the context stream and keeps the fastest; later calls use the cached choice. No
 // timing while the stream is captured, when can_time is false (the output is
 // also an input) or under RSTSR_HIP_NO_TUNE: then the cached choice or the
 // heuristic runs. RSTSR_HIP_TUNE_CACHE=<file> loads and appends choices.

@@ -365,11 +365,22 @@ __device__ __forceinline__ void row_offsets(const RedPlan& p, uint64_t u, int64_
 }
 
 constexpr int RB = 256;  // threads per block (4 waves)
+constexpr int COLU = 4;  // reduced positions loaded per iteration by the column kernel
 
-template <class A, class T>
+// VEC consecutive elements loaded as one aligned access
+template <class T, int VEC>
+struct alignas(sizeof(T) * VEC) Pack {
+    T v[VEC];
+};
+
+// One wave per (output, split) over rows of the unit-stride reduced axis; each
+// lane loads VEC consecutive elements per access (16-byte loads when the host
+// proved alignment), four accesses in flight per iteration.
+template <class A, class T, int VEC>
 __global__ void __launch_bounds__(RB) k_red_row(const T* __restrict__ a, typename A::Out* out,
                                                 typename A::S* part, RedPlan p) {
     using S = typename A::S;
+    using vt = Pack<T, VEC>;
     const uint32_t wave = blockIdx.x * (RB / 64) + (threadIdx.x >> 6);
     const uint32_t lane = threadIdx.x & 63;
     const uint64_t nwork = (uint64_t)p.M * p.nsplit;
@@ -384,8 +395,18 @@ __global__ void __launch_bounds__(RB) k_red_row(const T* __restrict__ a, typenam
     const int64_t s0 = p.rin[0];
     const uint64_t l0 = (uint64_t)p.rlog[0];
     const uint32_t L0 = (uint32_t)p.rshape[0];
+    constexpr uint32_t W = 64 * VEC;  // elements per wave access
     auto sweep = [&](auto nt) {
         constexpr bool NT = decltype(nt)::value;
+        auto get = [&](const T* base, uint32_t e) -> vt {
+            if constexpr (VEC == 1) {
+                vt x;
+                x.v[0] = ld<NT>(base + (int64_t)e * s0);
+                return x;
+            } else {
+                return ld<NT>((const vt*)(base + e));  // s0 == 1 on this path
+            }
+        };
         for (uint64_t u = u0; u < u1; ++u) {
             uint64_t row = u / p.nseg0;
             uint32_t seg = (uint32_t)(u - row * p.nseg0);
@@ -393,41 +414,51 @@ __global__ void __launch_bounds__(RB) k_red_row(const T* __restrict__ a, typenam
             uint64_t rli;
             row_offsets(p, row, roff, rli);
             const T* base = a + oin + roff;
-            uint32_t e = seg * p.seg0 + lane;
+            uint32_t e = seg * p.seg0 + lane * VEC;
             uint32_t eend = min(L0, (seg + 1) * p.seg0);
             if constexpr (is_arg<A>()) {
-                // positions e increase with the logical index inside a row (l0 > 0)
+                // positions increase with the logical index inside a row (l0 > 0)
                 T bv{};
                 uint32_t be = ~0u;
-                for (; e + 192 < eend; e += 256) {
-                    T x0 = ld<NT>(base + (int64_t)e * s0);
-                    T x1 = ld<NT>(base + (int64_t)(e + 64) * s0);
-                    T x2 = ld<NT>(base + (int64_t)(e + 128) * s0);
-                    T x3 = ld<NT>(base + (int64_t)(e + 192) * s0);
-                    A::scan(x0, e, bv, be);
-                    A::scan(x1, e + 64, bv, be);
-                    A::scan(x2, e + 128, bv, be);
-                    A::scan(x3, e + 192, bv, be);
+                for (; e + 3 * W < eend; e += 4 * W) {
+                    vt x0 = get(base, e), x1 = get(base, e + W), x2 = get(base, e + 2 * W), x3 = get(base, e + 3 * W);
+#pragma unroll
+                    for (int v = 0; v < VEC; ++v) A::scan(x0.v[v], e + v, bv, be);
+#pragma unroll
+                    for (int v = 0; v < VEC; ++v) A::scan(x1.v[v], e + W + v, bv, be);
+#pragma unroll
+                    for (int v = 0; v < VEC; ++v) A::scan(x2.v[v], e + 2 * W + v, bv, be);
+#pragma unroll
+                    for (int v = 0; v < VEC; ++v) A::scan(x3.v[v], e + 3 * W + v, bv, be);
                 }
-                for (; e < eend; e += 64) A::scan(ld<NT>(base + (int64_t)e * s0), e, bv, be);
+                for (; e < eend; e += W) {
+                    vt x = get(base, e);
+#pragma unroll
+                    for (int v = 0; v < VEC; ++v) A::scan(x.v[v], e + v, bv, be);
+                }
                 if (be != ~0u) A::add(acc, bv, rli + (uint64_t)be * l0);
                 continue;
             }
-            for (; e + 192 < eend; e += 256) {
-                T x0 = ld<NT>(base + (int64_t)e * s0);
-                T x1 = ld<NT>(base + (int64_t)(e + 64) * s0);
-                T x2 = ld<NT>(base + (int64_t)(e + 128) * s0);
-                T x3 = ld<NT>(base + (int64_t)(e + 192) * s0);
-                A::add(acc, x0, rli + (uint64_t)e * l0);
-                A::add(acc, x1, rli + (uint64_t)(e + 64) * l0);
-                A::add(acc, x2, rli + (uint64_t)(e + 128) * l0);
-                A::add(acc, x3, rli + (uint64_t)(e + 192) * l0);
+            for (; e + 3 * W < eend; e += 4 * W) {
+                vt x0 = get(base, e), x1 = get(base, e + W), x2 = get(base, e + 2 * W), x3 = get(base, e + 3 * W);
+#pragma unroll
+                for (int v = 0; v < VEC; ++v) A::add(acc, x0.v[v], rli + (uint64_t)(e + v) * l0);
+#pragma unroll
+                for (int v = 0; v < VEC; ++v) A::add(acc, x1.v[v], rli + (uint64_t)(e + W + v) * l0);
+#pragma unroll
+                for (int v = 0; v < VEC; ++v) A::add(acc, x2.v[v], rli + (uint64_t)(e + 2 * W + v) * l0);
+#pragma unroll
+                for (int v = 0; v < VEC; ++v) A::add(acc, x3.v[v], rli + (uint64_t)(e + 3 * W + v) * l0);
             }
-            for (; e < eend; e += 64) A::add(acc, ld<NT>(base + (int64_t)e * s0), rli + (uint64_t)e * l0);
+            for (; e < eend; e += W) {
+                vt x = get(base, e);
+#pragma unroll
+                for (int v = 0; v < VEC; ++v) A::add(acc, x.v[v], rli + (uint64_t)(e + v) * l0);
+            }
         }
     };
-    // streaming loads pay off for 8-byte and wider elements; narrower ones measured slower
-    if (sizeof(T) >= 8 && p.stream) sweep(std::true_type{});
+    // streaming loads pay off for accesses of 8 bytes and wider; 4-byte ones measured slower
+    if (sizeof(T) * VEC >= 8 && p.stream) sweep(std::true_type{});
     else sweep(std::false_type{});
 #pragma unroll
     for (int m = 32; m >= 1; m >>= 1) {
@@ -440,12 +471,6 @@ __global__ void __launch_bounds__(RB) k_red_row(const T* __restrict__ a, typenam
         else part[(uint64_t)o * p.nsplit + sp] = acc;
     }
 }
-
-// VEC consecutive elements loaded as one aligned access
-template <class T, int VEC>
-struct alignas(sizeof(T) * VEC) Pack {
-    T v[VEC];
-};
 
 // VEC adjacent outputs per thread along the unit-stride kept axis (VEC * sizeof(T)
 // = 16 B vector loads when the host proved alignment), sequential over the reduced range.
@@ -499,18 +524,14 @@ __global__ void __launch_bounds__(RB) k_red_col(const T* __restrict__ a, typenam
                 uint32_t be[VEC];
 #pragma unroll
                 for (int v = 0; v < VEC; ++v) be[v] = ~0u;
-                for (; j + 4 <= n; j += 4) {
-                    vt x0 = ld<NT>((const vt*)(rowp + (int64_t)j * s0));
-                    vt x1 = ld<NT>((const vt*)(rowp + (int64_t)(j + 1) * s0));
-                    vt x2 = ld<NT>((const vt*)(rowp + (int64_t)(j + 2) * s0));
-                    vt x3 = ld<NT>((const vt*)(rowp + (int64_t)(j + 3) * s0));
+                for (; j + COLU <= n; j += COLU) {
+                    vt x[COLU];
 #pragma unroll
-                    for (int v = 0; v < VEC; ++v) {
-                        A::scan(x0.v[v], (uint32_t)j, bv[v], be[v]);
-                        A::scan(x1.v[v], (uint32_t)j + 1, bv[v], be[v]);
-                        A::scan(x2.v[v], (uint32_t)j + 2, bv[v], be[v]);
-                        A::scan(x3.v[v], (uint32_t)j + 3, bv[v], be[v]);
-                    }
+                    for (int q = 0; q < COLU; ++q) x[q] = ld<NT>((const vt*)(rowp + (int64_t)(j + q) * s0));
+#pragma unroll
+                    for (int q = 0; q < COLU; ++q)
+#pragma unroll
+                        for (int v = 0; v < VEC; ++v) A::scan(x[q].v[v], (uint32_t)(j + q), bv[v], be[v]);
                 }
                 for (; j < n; ++j) {
                     vt x = ld<NT>((const vt*)(rowp + (int64_t)j * s0));
@@ -522,18 +543,14 @@ __global__ void __launch_bounds__(RB) k_red_col(const T* __restrict__ a, typenam
                     if (be[v] != ~0u) A::add(acc[v], bv[v], li + (uint64_t)be[v] * l0);
                 return;
             }
-            for (; j + 4 <= n; j += 4) {
-                vt x0 = ld<NT>((const vt*)(rowp + (int64_t)j * s0));
-                vt x1 = ld<NT>((const vt*)(rowp + (int64_t)(j + 1) * s0));
-                vt x2 = ld<NT>((const vt*)(rowp + (int64_t)(j + 2) * s0));
-                vt x3 = ld<NT>((const vt*)(rowp + (int64_t)(j + 3) * s0));
+            for (; j + COLU <= n; j += COLU) {
+                vt x[COLU];
 #pragma unroll
-                for (int v = 0; v < VEC; ++v) {
-                    A::add(acc[v], x0.v[v], li + j * l0);
-                    A::add(acc[v], x1.v[v], li + (j + 1) * l0);
-                    A::add(acc[v], x2.v[v], li + (j + 2) * l0);
-                    A::add(acc[v], x3.v[v], li + (j + 3) * l0);
-                }
+                for (int q = 0; q < COLU; ++q) x[q] = ld<NT>((const vt*)(rowp + (int64_t)(j + q) * s0));
+#pragma unroll
+                for (int q = 0; q < COLU; ++q)
+#pragma unroll
+                    for (int v = 0; v < VEC; ++v) A::add(acc[v], x[q].v[v], li + (j + q) * l0);
             }
             for (; j < n; ++j) {
                 vt x = ld<NT>((const vt*)(rowp + (int64_t)j * s0));
@@ -543,7 +560,7 @@ __global__ void __launch_bounds__(RB) k_red_col(const T* __restrict__ a, typenam
         };
         while (remaining > 0) {
             uint64_t n = min(remaining, p.rshape[0] - coord[0]);
-            if (sizeof(T) >= 8 && p.stream) run_row(std::true_type{}, n);
+            if (sizeof(T) * VEC >= 8 && p.stream) run_row(std::true_type{}, n);
             else run_row(std::false_type{}, n);
             remaining -= n;
             if (remaining == 0) break;
@@ -608,7 +625,9 @@ struct RedSetup {
 };
 
 // la: input layout; axes: reduced axes (given order); lo: output layout (kept shape)
-static RedSetup reduce_plan(rstsr_hip_ctx* ctx, const Layout& la, const std::vector<ptrdiff_t>& axes, const Layout& lo) {
+// esize: element bytes, which sets the column kernel's vector width (16-byte loads)
+static RedSetup reduce_plan(rstsr_hip_ctx* ctx, const Layout& la, const std::vector<ptrdiff_t>& axes, const Layout& lo,
+                            size_t esize) {
     RedSetup s;
     auto sp = la.dim_split_axes(axes);
     const Layout& ls = sp.first;
@@ -724,7 +743,10 @@ static RedSetup reduce_plan(rstsr_hip_ctx* ctx, const Layout& la, const std::vec
         s.kind = RedKind::Col;
         // argmin/argmax track positions along reduced axis 0 in 32 bits
         if (p.rshape[0] >= 0xffffffffull) RSTSR_RAISE(UnImplemented, "reduced extent too large");
-        uint64_t want = std::max<uint64_t>(1, (target_waves * 64) / M);
+        // about 8 waves per CU (measured: more splits lose DRAM locality along the
+        // reduced walk, fewer lose latency hiding); a thread covers VEC outputs
+        const uint64_t vec = (esize == 4 || esize == 8) ? 16 / esize : 1;
+        uint64_t want = std::max<uint64_t>(1, ((uint64_t)std::max(ctx->num_cu, 1) * 8 * 64 * vec) / M);
         uint64_t nsplit = std::min<uint64_t>(want, std::max<uint64_t>(1, R / 64));
         nsplit = std::min<uint64_t>(nsplit, 65536);
         p.upw = (R + nsplit - 1) / nsplit;
@@ -742,13 +764,21 @@ static void run_reduce(rstsr_hip_ctx* ctx, const RedSetup& s, const T* a, typena
     using S = typename A::S;
     S* part = nullptr;
     if (p.nsplit > 1) part = (S*)workspace(ctx, (size_t)p.M * p.nsplit * sizeof(S));
+    // 16-byte loads when every address a thread touches is 16-byte aligned
+    constexpr int V16 = (std::is_arithmetic<T>::value && (sizeof(T) == 8 || sizeof(T) == 4)) ? (int)(16 / sizeof(T)) : 1;
     if (s.kind == RedKind::Row) {
         uint64_t nwaves = (uint64_t)p.M * p.nsplit;
         uint32_t nb = ceil_div_u32(nwaves, RB / 64);
-        hipLaunchKernelGGL((k_red_row<A, T>), dim3(nb), dim3(RB), 0, ctx->stream, a, out, part, p);
+        // rows start 16-byte aligned and run in whole 16-byte words
+        bool vec = V16 > 1 && p.rin[0] == 1 && p.rshape[0] % V16 == 0 && p.seg0 % V16 == 0 &&
+                   (p.in_off % V16) == 0 && ((uintptr_t)a % 16) == 0;
+        for (int d = 0; d < p.nk && vec; ++d) vec = (p.kin[d] % V16) == 0;
+        for (int d = 1; d < p.nr && vec; ++d) vec = (p.rin[d] % V16) == 0;
+        if (vec)
+            hipLaunchKernelGGL((k_red_row<A, T, V16>), dim3(nb), dim3(RB), 0, ctx->stream, a, out, part, p);
+        else
+            hipLaunchKernelGGL((k_red_row<A, T, 1>), dim3(nb), dim3(RB), 0, ctx->stream, a, out, part, p);
     } else {
-        // 16-byte loads when every address a thread touches is 16-byte aligned
-        constexpr int V16 = (std::is_arithmetic<T>::value && (sizeof(T) == 8 || sizeof(T) == 4)) ? (int)(16 / sizeof(T)) : 1;
         bool vec = V16 > 1 && p.nk >= 1 && p.kin[0] == 1 && p.kshape[0] % V16 == 0 && p.M % V16 == 0 &&
                    (p.in_off % V16) == 0 && ((uintptr_t)a % 16) == 0;
         for (int d = 1; d < p.nk && vec; ++d) vec = (p.kin[d] % V16) == 0;
@@ -920,7 +950,7 @@ static void reduce_axes_impl(rstsr_hip_ctx* ctx, rstsr_redop_t op, rstsr_dtype_t
             RSTSR_RAISE(InvalidValue, "zero-size array is not supported for max/min");
         RSTSR_RAISE(InvalidLayout, "empty sequence is not allowed for reduce_arg.");
     }
-    RedSetup s = reduce_plan(ctx, la, axes, lo);
+    RedSetup s = reduce_plan(ctx, la, axes, lo, dtype_size(t));
     if (s.empty_out) return;
     if (s.empty_reduce) {
         // outputs exist but each reduces zero elements: every output is the empty value
@@ -1046,7 +1076,7 @@ static void reduce_all_common(rstsr_hip_ctx_t* ctx, rstsr_redop_t op, rstsr_dtyp
         return;
     }
     Layout lo({}, {}, 0);
-    s = reduce_plan(ctx, la, all_axes(la.ndim()), lo);
+    s = reduce_plan(ctx, la, all_axes(la.ndim()), lo, dtype_size(t));
     size_t part_bytes = 0;
     if (s.p.nsplit > 1) part_bytes = (size_t)s.p.M * s.p.nsplit * 64;  // upper bound of sizeof(S)
     char* ws = (char*)workspace(ctx, part_bytes + 256);
