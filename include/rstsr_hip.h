@@ -258,6 +258,23 @@ typedef struct {
 int rstsr_hip_gemm_grouped(rstsr_hip_ctx_t* ctx, rstsr_dtype_t t, rstsr_gemm_mode_t mode, size_t count,
                            const rstsr_gemm_desc_t* descs, const void* alpha, const void* beta);
 
+/* ---- BLAS-style drivers (cblas flag values of rstsr-common/src/flags.rs:62-202: order 101 C / 102 F,
+ * trans 111 N / 112 T / 113 C (conjugate transpose) / 114 CN (conjugate only), uplo 121 U / 122 L,
+ * side 141 L / 142 R). Dtypes f32, f64, c64, c128; mode as for rstsr_hip_gemm. ---- */
+/* GEMMDriverAPI::driver_gemm (rstsr-blas-traits/src/blas3/gemm.rs:4-20):
+ * C (m x n) = alpha op(A) op(B) + beta C, leading dimensions as in BLAS (at least the stored row
+ * length for order C, column length for order F). Conjugated complex operands are copied
+ * conjugated into context scratch first. */
+int rstsr_hip_driver_gemm(rstsr_hip_ctx_t* ctx, rstsr_dtype_t t, rstsr_gemm_mode_t mode, int order, int transa,
+                          int transb, size_t m, size_t n, size_t k, const void* alpha, const void* a, size_t lda,
+                          const void* b, size_t ldb, const void* beta, void* c, size_t ldc);
+/* SYHEMMDriverAPI::driver_syhemm (blas3/syhemm.rs:4-19): C = alpha A B + beta C (side L, A m x m) or
+ * alpha B A + beta C (side R, A n x n) with A symmetric (hermi 0) or Hermitian (hermi 1); only
+ * the uplo triangle of A is read (and only the real part of a Hermitian diagonal). */
+int rstsr_hip_driver_syhemm(rstsr_hip_ctx_t* ctx, rstsr_dtype_t t, rstsr_gemm_mode_t mode, int hermi, int order,
+                            int side, int uplo, size_t m, size_t n, const void* alpha, const void* a, size_t lda,
+                            const void* b, size_t ldb, const void* beta, void* c, size_t ldc);
+
 /* ---- hipGraph capture of the context stream (no counterpart in the reference) ----
  * Everything enqueued between begin and end is recorded, not run; entry points that hand
  * values to the host (*_all, memcpy_d2h, allclose) cannot be captured and fail. */

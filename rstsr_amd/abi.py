@@ -60,6 +60,10 @@ def lib():
         "rstsr_hip_matmul": [vp, C.c_int, vp, P, vp, P, vp, P, vp, vp, C.c_int],
         "rstsr_hip_gemm": [vp, C.c_int, C.c_int, C.c_size_t, C.c_size_t, C.c_size_t, vp, vp, C.c_ssize_t,
                            C.c_ssize_t, vp, C.c_ssize_t, C.c_ssize_t, vp, vp, C.c_ssize_t, C.c_ssize_t],
+        "rstsr_hip_driver_gemm": [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_size_t, C.c_size_t,
+                                  C.c_size_t, vp, vp, C.c_size_t, vp, C.c_size_t, vp, vp, C.c_size_t],
+        "rstsr_hip_driver_syhemm": [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_size_t,
+                                    C.c_size_t, vp, vp, C.c_size_t, vp, C.c_size_t, vp, vp, C.c_size_t],
         "rstsr_hip_event_record": [vp, C.c_int],
         "rstsr_hip_event_elapsed": [vp, C.c_int, C.c_int, C.POINTER(C.c_float)],
         "rstsr_hip_synchronize": [vp],

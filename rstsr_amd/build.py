@@ -30,6 +30,7 @@ DEVICE_SOURCES = [
     ("unary.hip", ["-ffp-contract=off"]),
     ("gemm.hip", []),
     ("ozaki.hip", []),
+    ("blas.hip", []),
 ]
 
 HIP_FLAGS = [

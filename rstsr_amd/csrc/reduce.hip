@@ -457,8 +457,9 @@ __global__ void __launch_bounds__(RB) k_red_row(const T* __restrict__ a, typenam
             }
         }
     };
-    // streaming loads pay off for accesses of 8 bytes and wider; 4-byte ones measured slower
-    if (sizeof(T) * VEC >= 8 && p.stream) sweep(std::true_type{});
+    // streaming loads pay off for 8-byte and wider elements; narrower ones (also as
+    // 16-byte vectors) measured slower
+    if (sizeof(T) >= 8 && p.stream) sweep(std::true_type{});
     else sweep(std::false_type{});
 #pragma unroll
     for (int m = 32; m >= 1; m >>= 1) {
@@ -560,7 +561,7 @@ __global__ void __launch_bounds__(RB) k_red_col(const T* __restrict__ a, typenam
         };
         while (remaining > 0) {
             uint64_t n = min(remaining, p.rshape[0] - coord[0]);
-            if (sizeof(T) * VEC >= 8 && p.stream) run_row(std::true_type{}, n);
+            if (sizeof(T) >= 8 && p.stream) run_row(std::true_type{}, n);
             else run_row(std::false_type{}, n);
             remaining -= n;
             if (remaining == 0) break;
