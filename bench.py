@@ -52,6 +52,7 @@ def parse():
     p.add_argument("--gemm-modes", default="plain,ozaki")
     p.add_argument("--gemm-only", action="store_true", help="skip the strided suite (profiling the GEMM legs)")
     p.add_argument("--gemm-legs", default="dense,batch,ri,grouped,chain")
+    p.add_argument("--no-transfer", action="store_true")
     return p.parse_args()
 
 
@@ -268,6 +269,12 @@ def main():
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 4), "traffic": traffic,
                 "algorithmic_bytes_per_launch": dom.nbytes, "avg_launch_ms": dms}
 
+    # ---------------- host <-> device transfers (PCIe-inclusive; not part of the step) ----------------
+    transfer = {}
+    if not args.no_transfer and not args.only:
+        transfer = transfer_leg(dev)
+        log("[bench] transfer: " + json.dumps(transfer))
+
     # ---------------- FP64 GEMM leg ----------------
     gemm = {}
     if not args.no_gemm and not args.only:
@@ -300,6 +307,7 @@ def main():
         "ops": per_op,
         "parity": parity,
         "gemm": gemm,
+        "transfer": transfer,
     }
     print(json.dumps(out), flush=True)
 
@@ -515,6 +523,48 @@ def chain_leg(dev, rt, orc, modes, n=4096, steps=20):
         log(f"[bench] chain {mode}: eager {eager_ms:.3f} ms/step, graph {graph_ms:.3f} ms/step, "
             f"err {err_eager:.2e}/{err_graph:.2e}")
         del keep, g, holder
+    return res
+
+
+def transfer_leg(dev, nbytes=1 << 30, reps=3):
+    """Host <-> device GB/s for 1 GiB: pageable memory through the HIP runtime (mode 1), through the
+    library's pinned double-buffered staging (mode 2), and from pinned memory (hipHostMalloc) by
+    direct DMA; wall clock around each synchronous call, best of `reps`."""
+    from rstsr_amd import abi
+
+    L = abi.lib()
+    ctx = C.c_void_p(dev._ctx_ptr())
+    d = C.c_void_p()
+    abi.check(L.rstsr_hip_malloc(ctx, nbytes, C.byref(d)))
+    host = np.ones(nbytes // 8)
+    hp = host.ctypes.data_as(C.c_void_p)
+    hip = C.CDLL("libamdhip64.so")
+    pinned = C.c_void_p()
+    res = {"bytes": nbytes}
+    try:
+        if hip.hipHostMalloc(C.byref(pinned), C.c_size_t(nbytes), C.c_uint(0)) != 0:
+            pinned = None
+
+        def best(fn):
+            ts = []
+            for _ in range(reps):
+                t0 = time.perf_counter()
+                fn()
+                ts.append(time.perf_counter() - t0)
+            return round(nbytes / 1e9 / min(ts), 1)
+
+        for name, mode in (("runtime", 1), ("staged", 2)):
+            res[f"h2d_{name}_GBps"] = best(lambda: abi.check(L.rstsr_hip_memcpy_h2d_mode(ctx, d, hp, nbytes, mode)))
+            res[f"d2h_{name}_GBps"] = best(lambda: abi.check(L.rstsr_hip_memcpy_d2h_mode(ctx, hp, d, nbytes, mode)))
+        if pinned:
+            res["h2d_pinned_GBps"] = best(lambda: abi.check(L.rstsr_hip_memcpy_h2d(ctx, d, pinned, nbytes)))
+            res["d2h_pinned_GBps"] = best(lambda: abi.check(L.rstsr_hip_memcpy_d2h(ctx, pinned, d, nbytes)))
+        res["h2d_default_GBps"] = best(lambda: abi.check(L.rstsr_hip_memcpy_h2d(ctx, d, hp, nbytes)))
+        res["d2h_default_GBps"] = best(lambda: abi.check(L.rstsr_hip_memcpy_d2h(ctx, hp, d, nbytes)))
+    finally:
+        if pinned:
+            hip.hipHostFree(pinned)
+        abi.check(L.rstsr_hip_free(ctx, d))
     return res
 
 

@@ -155,6 +155,12 @@ int rstsr_hip_free(rstsr_hip_ctx_t* ctx, void* ptr);
 int rstsr_hip_memcpy_h2d(rstsr_hip_ctx_t* ctx, void* dst, const void* src, size_t nbytes);
 /* to_cpu_vec / into_cpu_vec / get_index (device -> host, synchronous). */
 int rstsr_hip_memcpy_d2h(rstsr_hip_ctx_t* ctx, void* dst, const void* src, size_t nbytes);
+/* Host transfers above run pageable copies of 8 MiB and more through two pinned 32 MiB chunks
+ * (the host copy of one overlaps the DMA of the other); pinned host memory goes by direct DMA.
+ * The _mode variants (no counterpart in the reference; for measurement) force a path:
+ * 0 automatic, 1 the HIP runtime's own copy, 2 the staged copy. */
+int rstsr_hip_memcpy_h2d_mode(rstsr_hip_ctx_t* ctx, void* dst, const void* src, size_t nbytes, int mode);
+int rstsr_hip_memcpy_d2h_mode(rstsr_hip_ctx_t* ctx, void* dst, const void* src, size_t nbytes, int mode);
 /* Raw: Clone (device -> device, stream ordered). */
 int rstsr_hip_memcpy_d2d(rstsr_hip_ctx_t* ctx, void* dst, const void* src, size_t nbytes);
 

@@ -64,6 +64,12 @@ def lib():
                                   C.c_size_t, vp, vp, C.c_size_t, vp, C.c_size_t, vp, vp, C.c_size_t],
         "rstsr_hip_driver_syhemm": [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_size_t,
                                     C.c_size_t, vp, vp, C.c_size_t, vp, C.c_size_t, vp, vp, C.c_size_t],
+        "rstsr_hip_memcpy_h2d_mode": [vp, vp, vp, C.c_size_t, C.c_int],
+        "rstsr_hip_memcpy_d2h_mode": [vp, vp, vp, C.c_size_t, C.c_int],
+        "rstsr_hip_memcpy_h2d": [vp, vp, vp, C.c_size_t],
+        "rstsr_hip_memcpy_d2h": [vp, vp, vp, C.c_size_t],
+        "rstsr_hip_malloc": [vp, C.c_size_t, C.POINTER(C.c_void_p)],
+        "rstsr_hip_free": [vp, vp],
         "rstsr_hip_event_record": [vp, C.c_int],
         "rstsr_hip_event_elapsed": [vp, C.c_int, C.c_int, C.POINTER(C.c_float)],
         "rstsr_hip_synchronize": [vp],

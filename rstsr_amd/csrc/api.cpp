@@ -5,9 +5,12 @@
 // Every allocation made here is registered with its extent so that each kernel
 // entry point can prove, on the host, that the layouts it is given stay inside
 // the buffers they index before anything is launched.
+#include <algorithm>
 #include <cstdio>
 #include <cstdlib>
 #include <map>
+#include <thread>
+#include <vector>
 
 #include "common.hpp"
 #include "registry.hpp"
@@ -116,6 +119,86 @@ static void* grow(rstsr_hip_ctx* ctx, Workspace& w, size_t bytes) {
     w.ptr = p;
     w.bytes = nb;
     return p;
+}
+
+/* ---------------- host transfers ---------------- */
+
+constexpr size_t STAGE_CHUNK = 32ull << 20;  // bytes per pinned chunk
+constexpr size_t STAGE_MIN = 8ull << 20;     // smaller pageable copies go through the runtime directly
+
+static bool host_is_pinned(const void* p) {
+    hipPointerAttribute_t at;
+    if (hipPointerGetAttributes(&at, p) != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    return at.type == hipMemoryTypeHost;
+}
+
+// Host memcpy split over a few threads (one thread cannot keep up with the DMA).
+static void par_memcpy(void* d, const void* s, size_t n) {
+    const size_t nt = n >= (8ull << 20) ? 4 : 1;
+    if (nt == 1) {
+        std::memcpy(d, s, n);
+        return;
+    }
+    std::vector<std::thread> th;
+    const size_t per = (n + nt - 1) / nt;
+    for (size_t i = 0; i < nt; ++i) {
+        const size_t o = i * per, len = o < n ? std::min(per, n - o) : 0;
+        if (len) th.emplace_back([=] { std::memcpy((char*)d + o, (const char*)s + o, len); });
+    }
+    for (auto& t : th) t.join();
+}
+
+// Copy between pageable host memory and the device and return when it is done.
+// mode 0 (default): a host pointer the runtime reports as pinned goes by direct
+// DMA; a pageable copy of at least STAGE_MIN bytes runs through two pinned
+// chunks, the host copy of one chunk overlapping the DMA of the other; smaller
+// ones go through the runtime. Mode 1 forces the runtime path, mode 2 the
+// staged path (for measurement).
+static void host_transfer(rstsr_hip_ctx* ctx, void* dst, const void* src, size_t n, bool h2d, int mode = 0) {
+    const void* host = h2d ? src : dst;
+    const bool staged = mode == 2 || (mode == 0 && n >= STAGE_MIN && !host_is_pinned(host));
+    const hipMemcpyKind kind = h2d ? hipMemcpyHostToDevice : hipMemcpyDeviceToHost;
+    if (!staged) {
+        HIP_CHECK(hipMemcpyAsync(dst, src, n, kind, ctx->stream));
+        HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        return;
+    }
+    if (stream_capturing(ctx)) RSTSR_RAISE(InvalidValue, "host transfers cannot be captured into a graph");
+    for (int i = 0; i < 2; ++i) {
+        if (!ctx->stage[i]) HIP_CHECK(hipHostMalloc(&ctx->stage[i], STAGE_CHUNK, hipHostMallocDefault));
+        if (!ctx->stage_ev[i]) HIP_CHECK(hipEventCreateWithFlags(&ctx->stage_ev[i], hipEventDisableTiming));
+    }
+    const size_t nchunk = (n + STAGE_CHUNK - 1) / STAGE_CHUNK;
+    if (h2d) {
+        for (size_t c = 0; c < nchunk; ++c) {
+            const int b = (int)(c & 1);
+            const size_t off = c * STAGE_CHUNK, len = std::min(STAGE_CHUNK, n - off);
+            if (c >= 2) HIP_CHECK(hipEventSynchronize(ctx->stage_ev[b]));  // chunk b's previous DMA done
+            par_memcpy(ctx->stage[b], (const char*)src + off, len);
+            HIP_CHECK(hipMemcpyAsync((char*)dst + off, ctx->stage[b], len, kind, ctx->stream));
+            HIP_CHECK(hipEventRecord(ctx->stage_ev[b], ctx->stream));
+        }
+    } else {
+        // DMA of chunk c+1 runs while chunk c is copied out of its pinned buffer
+        auto issue = [&](size_t c) {
+            const int b = (int)(c & 1);
+            const size_t off = c * STAGE_CHUNK, len = std::min(STAGE_CHUNK, n - off);
+            HIP_CHECK(hipMemcpyAsync(ctx->stage[b], (const char*)src + off, len, kind, ctx->stream));
+            HIP_CHECK(hipEventRecord(ctx->stage_ev[b], ctx->stream));
+        };
+        issue(0);
+        for (size_t c = 0; c < nchunk; ++c) {
+            const int b = (int)(c & 1);
+            const size_t off = c * STAGE_CHUNK, len = std::min(STAGE_CHUNK, n - off);
+            HIP_CHECK(hipEventSynchronize(ctx->stage_ev[b]));
+            if (c + 1 < nchunk) issue(c + 1);
+            par_memcpy((char*)dst + off, ctx->stage[b], len);
+        }
+    }
+    HIP_CHECK(hipStreamSynchronize(ctx->stream));
 }
 
 bool stream_capturing(rstsr_hip_ctx* ctx) {
@@ -251,6 +334,10 @@ int rstsr_hip_ctx_destroy(rstsr_hip_ctx_t* ctx) {
         for (auto& ev : ctx->events)
             if (ev) HIP_CHECK(hipEventDestroy(ev));
         if (ctx->host_pinned) HIP_CHECK(hipHostFree(ctx->host_pinned));
+        for (int i = 0; i < 2; ++i) {
+            if (ctx->stage[i]) HIP_CHECK(hipHostFree(ctx->stage[i]));
+            if (ctx->stage_ev[i]) HIP_CHECK(hipEventDestroy(ctx->stage_ev[i]));
+        }
         HIP_CHECK(hipStreamDestroy(ctx->stream));
         delete ctx;
     });
@@ -317,9 +404,8 @@ int rstsr_hip_memcpy_h2d(rstsr_hip_ctx_t* ctx, void* dst, const void* src, size_
         if (!src) RSTSR_RAISE(InvalidValue, "null host source");
         bind(ctx);
         check_extent_bytes(dst, nbytes, "dst");
-        HIP_CHECK(hipMemcpyAsync(dst, src, nbytes, hipMemcpyHostToDevice, ctx->stream));
-        // pageable host memory: the copy must complete before the caller may reuse src
-        HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        std::lock_guard<std::mutex> g(ctx->mu);
+        host_transfer(ctx, dst, src, nbytes, true);
     });
 }
 
@@ -329,8 +415,32 @@ int rstsr_hip_memcpy_d2h(rstsr_hip_ctx_t* ctx, void* dst, const void* src, size_
         if (!dst) RSTSR_RAISE(InvalidValue, "null host destination");
         bind(ctx);
         check_extent_bytes(src, nbytes, "src");
-        HIP_CHECK(hipMemcpyAsync(dst, src, nbytes, hipMemcpyDeviceToHost, ctx->stream));
-        HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        std::lock_guard<std::mutex> g(ctx->mu);
+        host_transfer(ctx, dst, src, nbytes, false);
+    });
+}
+
+int rstsr_hip_memcpy_h2d_mode(rstsr_hip_ctx_t* ctx, void* dst, const void* src, size_t nbytes, int mode) {
+    return guarded([&] {
+        if (nbytes == 0) return;
+        if (!src) RSTSR_RAISE(InvalidValue, "null host source");
+        if (mode < 0 || mode > 2) RSTSR_RAISE(InvalidValue, "transfer mode must be 0, 1 or 2");
+        bind(ctx);
+        check_extent_bytes(dst, nbytes, "dst");
+        std::lock_guard<std::mutex> g(ctx->mu);
+        host_transfer(ctx, dst, src, nbytes, true, mode);
+    });
+}
+
+int rstsr_hip_memcpy_d2h_mode(rstsr_hip_ctx_t* ctx, void* dst, const void* src, size_t nbytes, int mode) {
+    return guarded([&] {
+        if (nbytes == 0) return;
+        if (!dst) RSTSR_RAISE(InvalidValue, "null host destination");
+        if (mode < 0 || mode > 2) RSTSR_RAISE(InvalidValue, "transfer mode must be 0, 1 or 2");
+        bind(ctx);
+        check_extent_bytes(src, nbytes, "src");
+        std::lock_guard<std::mutex> g(ctx->mu);
+        host_transfer(ctx, dst, src, nbytes, false, mode);
     });
 }
 

@@ -73,6 +73,9 @@ struct rstsr_hip_ctx {
     std::vector<void*> deferred;  // caller buffers freed while the stream was being captured
     void* host_pinned = nullptr;  // 4 KiB pinned staging for scalar results
     hipEvent_t events[16] = {};
+    // two pinned chunks for host transfers of pageable memory (allocated on first use)
+    void* stage[2] = {nullptr, nullptr};
+    hipEvent_t stage_ev[2] = {nullptr, nullptr};
 };
 
 namespace rstsr {
