@@ -126,6 +126,16 @@ class Tensor:
     def __repr__(self):
         return f"Tensor(shape={self.shape}, stride={self.stride}, offset={self.offset}, dtype={self.dtype})"
 
+    def __str__(self):
+        """Display (format/format_tensor.rs:204-226): the storage range the view covers is copied to the
+        host and printed with the reference's layout walk and elision (MIN_PRINT 3, MAX_PRINT 8)."""
+        if self.size == 0:
+            buf, off = np.zeros(0, dtype=self.dtype), 0
+        else:
+            raw, off = self._t._raw_range()
+            buf = np.frombuffer(raw, dtype=self.dtype)
+        return format_tensor(buf, self.shape, self.stride, off)
+
     # ---------------- host transfer ----------------
     def to_numpy(self):
         """Logical contents as a NumPy array (to_vec in logical order)."""
@@ -138,6 +148,10 @@ class Tensor:
             buf[off:], shape=self.shape, strides=[s * it for s in self.stride]
         )
         return np.array(v, copy=True)
+
+    def to_device(self, device):
+        """to_device(&device): a copy of the storage range this view covers on `device`, same layout."""
+        return Tensor(self._t.to_device(device))
 
     def to_vec(self):
         return self.to_numpy().reshape(-1).tolist()
@@ -438,6 +452,110 @@ for _name in UNARY_OPS:
     setattr(Tensor, _name, _unary_method(_name))
     globals()[_name] = _unary_func(_name)
 del _name
+
+
+MIN_PRINT, MAX_PRINT = 3, 8
+
+
+def _fmt_float(x):
+    """Rust's Display of f32/f64: shortest round-trip digits in positional notation."""
+    if np.isnan(x):
+        return "NaN"
+    if np.isinf(x):
+        return "inf" if x > 0 else "-inf"
+    return np.format_float_positional(x, unique=True, trim="-")
+
+
+def _fmt_elem(x):
+    if isinstance(x, (np.bool_, bool)):
+        return "true" if x else "false"
+    if isinstance(x, np.complexfloating):
+        re, im = x.real, x.imag
+        # num-complex Display: re, then '-' |im| when im < 0, else '+' im, then 'i'
+        return _fmt_float(re) + ("-" + _fmt_float(-im) if im < 0 else "+" + _fmt_float(im)) + "i"
+    if isinstance(x, np.floating):
+        return _fmt_float(x)
+    return str(int(x))
+
+
+def format_tensor(vec, shape, stride, offset, max_print=None, min_print=None, fmt_elem=_fmt_elem):
+    """The layout walk of format/format_tensor.rs:25-190 (print_vec_with_layout_dfs_inner) over a
+    host buffer `vec` with the view's (shape, stride, offset) in elements; the recursion there is a
+    tail call in every branch, so it runs here as a loop."""
+    max_print = MAX_PRINT if max_print is None else max_print
+    min_print = MIN_PRINT if min_print is None else min_print
+    ndim = len(shape)
+    out = []
+    if ndim == 0:
+        return fmt_elem(vec[offset])
+    idx, off = [], offset
+    while True:
+        lp = len(idx)
+        if idx and idx[-1] == shape[lp - 1]:
+            # hit the end of an axis
+            if idx[-1] == 0:
+                out.append("[]")
+                break
+            if lp == 1:
+                break
+            p1 = idx.pop()
+            p2 = idx.pop()
+            off = off - p1 * stride[lp - 1] - p2 * stride[lp - 2]
+            if shape[lp - 2] < max_print or p2 + min_print >= shape[lp - 2] or p2 + 1 < min_print:
+                q = p2 + 1
+            else:
+                out.append(" " * (lp - 1) + "...\n\n")
+                q = shape[lp - 2] - min_print
+            idx.append(q)
+            off += q * stride[lp - 2]
+            continue
+        if lp + 1 != ndim:
+            idx.append(0)
+            continue
+        # the innermost axis: one printed line
+        nlast, slast = shape[-1], stride[-1]
+        if nlast == 0:
+            out.append("[]")
+            break
+        if not idx:
+            out.append("[")
+        else:
+            nbra = 1
+            for v in reversed(idx):
+                if v != 0:
+                    break
+                nbra += 1
+            out.append(" " * (ndim - nbra) + "[" * nbra)
+        if nlast <= max(max_print, 2 * min_print + 1):
+            pos = range(nlast)
+            out.extend(" " + fmt_elem(vec[off + i * slast]) for i in pos)
+        else:
+            out.extend(" " + fmt_elem(vec[off + i * slast]) for i in range(min_print))
+            out.append(" ...")
+            out.extend(" " + fmt_elem(vec[off + i * slast]) for i in range(nlast - min_print, nlast))
+        if not idx:
+            out.append("]")
+        else:
+            nket = 1
+            for i in range(len(idx) - 1, -1, -1):
+                if idx[i] != shape[i] - 1:
+                    break
+                nket += 1
+            out.append("]" * nket)
+            if nket != ndim:
+                out.append("\n\n" if nket > 1 else "\n")
+        if lp == 0:
+            break
+        p1 = idx.pop()
+        off -= p1 * stride[lp - 1]
+        if shape[lp - 1] < max_print or p1 + min_print >= shape[lp - 1] or p1 + 1 < min_print:
+            q = p1 + 1
+        else:
+            out.append(" " * lp + "...\n")
+            q = shape[lp - 1] - min_print
+        idx.append(q)
+        off += q * stride[lp - 1]
+    return "".join(out)
 
 
 def asarray(x, device=None, dtype=None):

@@ -65,6 +65,7 @@ PYBIND11_MODULE(_rstsr, m) {
         .def_property_readonly("device", [](const Tensor& t) { return t.device(); })
         .def("_data_ptr", [](const Tensor& t) { return (uintptr_t)t.ptr(); })
         .def("_nbytes", [](const Tensor& t) { return t.storage->nbytes; })
+        .def("to_device", &Tensor::to_device)
         .def("_raw_range", [](const Tensor& t) {
             auto r = t.raw_range();
             return py::make_tuple(py::bytes(r.first), r.second);

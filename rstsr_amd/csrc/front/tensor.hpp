@@ -130,6 +130,21 @@ class Tensor {
 
     Tensor view_with(const Layout& l) const { return Tensor(storage, l, dtype); }
 
+    // DeviceChangeAPI::to_device (storage/conversion.rs:3-21, tensor/device_conversion.rs:36-58):
+    // the storage range the view covers moves to `dev` and the layout is kept (offset rebased).
+    // Work queued on the source stream finishes first; the copy is ordered on the target stream.
+    Tensor to_device(const DeviceHIP& dev) const {
+        if (size() == 0) return uninit(dev, layout, dtype);
+        auto b = layout.bounds_index();
+        const size_t n = (b.second - b.first) * esize();
+        auto st = std::make_shared<Storage>(dev, std::max<size_t>(n, 1));
+        check(rstsr_hip_synchronize(device().raw()));
+        check(rstsr_hip_memcpy_d2d(dev.raw(), st->ptr, (char*)ptr() + b.first * esize(), n));
+        Layout l = layout;
+        l.offset = layout.offset - b.first;
+        return Tensor(st, l, dtype);
+    }
+
     /* ---------- assignment ---------- */
     void assign(const Tensor& a) {
         Layout la = broadcast_layout_to_shape(a.layout, layout.shape, device().default_order());

@@ -54,3 +54,39 @@ def test_tensor_roundtrip_large(dev):
     x = np.random.default_rng(0).standard_normal(25 * MiB)
     t = rt.asarray(x, device=dev)
     assert np.array_equal(t.to_numpy(), x)
+
+
+def test_display_of_device_tensors(dev):
+    """str(tensor) copies the covered storage to the host and prints the reference's layout walk."""
+    import json
+    import os
+
+    ref = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "reference_cases.json")))["display"]
+    a = rt.arange(8, device=dev).reshape([2, 2, 2])
+    assert str(a) == ref["cases"][0]["out"]
+    assert str(a[::-1]) == ref["cases"][1]["out"]          # flip(0)
+    assert str(a[:, ::-1]) == ref["cases"][2]["out"]       # flip(1)
+    assert str(rt.asarray(np.array([-1, 2, -3]), device=dev).unary("abs")) == "[ 1 2 3]"
+    assert str(rt.asarray(np.array([0.0, 1.0, 4.0]), device=dev).unary("sqrt")) == "[ 0 1 2]"
+    x = rt.asarray(np.array([0.0, np.nan, np.inf]), device=dev)
+    assert str(x.unary("is_nan")) == "[ false true false]"
+    big = rt.arange(1800, device=dev).reshape([10, 12, 15]).transpose([2, 0, 1])
+    from rstsr_amd import format_tensor
+    host = np.arange(1800).reshape(10, 12, 15).transpose(2, 0, 1)
+    assert str(big) == format_tensor(np.arange(1800), [15, 10, 12], [1, 180, 15], 0)
+    assert "..." in str(big) and str(big).startswith("[[[ 0 15 30")
+    assert host[0, 0, 1] == 15
+
+
+def test_to_device_keeps_layout(dev):
+    dev2 = rt.DeviceHIP(0)
+    x = np.arange(600.0).reshape(20, 30)
+    t = rt.asarray(x, device=dev)[2:18:3, ::-2]
+    u = t.to_device(dev2)
+    assert u.shape == t.shape and u.stride == t.stride
+    assert np.array_equal(u.to_numpy(), x[2:18:3, ::-2])
+    # the copy is independent of the source
+    t.fill(0.0)
+    assert np.array_equal(u.to_numpy(), x[2:18:3, ::-2])
+    z = rt.empty((0, 4), device=dev).to_device(dev2)
+    assert z.shape == (0, 4) and z.size == 0
