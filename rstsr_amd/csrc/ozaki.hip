@@ -387,17 +387,58 @@ __device__ __forceinline__ void i8_tile(const int8_t* __restrict__ A, const int8
 }
 
 // Batches on blockIdx.y: plane offsets sA / sB (0 for a broadcast operand) and sR.
+// Tiles touching the lower triangle of a square product with BN = R BM: tile row tm
+// holds floor(tm / R) + 1 of them. Index t (row-major over those) -> (tm, tn); the
+// floating root estimate is corrected exactly.
+__host__ __device__ inline uint64_t lower_tile_prefix(uint64_t r, uint64_t R) {
+    const uint64_t q = r / R, m = r - q * R;
+    return R * q * (q + 1) / 2 + m * (q + 1);
+}
+__device__ __forceinline__ void lower_tile(uint32_t t, uint32_t R, uint32_t tiles_m, uint32_t& tm, uint32_t& tn) {
+    if (R == 2) {
+        // groups of four tile rows 4g .. 4g+3 (counts 2g+1, 2g+1, 2g+2, 2g+2), walked down
+        // the columns so that blocks resident together share A and B panels in L2:
+        // columns 0 .. 2g for all four rows, then column 2g+1 for rows 4g+2 and 4g+3.
+        // Group g starts at 4g^2 + 2g and holds 8g + 6 tiles; a last group of two rows
+        // (tiles_m = 2 mod 4) holds 4g + 2 tiles, column by column.
+        uint64_t g = (uint64_t)((sqrt(1.0 + 16.0 * (double)t) - 1.0) * 0.125);
+        while (g > 0 && 4 * g * g + 2 * g > t) --g;
+        while (4 * (g + 1) * (g + 1) + 2 * (g + 1) <= t) ++g;
+        const uint32_t u = (uint32_t)(t - (4 * g * g + 2 * g));
+        const uint32_t r0 = (uint32_t)(4 * g);
+        if (r0 + 2 >= tiles_m) {  // the two-row last group
+            tn = u / 2;
+            tm = r0 + u % 2;
+        } else if (u < 4 * (2 * g + 1)) {
+            tn = u / 4;
+            tm = r0 + u % 4;
+        } else {
+            tn = (uint32_t)(2 * g + 1);
+            tm = r0 + 2 + (u - 4 * (uint32_t)(2 * g + 1));
+        }
+        return;
+    }
+    uint64_t r = (uint64_t)sqrt(2.0 * R * (double)t);
+    while (r > 0 && lower_tile_prefix(r, R) > t) --r;
+    while (lower_tile_prefix(r + 1, R) <= t) ++r;
+    tm = (uint32_t)r;
+    tn = (uint32_t)(t - lower_tile_prefix(r, R));
+}
+
 template <class CF, bool ROW4G>
 __global__ void __launch_bounds__(CF::NT, CF::OCC) k_i8gemm_mod(const int8_t* __restrict__ A,
                                                                 const int8_t* __restrict__ B, uint8_t* __restrict__ R,
                                                                 int64_t M, int64_t N, int64_t ldk, int64_t Kc,
                                                                 int64_t ldr, int mod, int accum, uint32_t tiles_m,
                                                                 uint32_t tiles_n, int64_t sA, int64_t sB, int64_t sR,
-                                                                const int* __restrict__ skip) {
+                                                                const int* __restrict__ skip, int lower) {
     if (*skip) return;
     __shared__ __attribute__((aligned(16))) I8Smem<CF> sm;
     uint32_t tm, tn;
-    tile_coords<4>(blockIdx.x, tiles_m, tiles_n, tm, tn);
+    // symmetric product: the grid enumerates only the tiles touching the lower triangle,
+    // each XCD one contiguous range of them (an equal share of the work)
+    if (lower) lower_tile(xcd_chunk(blockIdx.x, gridDim.x), CF::BN / CF::BM, tiles_m, tm, tn);
+    else tile_coords<4>(blockIdx.x, tiles_m, tiles_n, tm, tn);
     i8_tile<CF, ROW4G>(A + (int64_t)blockIdx.y * sA, B + (int64_t)blockIdx.y * sB, R + (int64_t)blockIdx.y * sR, M, N,
                        ldk, Kc, ldr, mod, accum != 0, tm, tn, sm);
 }
@@ -460,7 +501,7 @@ __global__ void __launch_bounds__(256) k_crt(const uint8_t* __restrict__ R, int6
                                              int64_t Np, int64_t pl, int a_batched, int b_batched, BatchDesc bd,
                                              CrtConst cc, const int* __restrict__ eA, const int* __restrict__ fB,
                                              double* C, int64_t rsc, int64_t csc, double alpha, double beta,
-                                             int beta_zero, const int* __restrict__ skip) {
+                                             int beta_zero, const int* __restrict__ skip, int lower) {
     static_assert(V == 1 || V == 4, "word size");
     if (*skip) return;
     const int64_t j0 = ((int64_t)blockIdx.x * 256 + threadIdx.x) * V;
@@ -474,6 +515,7 @@ __global__ void __launch_bounds__(256) k_crt(const uint8_t* __restrict__ R, int6
         const uint8_t* rp = R + ((int64_t)b * Mp + i) * N + j0;
         double* cp = C + oc + i * rsc + j0 * csc;
         const int ei = eAb[i];
+        if (lower && j0 > i) continue;  // symmetric product: the mirror fills j > i
         if constexpr (V == 1) {
             const double v = ldexp(crt_value(rp, pl, cc), -(ei + fBb[j0]));
             *cp = beta_zero ? alpha * v : alpha * v + beta * (*cp);
@@ -486,6 +528,7 @@ __global__ void __launch_bounds__(256) k_crt(const uint8_t* __restrict__ R, int6
                 uint32_t r[OZ_MAXMOD];
 #pragma unroll
                 for (int l = 0; l < OZ_MAXMOD; ++l) r[l] = (w[l] >> (8 * k)) & 255u;
+                if (lower && j0 + k > i) break;
                 const double v = ldexp(crt_from(r, cc), -(ei + fBb[j0 + k]));
                 double* ck = cp + k * csc;
                 *ck = beta_zero ? alpha * v : alpha * v + beta * (*ck);
@@ -758,7 +801,14 @@ void ozaki_gemm_f64(rstsr_hip_ctx* ctx, const GemmProblem& P, double alpha, doub
     if (K > (int64_t)1 << 40) RSTSR_RAISE(UnImplemented, "K too large for the split GEMM");
     const I8Tile tile = pick_i8_tile(N);
     const int64_t bn = (tile == I8Tile::Wide || tile == I8Tile::Wide8) ? 256 : 128;
-    const int64_t Mp = (M + I8_MPAD - 1) / I8_MPAD * I8_MPAD, Np = (N + bn - 1) / bn * bn;
+    // Symmetric product (the SYRK condition of device_faer/matmul.rs:38-48: beta = 0, A is B's
+    // transpose in the same buffer): B's columns are A's rows, so B's scales and residue planes are
+    // A's; only the INT8 tiles touching the lower triangle run, the CRT fills i >= j, and the mirror
+    // copies the rest. Residue sums are exact integers, so the result is the full product bit for bit.
+    bool sym = beta_zero && P.a == P.b && M == N && P.rsa == P.csb && P.csa == P.rsb;
+    for (int d = 0; d < P.bd.nbd && sym; ++d) sym = P.bd.sa[d] == P.bd.sb[d];
+    const int64_t Np = (N + bn - 1) / bn * bn;
+    const int64_t Mp = sym ? Np : (M + I8_MPAD - 1) / I8_MPAD * I8_MPAD;  // one padded extent when shared
     const int64_t Kp = (K + I8_KPAD - 1) / I8_KPAD * I8_KPAD;
     const uint64_t nb = P.nbatch;
     if (nb >= 65536) RSTSR_RAISE(UnImplemented, "too many batches for the split GEMM");
@@ -816,17 +866,19 @@ void ozaki_gemm_f64(rstsr_hip_ctx* ctx, const GemmProblem& P, double alpha, doub
     const size_t o_part = take(std::max(part_bytes(qa), part_bytes(qb)));
     const size_t planeA = (size_t)nbA * Mp * Kp, planeB = (size_t)nbB * Np * Kp, planeR = (size_t)nb * Mp * N;
     const size_t o_A8 = take((size_t)n * planeA);
-    const size_t o_B8 = take((size_t)n * planeB);
+    const size_t o_B8 = sym ? o_A8 : take((size_t)n * planeB);
     const size_t o_R = take((size_t)n * planeR);
     char* ws = (char*)workspace(ctx, off);
     int* flag = (int*)(ws + o_flag);
     int* eA = (int*)(ws + o_eA);
-    int* fB = (int*)(ws + o_fB);
+    int* fB = sym ? eA : (int*)(ws + o_fB);
     NormAcc* part = (NormAcc*)(ws + o_part);
     int8_t* A8 = (int8_t*)(ws + o_A8);
     int8_t* B8 = (int8_t*)(ws + o_B8);
     uint8_t* R = (uint8_t*)(ws + o_R);
-    const uint32_t tiles_n = (uint32_t)(Np / bn), ntiles = (uint32_t)((Mp / I8_MPAD) * tiles_n);
+    const uint32_t tiles_n = (uint32_t)(Np / bn);
+    const uint32_t ntiles = sym ? (uint32_t)lower_tile_prefix((uint64_t)(Mp / I8_MPAD), (uint64_t)(bn / I8_MPAD))
+                                : (uint32_t)((Mp / I8_MPAD) * tiles_n);
 
     hipLaunchKernelGGL(k_zero_i32, dim3(ceil_div_u32(n_zero, 256)), dim3(256), 0, ctx->stream, flag, (int64_t)n_zero);
     check_launch("ozaki zero");
@@ -849,11 +901,13 @@ void ozaki_gemm_f64(rstsr_hip_ctx* ctx, const GemmProblem& P, double alpha, doub
                        ctx->stream, (const double*)P.a, M, K, P.rsa, P.csa, P.bd, 0, eA, Mp, Kp, A8, ms,
                        std::llabs(P.csa) == 1 ? 1 : 0, flag, (int64_t)planeA);
     check_launch("ozaki split A");
-    run_norm(qb, fB);  // reuses the partials buffer after the A norms (stream order)
-    hipLaunchKernelGGL(k_split, dim3((uint32_t)(Kp / 64), (uint32_t)(Np / 64), (uint32_t)nbB), dim3(256), 0,
-                       ctx->stream, (const double*)P.b, N, K, P.csb, P.rsb, P.bd, 1, fB, Np, Kp, B8, ms,
-                       std::llabs(P.rsb) == 1 ? 1 : 0, flag, (int64_t)planeB);
-    check_launch("ozaki split B");
+    if (!sym) {
+        run_norm(qb, fB);  // reuses the partials buffer after the A norms (stream order)
+        hipLaunchKernelGGL(k_split, dim3((uint32_t)(Kp / 64), (uint32_t)(Np / 64), (uint32_t)nbB), dim3(256), 0,
+                           ctx->stream, (const double*)P.b, N, K, P.csb, P.rsb, P.bd, 1, fB, Np, Kp, B8, ms,
+                           std::llabs(P.rsb) == 1 ? 1 : 0, flag, (int64_t)planeB);
+        check_launch("ozaki split B");
+    }
     // Non-finite inputs set *flag: the split kernels exit at once and the plain
     // FP64 kernel (which runs only when *flag != 0) computes C with IEEE special
     // values. No host round trip, so the sequence can be graph-captured.
@@ -868,7 +922,7 @@ void ozaki_gemm_f64(rstsr_hip_ctx* ctx, const GemmProblem& P, double alpha, doub
                                    ctx->stream, A8 + (size_t)l * planeA + k0, B8 + (size_t)l * planeB + k0,
                                    R + (size_t)l * planeR, Mp, N, Kp, Kc, N, (int)cc.mod[l], k0 > 0 ? 1 : 0,
                                    (uint32_t)(Mp / I8_MPAD), tiles_n, a_b ? Mp * Kp : (int64_t)0,
-                                   b_b ? Np * Kp : (int64_t)0, Mp * N, flag);
+                                   b_b ? Np * Kp : (int64_t)0, Mp * N, flag, sym ? 1 : 0);
             };
             switch (tile) {
                 case I8Tile::Wide: go(I8Wide{}); break;
@@ -882,13 +936,16 @@ void ozaki_gemm_f64(rstsr_hip_ctx* ctx, const GemmProblem& P, double alpha, doub
     // four columns per thread when every row and plane of R starts on a 4-byte boundary
     auto crt = [&](auto v) {
         constexpr int V = decltype(v)::value;
-        hipLaunchKernelGGL((k_crt<V>), dim3(ceil_div_u32(N, 256 * V), (uint32_t)std::min<int64_t>(M, 65535), (uint32_t)nb),
+        // four rows per block (rows i, i + gridDim.y, ...): fewer, longer blocks
+        hipLaunchKernelGGL((k_crt<V>), dim3(ceil_div_u32(N, 256 * V), (uint32_t)std::min<int64_t>((M + 3) / 4, 65535),
+                                            (uint32_t)nb),
                            dim3(256), 0, ctx->stream, R, M, N, Mp, Np, (int64_t)planeR, a_b ? 1 : 0, b_b ? 1 : 0, P.bd,
-                           cc, eA, fB, (double*)P.c, P.rsc, P.csc, alpha, beta, beta_zero ? 1 : 0, flag);
+                           cc, eA, fB, (double*)P.c, P.rsc, P.csc, alpha, beta, beta_zero ? 1 : 0, flag, sym ? 1 : 0);
     };
     if (N % 4 == 0) crt(std::integral_constant<int, 4>{});
     else crt(std::integral_constant<int, 1>{});
     check_launch("ozaki crt");
+    if (sym) mirror_lower_f64(ctx, P);
 }
 
 void ozaki_grouped_f64(rstsr_hip_ctx* ctx, const std::vector<GemmProblem>& probs, double alpha, double beta,

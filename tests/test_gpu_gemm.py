@@ -463,3 +463,38 @@ def test_free_during_capture_is_deferred(dev):
     g.launch()
     dev.synchronize()
     assert np.array_equal(tmp.to_numpy(), np.arange(64.0).reshape(8, 8) ** 2)
+
+
+@pytest.mark.parametrize("m,k", [(1, 5), (100, 64), (300, 257), (1024, 512)])
+def test_ozaki_syrk_bit_identical(dev, oracle, m, k):
+    """The split GEMM's symmetric path (X^T X in one buffer, beta = 0: shared scales and residue
+    planes, lower-triangle INT8 tiles, CRT on i >= j, mirror) equals the split GEMM with B in a
+    separate buffer bit for bit (residue sums are exact integers), and stays within 1e-14 normwise."""
+    rng = np.random.default_rng(m * 3 + k)
+    X = wide_exponent(rng, (k, m))
+    tX = rt.asarray(X, device=dev)
+    tY = rt.asarray(X.copy(), device=dev)
+    sym = rt.matmul(tX.T, tX, mode="ozaki").to_numpy()
+    full = rt.matmul(tX.T, tY, mode="ozaki").to_numpy()
+    assert np.array_equal(sym.view(np.uint8), full.view(np.uint8))
+    assert np.array_equal(sym, sym.T)
+    ref = oracle.gemm_dot2(np.ascontiguousarray(X.T), np.ascontiguousarray(X))
+    assert np.max(np.abs(sym - ref)) <= 1e-14 * np.max(np.abs(ref))
+
+
+def test_ozaki_syrk_batched_and_nonfinite(dev):
+    rng = np.random.default_rng(9)
+    X = rng.normal(size=(3, 70, 130))
+    tX = rt.asarray(X, device=dev)
+    got = rt.matmul(tX.swapaxes(-1, -2), tX, mode="ozaki").to_numpy()
+    ref = np.einsum("bki,bkj->bij", X, X)
+    assert np.max(np.abs(got - ref)) <= 1e-13 * np.max(np.abs(ref))
+    assert np.array_equal(got, got.swapaxes(-1, -2))
+    # a NaN input takes the plain fallback inside the same launch sequence
+    X[1, 5, 7] = np.nan
+    tX = rt.asarray(X, device=dev)
+    got = rt.matmul(tX.swapaxes(-1, -2), tX, mode="ozaki").to_numpy()
+    ref = np.einsum("bki,bkj->bij", X, X)
+    assert np.array_equal(np.isnan(got), np.isnan(ref))
+    fin = np.isfinite(ref)
+    assert np.max(np.abs(got[fin] - ref[fin])) <= 1e-12 * np.max(np.abs(ref[fin]))
