@@ -119,8 +119,8 @@ typedef enum {
     RSTSR_UN_NOT = 36
 } rstsr_unop_t;
 
-/* GEMM arithmetic: plain FP64 MFMA, or Ozaki-II split over INT8 MFMA with CRT (f64; c128 as
- * four real products with real alpha/beta). */
+/* GEMM arithmetic: plain FP64 MFMA, or Ozaki-II split over INT8 MFMA with CRT (f64; c128 with
+ * real alpha/beta as three real products when beta = 0 and C is one matrix, else four). */
 typedef enum { RSTSR_GEMM_PLAIN = 0, RSTSR_GEMM_OZAKI_INT8 = 1 } rstsr_gemm_mode_t;
 
 /* Layout<IxD>: rstsr-common/src/layout/layoutbase.rs:15-23 (shape: usize, stride: isize, offset: usize). */

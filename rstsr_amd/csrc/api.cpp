@@ -273,6 +273,7 @@ int tune_select(rstsr_hip_ctx* ctx, const std::string& key, int ncand, int heuri
 
 void* workspace(rstsr_hip_ctx* ctx, size_t bytes) { return grow(ctx, ctx->ws, bytes); }
 void* aux_buffer(rstsr_hip_ctx* ctx, size_t bytes) { return grow(ctx, ctx->aux, bytes); }
+void* zbuf_buffer(rstsr_hip_ctx* ctx, size_t bytes) { return grow(ctx, ctx->zbuf, bytes); }
 
 }  // namespace hip
 }  // namespace rstsr
@@ -329,6 +330,7 @@ int rstsr_hip_ctx_destroy(rstsr_hip_ctx_t* ctx) {
         HIP_CHECK(hipStreamSynchronize(ctx->stream));
         if (ctx->ws.ptr) HIP_CHECK(hipFree(ctx->ws.ptr));
         if (ctx->aux.ptr) HIP_CHECK(hipFree(ctx->aux.ptr));
+        if (ctx->zbuf.ptr) HIP_CHECK(hipFree(ctx->zbuf.ptr));
         for (void* p : ctx->retired) HIP_CHECK(hipFree(p));
         for (void* p : ctx->deferred) HIP_CHECK(hipFree(p));
         for (auto& ev : ctx->events)

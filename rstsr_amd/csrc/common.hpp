@@ -68,7 +68,8 @@ struct rstsr_hip_ctx {
     size_t l2_bytes = 0;
     std::mutex mu;  // serialises workspace growth and host staging on one context
     rstsr::hip::Workspace ws;   // partials, residue planes, scalar results
-    rstsr::hip::Workspace aux;  // small metadata (grouped GEMM descriptors)
+    rstsr::hip::Workspace aux;  // small metadata (grouped GEMM descriptors), BLAS driver scratch
+    rstsr::hip::Workspace zbuf;  // temporaries of the three-product complex split GEMM
     std::vector<void*> retired;  // buffers replaced while a stream was being captured
     std::vector<void*> deferred;  // caller buffers freed while the stream was being captured
     void* host_pinned = nullptr;  // 4 KiB pinned staging for scalar results
@@ -93,6 +94,7 @@ inline void bind(rstsr_hip_ctx* ctx) {
 // captured graph keeps valid addresses.
 void* workspace(rstsr_hip_ctx* ctx, size_t bytes);
 void* aux_buffer(rstsr_hip_ctx* ctx, size_t bytes);
+void* zbuf_buffer(rstsr_hip_ctx* ctx, size_t bytes);
 
 // True while the context stream is being captured into a graph.
 bool stream_capturing(rstsr_hip_ctx* ctx);

@@ -766,11 +766,92 @@ static bool is_zero_scalar(rstsr_dtype_t t, const void* v) {
     }
 }
 
-// Complex split GEMM from four real split products on the real/imaginary
-// planes (stride-2 double views): Cr = a ArBr - a AiBi + b Cr, Ci = a ArBi + a AiBr + b Ci.
+// S (rows x cols, dense row-major) = Re X + Im X for a strided complex matrix X
+__global__ void __launch_bounds__(256) k_zsum_parts(double* __restrict__ S, const c128* __restrict__ X, int64_t rows,
+                                                    int64_t cols, int64_t rs, int64_t cs) {
+    const int64_t n = rows * cols;
+    for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < n; e += (int64_t)gridDim.x * 256) {
+        const int64_t i = e / cols, j = e - i * cols;
+        const c128 x = X[i * rs + j * cs];
+        S[e] = x.re + x.im;
+    }
+}
+
+// C holds (a T1, a T3) with T1 = Ar Br, T3 = (Ar + Ai)(Br + Bi); T2 = a Ai Bi (dense M x N).
+// C = (a T1 - a T2, a T3 - a T1 - a T2).
+__global__ void __launch_bounds__(256) k_z3m_combine(c128* __restrict__ C, int64_t rsc, int64_t csc,
+                                                     const double* __restrict__ T2, int64_t M, int64_t N) {
+    const int64_t n = M * N;
+    for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < n; e += (int64_t)gridDim.x * 256) {
+        const int64_t i = e / N, j = e - i * N;
+        c128* p = C + i * rsc + j * csc;
+        const c128 c = *p;
+        const double t2 = T2[e];
+        *p = c128{c.re - t2, (c.im - c.re) - t2};
+    }
+}
+
+// Complex split GEMM from real split products on the real/imaginary planes
+// (stride-2 double views). With beta = 0 and one matrix: three products (below);
+// otherwise four: Cr = a ArBr - a AiBi + b Cr, Ci = a ArBi + a AiBr + b Ci.
 static void ozaki_zgemm(rstsr_hip_ctx* ctx, const GemmProblem& P, c128 alpha, c128 beta, bool beta_zero) {
     if (alpha.im != 0.0 || beta.im != 0.0)
         RSTSR_RAISE(UnImplemented, "the complex split GEMM takes real alpha and beta");
+    if (beta_zero && P.nbatch == 1 && P.bd.nbd == 0 && P.K > 0) {
+        // Three real split products (Karatsuba form): T1 = Ar Br, T2 = Ai Bi,
+        // T3 = (Ar + Ai)(Br + Bi); Re C = a (T1 - T2), Im C = a (T3 - T1 - T2).
+        // Each product is accurate to the split GEMM's normwise bound, so the
+        // result keeps it relative to (|Ar| + |Ai|)(|Br| + |Bi|).
+        const int64_t M = P.M, N = P.N, K = P.K;
+        const size_t nSA = (size_t)M * K, nSB = (size_t)K * N, nT = (size_t)M * N;
+        auto al = [](size_t x) { return (x * sizeof(double) + 255) & ~(size_t)255; };
+        char* z = (char*)zbuf_buffer(ctx, al(nSA) + al(nSB) + al(nT));
+        double* SA = (double*)z;
+        double* SB = (double*)(z + al(nSA));
+        double* T2 = (double*)(z + al(nSA) + al(nSB));
+        auto grid = [](size_t n) { return dim3((uint32_t)std::min<size_t>((n + 255) / 256, 65536)); };
+        hipLaunchKernelGGL(k_zsum_parts, grid(nSA), dim3(256), 0, ctx->stream, SA, (const c128*)P.a, M, K, P.rsa, P.csa);
+        hipLaunchKernelGGL(k_zsum_parts, grid(nSB), dim3(256), 0, ctx->stream, SB, (const c128*)P.b, K, N, P.rsb, P.csb);
+        check_launch("complex split sums");
+        const double a = alpha.re;
+        GemmProblem Q = P;
+        Q.bd.nbd = 0;
+        Q.nbatch = 1;
+        // T1 into Re C
+        Q.a = P.a;
+        Q.rsa = 2 * P.rsa;
+        Q.csa = 2 * P.csa;
+        Q.b = P.b;
+        Q.rsb = 2 * P.rsb;
+        Q.csb = 2 * P.csb;
+        Q.c = P.c;
+        Q.rsc = 2 * P.rsc;
+        Q.csc = 2 * P.csc;
+        ozaki_gemm_f64(ctx, Q, a, 0.0, true);
+        // T3 into Im C
+        Q.a = SA;
+        Q.rsa = K;
+        Q.csa = 1;
+        Q.b = SB;
+        Q.rsb = N;
+        Q.csb = 1;
+        Q.c = (double*)P.c + 1;
+        ozaki_gemm_f64(ctx, Q, a, 0.0, true);
+        // T2 into the temporary
+        Q.a = (const double*)P.a + 1;
+        Q.rsa = 2 * P.rsa;
+        Q.csa = 2 * P.csa;
+        Q.b = (const double*)P.b + 1;
+        Q.rsb = 2 * P.rsb;
+        Q.csb = 2 * P.csb;
+        Q.c = T2;
+        Q.rsc = N;
+        Q.csc = 1;
+        ozaki_gemm_f64(ctx, Q, a, 0.0, true);
+        hipLaunchKernelGGL(k_z3m_combine, grid(nT), dim3(256), 0, ctx->stream, (c128*)P.c, P.rsc, P.csc, T2, M, N);
+        check_launch("complex split combine");
+        return;
+    }
     auto part = [&](const void* base, int im) { return (const void*)((const double*)base + im); };
     GemmProblem R = P;
     R.rsa *= 2;

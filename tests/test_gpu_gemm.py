@@ -498,3 +498,43 @@ def test_ozaki_syrk_batched_and_nonfinite(dev):
     assert np.array_equal(np.isnan(got), np.isnan(ref))
     fin = np.isfinite(ref)
     assert np.max(np.abs(got[fin] - ref[fin])) <= 1e-12 * np.max(np.abs(ref[fin]))
+
+
+def test_zgemm_ozaki_three_products_strided(dev):
+    """beta = 0 complex split GEMM (three real products) on transposed and negatively strided
+    operands; normwise error against an extended-precision product."""
+    rng = np.random.default_rng(21)
+    m, k, n = 150, 333, 97
+    A0 = wide_exponent(rng, (k, m)) + 1j * wide_exponent(rng, (k, m))
+    B0 = wide_exponent(rng, (n, k)) + 1j * wide_exponent(rng, (n, k))
+    A = A0.T                 # column-major view
+    B = B0[::-1].T           # negative column stride
+    ta, _ = from_np(dev, A)
+    tb, _ = from_np(dev, B)
+    got = rt.matmul(ta, tb, mode="ozaki").to_numpy()
+    ref = A.astype(np.clongdouble) @ B.astype(np.clongdouble)
+    err = float(np.max(np.abs(got - ref)) / np.max(np.abs(ref)))
+    assert err <= 1e-14, err
+
+
+def test_zgemm_ozaki_four_products_with_beta(dev):
+    """beta != 0 keeps the four-product form: C = a A B + b C through the C ABI."""
+    from rstsr_amd import abi
+
+    L = abi.lib()
+    ctx = C.c_void_p(dev._ctx_ptr())
+    rng = np.random.default_rng(22)
+    m, k, n = 70, 90, 60
+    A = rng.normal(size=(m, k)) + 1j * rng.normal(size=(m, k))
+    B = rng.normal(size=(k, n)) + 1j * rng.normal(size=(k, n))
+    C0 = rng.normal(size=(m, n)) + 1j * rng.normal(size=(m, n))
+    ta, tb, tc = rt.asarray(A, device=dev), rt.asarray(B, device=dev), rt.asarray(C0, device=dev)
+    al = np.array([1.5 + 0j])
+    be = np.array([-0.5 + 0j])
+    p = lambda t: C.c_void_p(t._t._data_ptr())  # noqa: E731
+    r = L.rstsr_hip_gemm(ctx, 5, 1, m, n, k, al.ctypes.data_as(C.c_void_p), p(ta), k, 1, p(tb), n, 1,
+                         be.ctypes.data_as(C.c_void_p), p(tc), n, 1)
+    assert r == 0, L.rstsr_hip_last_error()
+    ref = 1.5 * (A.astype(np.clongdouble) @ B.astype(np.clongdouble)) - 0.5 * C0
+    err = float(np.max(np.abs(tc.to_numpy() - ref)) / np.max(np.abs(ref)))
+    assert err <= 1e-14, err
