@@ -387,6 +387,11 @@ __device__ __forceinline__ void i8_tile(const int8_t* __restrict__ A, const int8
 }
 
 // Batches on blockIdx.y: plane offsets sA / sB (0 for a broadcast operand) and sR.
+// Moduli of one launch (blockIdx.z selects one)
+struct ModList {
+    int m[OZ_MAXMOD];
+};
+
 // Tiles touching the lower triangle of a square product with BN = R BM: tile row tm
 // holds floor(tm / R) + 1 of them. Index t (row-major over those) -> (tm, tn); the
 // floating root estimate is corrected exactly.
@@ -429,10 +434,17 @@ template <class CF, bool ROW4G>
 __global__ void __launch_bounds__(CF::NT, CF::OCC) k_i8gemm_mod(const int8_t* __restrict__ A,
                                                                 const int8_t* __restrict__ B, uint8_t* __restrict__ R,
                                                                 int64_t M, int64_t N, int64_t ldk, int64_t Kc,
-                                                                int64_t ldr, int mod, int accum, uint32_t tiles_m,
+                                                                int64_t ldr, ModList mods, int accum, uint32_t tiles_m,
                                                                 uint32_t tiles_n, int64_t sA, int64_t sB, int64_t sR,
+                                                                int64_t pA, int64_t pB, int64_t pR,
                                                                 const int* __restrict__ skip, int lower) {
     if (*skip) return;
+    // modulus l = blockIdx.z: every modulus's product in one launch (no tail between them)
+    const int l = (int)blockIdx.z;
+    A += l * pA;
+    B += l * pB;
+    R += l * pR;
+    const int mod = mods.m[l];
     __shared__ __attribute__((aligned(16))) I8Smem<CF> sm;
     uint32_t tm, tn;
     // symmetric product: the grid enumerates only the tiles touching the lower triangle,
@@ -912,26 +924,26 @@ void ozaki_gemm_f64(rstsr_hip_ctx* ctx, const GemmProblem& P, double alpha, doub
     // FP64 kernel (which runs only when *flag != 0) computes C with IEEE special
     // values. No host round trip, so the sequence can be graph-captured.
     plain_dgemm(ctx, P, alpha, beta, beta_zero, flag);
-    for (int l = 0; l < n; ++l) {
-        // K beyond I8_KCHUNK: further launches add their residues into R
-        for (int64_t k0 = 0; k0 < Kp; k0 += I8_KCHUNK) {
-            const int64_t Kc = std::min<int64_t>(I8_KCHUNK, Kp - k0);
-            auto go = [&](auto cfg) {
-                using CF = decltype(cfg);
-                hipLaunchKernelGGL((k_i8gemm_mod<CF, I8_ROW4G>), dim3(ntiles, (uint32_t)nb), dim3(CF::NT), 0,
-                                   ctx->stream, A8 + (size_t)l * planeA + k0, B8 + (size_t)l * planeB + k0,
-                                   R + (size_t)l * planeR, Mp, N, Kp, Kc, N, (int)cc.mod[l], k0 > 0 ? 1 : 0,
-                                   (uint32_t)(Mp / I8_MPAD), tiles_n, a_b ? Mp * Kp : (int64_t)0,
-                                   b_b ? Np * Kp : (int64_t)0, Mp * N, flag, sym ? 1 : 0);
-            };
-            switch (tile) {
-                case I8Tile::Wide: go(I8Wide{}); break;
-                case I8Tile::Wide8: go(I8Wide8{}); break;
-                case I8Tile::Sq32: go(I8Sq32{}); break;
-                default: go(I8Sq{}); break;
-            }
-            check_launch("ozaki int8 gemm");
+    ModList mods{};
+    for (int l = 0; l < n; ++l) mods.m[l] = (int)cc.mod[l];
+    // K beyond I8_KCHUNK: further launches add their residues into R
+    for (int64_t k0 = 0; k0 < Kp; k0 += I8_KCHUNK) {
+        const int64_t Kc = std::min<int64_t>(I8_KCHUNK, Kp - k0);
+        auto go = [&](auto cfg) {
+            using CF = decltype(cfg);
+            hipLaunchKernelGGL((k_i8gemm_mod<CF, I8_ROW4G>), dim3(ntiles, (uint32_t)nb, (uint32_t)n), dim3(CF::NT), 0,
+                               ctx->stream, A8 + k0, B8 + k0, R, Mp, N, Kp, Kc, N, mods, k0 > 0 ? 1 : 0,
+                               (uint32_t)(Mp / I8_MPAD), tiles_n, a_b ? Mp * Kp : (int64_t)0,
+                               b_b ? Np * Kp : (int64_t)0, Mp * N, (int64_t)planeA, (int64_t)planeB,
+                               (int64_t)planeR, flag, sym ? 1 : 0);
+        };
+        switch (tile) {
+            case I8Tile::Wide: go(I8Wide{}); break;
+            case I8Tile::Wide8: go(I8Wide8{}); break;
+            case I8Tile::Sq32: go(I8Sq32{}); break;
+            default: go(I8Sq{}); break;
         }
+        check_launch("ozaki int8 gemm");
     }
     // four columns per thread when every row and plane of R starts on a 4-byte boundary
     auto crt = [&](auto v) {
