@@ -746,7 +746,7 @@ static RedSetup reduce_plan(rstsr_hip_ctx* ctx, const Layout& la, const std::vec
         if (p.rshape[0] >= 0xffffffffull) RSTSR_RAISE(UnImplemented, "reduced extent too large");
         // about 8 waves per CU (measured: more splits lose DRAM locality along the
         // reduced walk, fewer lose latency hiding); a thread covers VEC outputs
-        const uint64_t vec = (esize == 4 || esize == 8) ? 16 / esize : 1;
+        const uint64_t vec = (esize == 4 || esize == 8) ? 2 : 1;
         uint64_t want = std::max<uint64_t>(1, ((uint64_t)std::max(ctx->num_cu, 1) * 8 * 64 * vec) / M);
         uint64_t nsplit = std::min<uint64_t>(want, std::max<uint64_t>(1, R / 64));
         nsplit = std::min<uint64_t>(nsplit, 65536);
@@ -780,13 +780,16 @@ static void run_reduce(rstsr_hip_ctx* ctx, const RedSetup& s, const T* a, typena
         else
             hipLaunchKernelGGL((k_red_row<A, T, 1>), dim3(nb), dim3(RB), 0, ctx->stream, a, out, part, p);
     } else {
-        bool vec = V16 > 1 && p.nk >= 1 && p.kin[0] == 1 && p.kshape[0] % V16 == 0 && p.M % V16 == 0 &&
-                   (p.in_off % V16) == 0 && ((uintptr_t)a % 16) == 0;
-        for (int d = 1; d < p.nk && vec; ++d) vec = (p.kin[d] % V16) == 0;
-        for (int d = 0; d < p.nr && vec; ++d) vec = (p.rin[d] % V16) == 0;
+        // column kernel: two outputs per thread for 4-byte elements too (8-byte loads), so a
+        // split covers the same number of lanes as for 8-byte elements
+        constexpr int VC = V16 > 2 ? 2 : V16;
+        bool vec = VC > 1 && p.nk >= 1 && p.kin[0] == 1 && p.kshape[0] % VC == 0 && p.M % VC == 0 &&
+                   (p.in_off % VC) == 0 && ((uintptr_t)a % (VC * sizeof(T))) == 0;
+        for (int d = 1; d < p.nk && vec; ++d) vec = (p.kin[d] % VC) == 0;
+        for (int d = 0; d < p.nr && vec; ++d) vec = (p.rin[d] % VC) == 0;
         if (vec) {
-            uint64_t nthr = (uint64_t)(p.M / V16) * p.nsplit;
-            hipLaunchKernelGGL((k_red_col<A, T, V16>), dim3(ceil_div_u32(nthr, RB)), dim3(RB), 0, ctx->stream, a, out,
+            uint64_t nthr = (uint64_t)(p.M / VC) * p.nsplit;
+            hipLaunchKernelGGL((k_red_col<A, T, VC>), dim3(ceil_div_u32(nthr, RB)), dim3(RB), 0, ctx->stream, a, out,
                                part, p);
         } else {
             uint64_t nthr = (uint64_t)p.M * p.nsplit;
