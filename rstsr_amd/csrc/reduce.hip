@@ -365,7 +365,16 @@ __device__ __forceinline__ void row_offsets(const RedPlan& p, uint64_t u, int64_
 }
 
 constexpr int RB = 256;  // threads per block (4 waves)
-constexpr int COLU = 4;  // reduced positions loaded per iteration by the column kernel
+
+// Reduced positions loaded per iteration by the column kernel, from the bytes one lane
+// loads per position: 16 (i32/f32 pairs) or 8 (f64/i64 pairs, c128) positions, so that a
+// wave keeps about 8 KiB in flight, and 4 for narrower loads. The deeper forms run with
+// half as many waves (col_waves_per_cu): fewer, longer walks keep DRAM pages open along
+// the reduced axis (measured on the strided column reductions: i32 argmax 51 -> 41 us).
+__host__ __device__ constexpr int col_unroll(int lane_bytes) {
+    return lane_bytes == 8 ? 16 : (lane_bytes >= 16 ? 8 : 4);
+}
+__host__ __device__ constexpr int col_waves_per_cu(int lane_bytes) { return col_unroll(lane_bytes) > 4 ? 4 : 8; }
 
 // VEC consecutive elements loaded as one aligned access
 template <class T, int VEC>
@@ -480,6 +489,7 @@ __global__ void __launch_bounds__(RB) k_red_col(const T* __restrict__ a, typenam
                                                 typename A::S* part, RedPlan p) {
     using S = typename A::S;
     using vt = Pack<T, VEC>;
+    constexpr int COLU = col_unroll((int)(sizeof(T) * VEC));
     const uint64_t t = (uint64_t)blockIdx.x * RB + threadIdx.x;
     const uint32_t MV = p.M / VEC;
     if (t >= (uint64_t)MV * p.nsplit) return;
@@ -626,9 +636,18 @@ struct RedSetup {
 };
 
 // la: input layout; axes: reduced axes (given order); lo: output layout (kept shape)
-// esize: element bytes, which sets the column kernel's vector width (16-byte loads)
+// Outputs per thread of the column kernel: two for the real 4- and 8-byte dtypes (one 8- or
+// 16-byte load per position), one otherwise. Matches run_reduce's VC when the host proves
+// alignment; the unaligned fallback keeps the same split count.
+static int col_vec(rstsr_dtype_t t) {
+    return (t == RSTSR_F32 || t == RSTSR_F64 || t == RSTSR_I32 || t == RSTSR_I64 || t == RSTSR_U64) ? 2 : 1;
+}
+
+// la: input layout; axes: reduced axes (given order); lo: output layout (kept shape);
+// t: element dtype, which sets the column kernel's vector width, unroll and split count
 static RedSetup reduce_plan(rstsr_hip_ctx* ctx, const Layout& la, const std::vector<ptrdiff_t>& axes, const Layout& lo,
-                            size_t esize) {
+                            rstsr_dtype_t t) {
+    const size_t esize = dtype_size(t);
     RedSetup s;
     auto sp = la.dim_split_axes(axes);
     const Layout& ls = sp.first;
@@ -744,10 +763,12 @@ static RedSetup reduce_plan(rstsr_hip_ctx* ctx, const Layout& la, const std::vec
         s.kind = RedKind::Col;
         // argmin/argmax track positions along reduced axis 0 in 32 bits
         if (p.rshape[0] >= 0xffffffffull) RSTSR_RAISE(UnImplemented, "reduced extent too large");
-        // about 8 waves per CU (measured: more splits lose DRAM locality along the
-        // reduced walk, fewer lose latency hiding); a thread covers VEC outputs
-        const uint64_t vec = (esize == 4 || esize == 8) ? 2 : 1;
-        uint64_t want = std::max<uint64_t>(1, ((uint64_t)std::max(ctx->num_cu, 1) * 8 * 64 * vec) / M);
+        // 4 or 8 waves per CU by the unroll depth (measured: more splits lose DRAM
+        // locality along the reduced walk, fewer lose latency hiding); a thread covers
+        // vec outputs
+        const uint64_t vec = (uint64_t)col_vec(t);
+        const uint64_t wpc = (uint64_t)col_waves_per_cu((int)(esize * vec));
+        uint64_t want = std::max<uint64_t>(1, ((uint64_t)std::max(ctx->num_cu, 1) * wpc * 64 * vec) / M);
         uint64_t nsplit = std::min<uint64_t>(want, std::max<uint64_t>(1, R / 64));
         nsplit = std::min<uint64_t>(nsplit, 65536);
         p.upw = (R + nsplit - 1) / nsplit;
@@ -954,7 +975,7 @@ static void reduce_axes_impl(rstsr_hip_ctx* ctx, rstsr_redop_t op, rstsr_dtype_t
             RSTSR_RAISE(InvalidValue, "zero-size array is not supported for max/min");
         RSTSR_RAISE(InvalidLayout, "empty sequence is not allowed for reduce_arg.");
     }
-    RedSetup s = reduce_plan(ctx, la, axes, lo, dtype_size(t));
+    RedSetup s = reduce_plan(ctx, la, axes, lo, t);
     if (s.empty_out) return;
     if (s.empty_reduce) {
         // outputs exist but each reduces zero elements: every output is the empty value
@@ -1080,7 +1101,7 @@ static void reduce_all_common(rstsr_hip_ctx_t* ctx, rstsr_redop_t op, rstsr_dtyp
         return;
     }
     Layout lo({}, {}, 0);
-    s = reduce_plan(ctx, la, all_axes(la.ndim()), lo, dtype_size(t));
+    s = reduce_plan(ctx, la, all_axes(la.ndim()), lo, t);
     size_t part_bytes = 0;
     if (s.p.nsplit > 1) part_bytes = (size_t)s.p.M * s.p.nsplit * 64;  // upper bound of sizeof(S)
     char* ws = (char*)workspace(ctx, part_bytes + 256);
