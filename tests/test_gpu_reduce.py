@@ -41,6 +41,9 @@ CASES = [
     ((5, 6, 7), (0, 1, 2), 1, [0, 1, 2]),
     ((2000, 3), (0, 1), 1, [0]),                # long strided reduced axis, few outputs
     ((3, 100000), (0, 1), 1, [1]),              # long unit reduced axis, few outputs (split)
+    ((1000, 70), (0, 1), 1, [0]),               # column kernel, 15 splits of 67 rows: ragged unroll tails
+    ((999, 33), (0, 1), 1, [0]),                # odd kept extent: one output per thread
+    ((37, 5, 96), (1, 0, 2), 1, [0, 1]),        # two reduced dims that do not chain: runs of 37
 ]
 
 
