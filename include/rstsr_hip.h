@@ -147,10 +147,20 @@ int rstsr_hip_get_stream(rstsr_hip_ctx_t* ctx, void** stream);
 int rstsr_hip_device_count(int* count);
 
 /* ---- storage (DeviceCreationAnyAPI storage/creation.rs:3-34, DeviceStorageAPI storage/device.rs:23-57) ---- */
-/* uninit_impl / empty_impl: nbytes of uninitialised device memory. */
+/* uninit_impl / empty_impl: nbytes of uninitialised device memory (a block freed earlier on
+ * this context is reused when one of the same size class is cached). */
 int rstsr_hip_malloc(rstsr_hip_ctx_t* ctx, size_t nbytes, void** ptr);
-/* Drop of Raw; the library frees only memory this call names. */
+/* Drop of Raw; the library frees only memory this call names. A buffer freed on the context
+ * that allocated it is kept in that context's cache for reuse (ordered by its stream), up to
+ * RSTSR_HIP_CACHE_BYTES (default: a quarter of the device memory); a copy between contexts
+ * finishes before rstsr_hip_memcpy_d2d returns, and with several contexts on one device a
+ * free waits for the device. */
 int rstsr_hip_free(rstsr_hip_ctx_t* ctx, void* ptr);
+/* No counterpart in the reference (its host allocator has no cache): return every cached
+ * block of the context to the runtime, e.g. before another library allocates device memory;
+ * and the bytes the cache holds. */
+int rstsr_hip_empty_cache(rstsr_hip_ctx_t* ctx);
+int rstsr_hip_cached_bytes(rstsr_hip_ctx_t* ctx, size_t* bytes);
 /* outof_cpu_vec / from_cpu_vec (host -> device). */
 int rstsr_hip_memcpy_h2d(rstsr_hip_ctx_t* ctx, void* dst, const void* src, size_t nbytes);
 /* to_cpu_vec / into_cpu_vec / get_index (device -> host, synchronous). */
@@ -285,9 +295,11 @@ int rstsr_hip_driver_syhemm(rstsr_hip_ctx_t* ctx, rstsr_dtype_t t, rstsr_gemm_mo
  * Everything enqueued between begin and end is recorded, not run; entry points that hand
  * values to the host (*_all, memcpy_d2h, allclose) cannot be captured and fail. */
 int rstsr_hip_graph_begin(rstsr_hip_ctx_t* ctx);
-/* Ends capture and returns an executable graph (hipGraphExec_t) in *graph_exec. */
+/* Ends capture and returns an executable graph (hipGraphExec_t) in *graph_exec. Buffers freed
+ * during the capture (temporaries the graph still uses) are held by the graph. */
 int rstsr_hip_graph_end(rstsr_hip_ctx_t* ctx, void** graph_exec);
 int rstsr_hip_graph_launch(rstsr_hip_ctx_t* ctx, void* graph_exec);
+/* Waits for running launches when the graph holds buffers, then frees them with the graph. */
 int rstsr_hip_graph_destroy(void* graph_exec);
 
 /* ---- timing on the context stream (no counterpart in the reference; used by benchmarks) ---- */

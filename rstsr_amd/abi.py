@@ -58,6 +58,8 @@ def lib():
         "rstsr_hip_unary_inplace": [vp, C.c_int, C.c_int, vp, P],
         "rstsr_hip_unary_out_dtype": [C.c_int, C.c_int, C.POINTER(C.c_int)],
         "rstsr_hip_matmul": [vp, C.c_int, vp, P, vp, P, vp, P, vp, vp, C.c_int],
+        "rstsr_hip_empty_cache": [vp],
+        "rstsr_hip_cached_bytes": [vp, C.POINTER(C.c_size_t)],
         "rstsr_hip_gemm": [vp, C.c_int, C.c_int, C.c_size_t, C.c_size_t, C.c_size_t, vp, vp, C.c_ssize_t,
                            C.c_ssize_t, vp, C.c_ssize_t, C.c_ssize_t, vp, vp, C.c_ssize_t, C.c_ssize_t],
         "rstsr_hip_driver_gemm": [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_size_t, C.c_size_t,

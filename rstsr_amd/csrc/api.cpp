@@ -49,6 +49,75 @@ std::pair<uintptr_t, size_t> registry_find(const void* p) {
     return {0, 0};
 }
 
+/* ------------------------------ block cache ------------------------------ */
+// A buffer of rstsr_hip_malloc belongs to the context that allocated it (its owner)
+// and has a capacity (its size class). rstsr_hip_free on the owner keeps the block in
+// the owner's cache, and a later rstsr_hip_malloc of a size in the same class on that
+// context reuses it, with no hipMalloc and no hipFree: each stalls the host (hipFree
+// waits for the device, and a 512 MiB hipMalloc maps pages for about half a
+// millisecond), which would otherwise sit between every allocating operation and the
+// next. Reuse is ordered by the owner's stream: work queued before the free finishes
+// before work queued after the reuse. Two things could break that order, and both are
+// handled: a copy between contexts runs on one context's stream while the other owns
+// a buffer (rstsr_hip_memcpy_d2d waits for such a copy before it returns), and a second
+// context on the same device could still be reading a block (a free then waits for
+// the whole device first). A buffer freed while the stream is captured into a graph
+// is still used by every replay: it is held by that graph and freed by
+// rstsr_hip_graph_destroy. Blocks are returned to the runtime when a hipMalloc fails
+// (then retried), beyond the cache limit, by rstsr_hip_empty_cache and at
+// rstsr_hip_ctx_destroy.
+struct BlockInfo {
+    rstsr_hip_ctx* owner;  // nullptr once the owner is destroyed
+    size_t cap;
+};
+static std::map<void*, BlockInfo>& blocks() {  // guarded by g_reg_mu
+    static std::map<void*, BlockInfo> m;
+    return m;
+}
+// Buffers freed while a graph was being captured, held until that graph is destroyed
+static std::map<void*, std::vector<void*>>& graph_blocks() {  // guarded by g_reg_mu
+    static std::map<void*, std::vector<void*>> m;
+    return m;
+}
+static std::map<int, int>& live_contexts() {  // device -> contexts alive; guarded by g_reg_mu
+    static std::map<int, int> m;
+    return m;
+}
+
+// Capacity class of a request: 256-byte multiples (vectorised tails never leave the
+// allocation), 2 MiB multiples above 1 MiB.
+static size_t size_class(size_t nbytes) {
+    size_t nb = std::max<size_t>((nbytes + 255) & ~(size_t)255, 256);
+    if (nb > ((size_t)1 << 20)) nb = (nb + ((size_t)2 << 20) - 1) & ~(((size_t)2 << 20) - 1);
+    return nb;
+}
+
+// Return every cached block of ctx to the runtime.
+static void release_cache(rstsr_hip_ctx* ctx) {
+    std::vector<void*> drop;
+    {
+        std::lock_guard<std::mutex> g(ctx->cache_mu);
+        for (auto& kv : ctx->cache) drop.push_back(kv.second);
+        ctx->cache.clear();
+        ctx->cached_bytes = 0;
+    }
+    if (drop.empty()) return;
+    HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    for (void* p : drop) {
+        HIP_CHECK(hipFree(p));
+        std::lock_guard<std::mutex> g(g_reg_mu);
+        blocks().erase(p);
+    }
+}
+
+static rstsr_hip_ctx* block_owner(const void* p) {
+    auto f = registry_find(p);
+    if (f.second == 0) return nullptr;
+    std::lock_guard<std::mutex> g(g_reg_mu);
+    auto it = blocks().find((void*)f.first);
+    return it == blocks().end() ? nullptr : it->second.owner;
+}
+
 bool same_buffer(const void* x, const void* y) {
     auto a = registry_find(x), b = registry_find(y);
     if (a.second == 0 || b.second == 0) return true;
@@ -313,11 +382,18 @@ int rstsr_hip_ctx_create(int device_id, rstsr_hip_ctx_t** out) {
             ctx->l2_bytes = prop.l2CacheSize;
             HIP_CHECK(hipHostMalloc(&ctx->host_pinned, 4096, hipHostMallocDefault));
             for (auto& ev : ctx->events) HIP_CHECK(hipEventCreate(&ev));
+            // cache limit: RSTSR_HIP_CACHE_BYTES, else a quarter of the device memory
+            ctx->cache_max = prop.totalGlobalMem / 4;
+            if (const char* e = std::getenv("RSTSR_HIP_CACHE_BYTES")) ctx->cache_max = (size_t)std::strtoull(e, nullptr, 10);
         } catch (...) {
             if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
             if (ctx->host_pinned) (void)hipHostFree(ctx->host_pinned);
             delete ctx;
             throw;
+        }
+        {
+            std::lock_guard<std::mutex> g(g_reg_mu);
+            live_contexts()[device_id] += 1;
         }
         *out = ctx;
     });
@@ -328,6 +404,14 @@ int rstsr_hip_ctx_destroy(rstsr_hip_ctx_t* ctx) {
         if (!ctx) return;
         bind(ctx);
         HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        release_cache(ctx);
+        {
+            // buffers still held by callers outlive the context: a later free returns them
+            std::lock_guard<std::mutex> g(g_reg_mu);
+            for (auto& kv : blocks())
+                if (kv.second.owner == ctx) kv.second.owner = nullptr;
+            live_contexts()[ctx->device] -= 1;
+        }
         if (ctx->ws.ptr) HIP_CHECK(hipFree(ctx->ws.ptr));
         if (ctx->aux.ptr) HIP_CHECK(hipFree(ctx->aux.ptr));
         if (ctx->zbuf.ptr) HIP_CHECK(hipFree(ctx->zbuf.ptr));
@@ -367,13 +451,32 @@ int rstsr_hip_malloc(rstsr_hip_ctx_t* ctx, size_t nbytes, void** ptr) {
     return guarded([&] {
         if (!ptr) RSTSR_RAISE(InvalidValue, "null output");
         bind(ctx);
+        const size_t cls = size_class(nbytes);
         void* p = nullptr;
-        // round up so vectorised tails never leave the allocation
-        size_t nb = std::max<size_t>((nbytes + 255) & ~(size_t)255, 256);
-        hipError_t e = hipMalloc(&p, nb);
-        if (e != hipSuccess) {
-            (void)hipGetLastError();
-            throw Error(ErrorKind::MemoryError, "hipMalloc of " + std::to_string(nbytes) + " bytes failed: " + hipGetErrorString(e));
+        {
+            // a cached block of this class, or up to 1/8 larger
+            std::lock_guard<std::mutex> g(ctx->cache_mu);
+            auto it = ctx->cache.lower_bound(cls);
+            if (it != ctx->cache.end() && it->first <= cls + cls / 8) {
+                p = it->second;
+                ctx->cached_bytes -= it->first;
+                ctx->cache.erase(it);
+            }
+        }
+        if (!p) {
+            hipError_t e = hipMalloc(&p, cls);
+            if (e != hipSuccess) {
+                (void)hipGetLastError();
+                release_cache(ctx);
+                e = hipMalloc(&p, cls);
+            }
+            if (e != hipSuccess) {
+                (void)hipGetLastError();
+                throw Error(ErrorKind::MemoryError,
+                            "hipMalloc of " + std::to_string(nbytes) + " bytes failed: " + hipGetErrorString(e));
+            }
+            std::lock_guard<std::mutex> g(g_reg_mu);
+            blocks()[p] = BlockInfo{ctx, cls};
         }
         registry_add(p, nbytes);
         *ptr = p;
@@ -385,18 +488,59 @@ int rstsr_hip_free(rstsr_hip_ctx_t* ctx, void* ptr) {
         if (!ptr) return;
         bind(ctx);
         if (!registry_remove(ptr)) RSTSR_RAISE(InvalidValue, "free of a pointer not returned by rstsr_hip_malloc");
-        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
-        HIP_CHECK(hipStreamIsCapturing(ctx->stream, &cs));
-        if (cs != hipStreamCaptureStatusNone) {
-            // a capturing stream cannot be synchronised: release after capture ends
+        BlockInfo bi{nullptr, 0};
+        int live = 0;
+        {
+            std::lock_guard<std::mutex> g(g_reg_mu);
+            auto it = blocks().find(ptr);
+            if (it != blocks().end()) bi = it->second;
+            live = live_contexts()[ctx->device];
+        }
+        const bool capturing = stream_capturing(ctx);
+        if (bi.owner == ctx && !capturing) {
+            // another context on this device may still read the block
+            if (live > 1) HIP_CHECK(hipDeviceSynchronize());
+            std::lock_guard<std::mutex> g(ctx->cache_mu);
+            if (ctx->cached_bytes + bi.cap <= ctx->cache_max) {
+                ctx->cache.emplace(bi.cap, ptr);
+                ctx->cached_bytes += bi.cap;
+                return;
+            }
+        }
+        if (capturing) {
+            // the graph being captured still writes and reads this buffer at every replay:
+            // it belongs to the graph from here on (rstsr_hip_graph_end), never to the cache
             ctx->deferred.push_back(ptr);
             return;
         }
-        // frees are stream ordered with respect to this context's queued work
-        HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        // returned to the runtime once every queued use has finished
+        HIP_CHECK(hipDeviceSynchronize());
         HIP_CHECK(hipFree(ptr));
-        for (void* p : ctx->deferred) HIP_CHECK(hipFree(p));
+        {
+            std::lock_guard<std::mutex> g(g_reg_mu);
+            blocks().erase(ptr);
+        }
+        for (void* p : ctx->deferred) {
+            HIP_CHECK(hipFree(p));
+            std::lock_guard<std::mutex> g(g_reg_mu);
+            blocks().erase(p);
+        }
         ctx->deferred.clear();
+    });
+}
+
+int rstsr_hip_empty_cache(rstsr_hip_ctx_t* ctx) {
+    return guarded([&] {
+        bind(ctx);
+        release_cache(ctx);
+    });
+}
+
+int rstsr_hip_cached_bytes(rstsr_hip_ctx_t* ctx, size_t* bytes) {
+    return guarded([&] {
+        if (!ctx || !bytes) RSTSR_RAISE(InvalidValue, "null argument");
+        std::lock_guard<std::mutex> g(ctx->cache_mu);
+        *bytes = ctx->cached_bytes;
     });
 }
 
@@ -453,6 +597,14 @@ int rstsr_hip_memcpy_d2d(rstsr_hip_ctx_t* ctx, void* dst, const void* src, size_
         check_extent_bytes(dst, nbytes, "dst");
         check_extent_bytes(src, nbytes, "src");
         HIP_CHECK(hipMemcpyAsync(dst, src, nbytes, hipMemcpyDeviceToDevice, ctx->stream));
+        // a buffer of another context may be freed into that context's cache and
+        // reused on its stream as soon as this returns: finish the copy first
+        rstsr_hip_ctx* od = block_owner(dst);
+        rstsr_hip_ctx* os = block_owner(src);
+        if ((od && od != ctx) || (os && os != ctx)) {
+            if (stream_capturing(ctx)) RSTSR_RAISE(UnImplemented, "copy between contexts during stream capture");
+            HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        }
     });
 }
 
@@ -489,7 +641,21 @@ int rstsr_hip_graph_end(rstsr_hip_ctx_t* ctx, void** graph_exec) {
         hipGraphExec_t ex = nullptr;
         hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
         (void)hipGraphDestroy(g);
-        if (e != hipSuccess) throw Error(ErrorKind::DeviceError, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+        std::vector<void*> held;
+        held.swap(ctx->deferred);  // buffers freed during the capture
+        if (e != hipSuccess) {
+            // no graph uses them
+            for (void* p : held) {
+                HIP_CHECK(hipFree(p));
+                std::lock_guard<std::mutex> lg(g_reg_mu);
+                blocks().erase(p);
+            }
+            throw Error(ErrorKind::DeviceError, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+        }
+        if (!held.empty()) {
+            std::lock_guard<std::mutex> lg(g_reg_mu);
+            graph_blocks()[(void*)ex] = std::move(held);
+        }
         *graph_exec = (void*)ex;
     });
 }
@@ -504,7 +670,24 @@ int rstsr_hip_graph_launch(rstsr_hip_ctx_t* ctx, void* graph_exec) {
 
 int rstsr_hip_graph_destroy(void* graph_exec) {
     return guarded([&] {
-        if (graph_exec) HIP_CHECK(hipGraphExecDestroy((hipGraphExec_t)graph_exec));
+        if (!graph_exec) return;
+        std::vector<void*> held;
+        {
+            std::lock_guard<std::mutex> lg(g_reg_mu);
+            auto it = graph_blocks().find(graph_exec);
+            if (it != graph_blocks().end()) {
+                held.swap(it->second);
+                graph_blocks().erase(it);
+            }
+        }
+        // a launch of the graph may still be running: it uses the buffers it holds
+        if (!held.empty()) HIP_CHECK(hipDeviceSynchronize());
+        HIP_CHECK(hipGraphExecDestroy((hipGraphExec_t)graph_exec));
+        for (void* p : held) {
+            HIP_CHECK(hipFree(p));
+            std::lock_guard<std::mutex> lg(g_reg_mu);
+            blocks().erase(p);
+        }
     });
 }
 

@@ -7,6 +7,7 @@
 #include <cstdint>
 #include <cstring>
 #include <functional>
+#include <map>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -77,6 +78,11 @@ struct rstsr_hip_ctx {
     // two pinned chunks for host transfers of pageable memory (allocated on first use)
     void* stage[2] = {nullptr, nullptr};
     hipEvent_t stage_ev[2] = {nullptr, nullptr};
+    // freed caller buffers kept for reuse, by capacity (rstsr_hip_malloc / rstsr_hip_free)
+    std::mutex cache_mu;
+    std::multimap<size_t, void*> cache;
+    size_t cached_bytes = 0;
+    size_t cache_max = 0;  // bytes; frees beyond it return memory to the runtime
 };
 
 namespace rstsr {
