@@ -25,6 +25,7 @@
 #include "registry.hpp"
 #include "gemm.hpp"
 
+#include <algorithm>
 #include <functional>
 
 namespace rstsr {
@@ -652,9 +653,16 @@ static void launch_grouped_cfg(rstsr_hip_ctx* ctx, const std::vector<TileArgs<T>
                                bool beta_zero, const int* run_if = nullptr) {
     std::vector<GroupProblem<T>> gp(probs.size());
     std::vector<uint32_t> prefix(probs.size() + 1, 0);
+    // Tiles are handed out in problem order to blocks that stride over them: longest K
+    // (the time of one tile) first, so that short tiles fill the end of the launch
+    // instead of a few long ones trailing it. Each problem writes only its own C, so the
+    // order changes no result.
+    std::vector<size_t> order(probs.size());
+    for (size_t i = 0; i < order.size(); ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) { return probs[x].K > probs[y].K; });
     uint64_t total = 0;
     for (size_t i = 0; i < probs.size(); ++i) {
-        const auto& t = probs[i];
+        const auto& t = probs[order[i]];
         uint32_t tm = ceil_div_u32(t.M, CF::BM), tn = ceil_div_u32(t.N, CF::BN);
         gp[i].t = t;
         gp[i].tiles_n = tn;
