@@ -67,6 +67,26 @@ def default_device():
     return _default_device
 
 
+def empty_cache(device=None):
+    """Return the device buffers the context keeps for reuse (freed tensors) to the HIP runtime."""
+    import ctypes as C
+    from . import abi
+
+    dev = device if device is not None else default_device()
+    abi.check(abi.lib().rstsr_hip_empty_cache(C.c_void_p(dev._ctx_ptr())))
+
+
+def cached_bytes(device=None):
+    """Bytes of freed device buffers the context keeps for reuse."""
+    import ctypes as C
+    from . import abi
+
+    dev = device if device is not None else default_device()
+    n = C.c_size_t()
+    abi.check(abi.lib().rstsr_hip_cached_bytes(C.c_void_p(dev._ctx_ptr()), C.byref(n)))
+    return n.value
+
+
 def _code(dtype):
     dt = np.dtype(dtype)
     if dt not in _NP2CODE:

@@ -131,3 +131,11 @@ def test_capture_temporaries_not_recycled(dev):
         for i, h in enumerate(held):
             assert np.all(h.to_numpy() == float(i)), i
     del g
+
+
+def test_python_cache_helpers(dev):
+    t = rt.zeros([1 << 20], device=dev)
+    del t
+    assert rt.cached_bytes(dev) >= 8 * MiB
+    rt.empty_cache(dev)
+    assert rt.cached_bytes(dev) == 0
