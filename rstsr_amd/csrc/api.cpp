@@ -341,6 +341,29 @@ int tune_select(rstsr_hip_ctx* ctx, const std::string& key, int ncand, int heuri
 }
 
 void* workspace(rstsr_hip_ctx* ctx, size_t bytes) { return grow(ctx, ctx->ws, bytes); }
+void* meta_stage(rstsr_hip_ctx* ctx, size_t bytes, const char* what) {
+    if (stream_capturing(ctx)) RSTSR_RAISE(UnImplemented, std::string(what) + " cannot be captured into a graph");
+    const int s = ctx->meta_slot;
+    if (ctx->meta_ev[s]) HIP_CHECK(hipEventSynchronize(ctx->meta_ev[s]));  // its last copy has run
+    if (ctx->meta_cap[s] < bytes) {
+        if (ctx->meta_host[s]) HIP_CHECK(hipHostFree(ctx->meta_host[s]));
+        ctx->meta_host[s] = nullptr;
+        ctx->meta_cap[s] = 0;
+        const size_t cap = (bytes + 65535) & ~(size_t)65535;
+        HIP_CHECK(hipHostMalloc(&ctx->meta_host[s], cap, hipHostMallocDefault));
+        ctx->meta_cap[s] = cap;
+    }
+    return ctx->meta_host[s];
+}
+
+void meta_upload(rstsr_hip_ctx* ctx, void* dst, size_t bytes) {
+    const int s = ctx->meta_slot;
+    HIP_CHECK(hipMemcpyAsync(dst, ctx->meta_host[s], bytes, hipMemcpyHostToDevice, ctx->stream));
+    if (!ctx->meta_ev[s]) HIP_CHECK(hipEventCreateWithFlags(&ctx->meta_ev[s], hipEventDisableTiming));
+    HIP_CHECK(hipEventRecord(ctx->meta_ev[s], ctx->stream));
+    ctx->meta_slot = s ^ 1;
+}
+
 void* aux_buffer(rstsr_hip_ctx* ctx, size_t bytes) { return grow(ctx, ctx->aux, bytes); }
 void* zbuf_buffer(rstsr_hip_ctx* ctx, size_t bytes) { return grow(ctx, ctx->zbuf, bytes); }
 
@@ -423,6 +446,8 @@ int rstsr_hip_ctx_destroy(rstsr_hip_ctx_t* ctx) {
         for (int i = 0; i < 2; ++i) {
             if (ctx->stage[i]) HIP_CHECK(hipHostFree(ctx->stage[i]));
             if (ctx->stage_ev[i]) HIP_CHECK(hipEventDestroy(ctx->stage_ev[i]));
+            if (ctx->meta_host[i]) HIP_CHECK(hipHostFree(ctx->meta_host[i]));
+            if (ctx->meta_ev[i]) HIP_CHECK(hipEventDestroy(ctx->meta_ev[i]));
         }
         HIP_CHECK(hipStreamDestroy(ctx->stream));
         delete ctx;

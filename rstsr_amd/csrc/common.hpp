@@ -78,6 +78,11 @@ struct rstsr_hip_ctx {
     // two pinned chunks for host transfers of pageable memory (allocated on first use)
     void* stage[2] = {nullptr, nullptr};
     hipEvent_t stage_ev[2] = {nullptr, nullptr};
+    // two pinned host slots for per-call problem tables (meta_stage / meta_upload)
+    void* meta_host[2] = {nullptr, nullptr};
+    size_t meta_cap[2] = {0, 0};
+    hipEvent_t meta_ev[2] = {nullptr, nullptr};
+    int meta_slot = 0;
     // freed caller buffers kept for reuse, by capacity (rstsr_hip_malloc / rstsr_hip_free)
     std::mutex cache_mu;
     std::multimap<size_t, void*> cache;
@@ -104,6 +109,15 @@ void* zbuf_buffer(rstsr_hip_ctx* ctx, size_t bytes);
 
 // True while the context stream is being captured into a graph.
 bool stream_capturing(rstsr_hip_ctx* ctx);
+
+// Per-call host tables (problem descriptors, prefix sums) reach the device without the
+// host waiting for the copy: meta_stage returns a pinned host region of `bytes` for the
+// caller to fill, and meta_upload copies it to `dst` on the context stream. Two regions
+// alternate, and each is refilled only after its previous copy has run (an event), so the
+// host prepares the next call while the device works on this one. Not available while
+// the stream is captured (a replay would read a region refilled since).
+void* meta_stage(rstsr_hip_ctx* ctx, size_t bytes, const char* what);
+void meta_upload(rstsr_hip_ctx* ctx, void* dst, size_t bytes);
 
 // Autotuner: run(c) performs the whole call with candidate c (all candidates give
 // identical results). The first call for `key` with can_time runs the heuristic

@@ -17,6 +17,7 @@
 // imaginary LDS planes and four real MFMAs per complex tile (4M form).
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
 #include <map>
 #include <string>
 #include <type_traits>
@@ -677,20 +678,19 @@ static void launch_grouped_cfg(rstsr_hip_ctx* ctx, const std::vector<TileArgs<T>
     prefix[probs.size()] = (uint32_t)total;
     if (total == 0) return;
     if (total >= (1ull << 31)) RSTSR_RAISE(UnImplemented, "grouped GEMM has too many tiles");
-    size_t bytes = gp.size() * sizeof(GroupProblem<T>) + prefix.size() * sizeof(uint32_t) + 256;
-    char* meta = (char*)aux_buffer(ctx, bytes);
-    HIP_CHECK(hipMemcpyAsync(meta, gp.data(), gp.size() * sizeof(GroupProblem<T>), hipMemcpyHostToDevice, ctx->stream));
-    size_t poff = (gp.size() * sizeof(GroupProblem<T>) + 255) & ~(size_t)255;
-    HIP_CHECK(hipMemcpyAsync(meta + poff, prefix.data(), prefix.size() * sizeof(uint32_t), hipMemcpyHostToDevice,
-                             ctx->stream));
+    const size_t poff = (gp.size() * sizeof(GroupProblem<T>) + 255) & ~(size_t)255;
+    const size_t bytes = poff + prefix.size() * sizeof(uint32_t);
+    char* meta = (char*)aux_buffer(ctx, bytes + 256);
+    char* host = (char*)meta_stage(ctx, bytes, "grouped GEMM");
+    std::memcpy(host, gp.data(), gp.size() * sizeof(GroupProblem<T>));
+    std::memcpy(host + poff, prefix.data(), prefix.size() * sizeof(uint32_t));
+    meta_upload(ctx, meta, bytes);
     // persistent grid: as many blocks as can be resident at once
     uint32_t grid = (uint32_t)std::min<uint64_t>(total, (uint64_t)std::max(ctx->num_cu, 1) * CF::OCC);
     hipLaunchKernelGGL((k_gemm_grouped<T, CF>), dim3(grid), dim3(CF::NT), 0, ctx->stream,
                        (const GroupProblem<T>*)meta, (int)gp.size(), (const uint32_t*)(meta + poff), (uint32_t)total,
                        alpha, beta, beta_zero ? 1 : 0, run_if);
     check_launch("grouped gemm");
-    // the host staging vectors die with this call: wait for the copies
-    HIP_CHECK(hipStreamSynchronize(ctx->stream));
 }
 
 template <class T>

@@ -1058,8 +1058,14 @@ void ozaki_grouped_f64(rstsr_hip_ctx* ctx, const std::vector<GemmProblem>& probs
     int8_t* A8 = (int8_t*)(ws + o_A8);
     int8_t* B8 = (int8_t*)(ws + o_B8);
     uint8_t* R = (uint8_t*)(ws + o_R);
-    HIP_CHECK(hipMemcpyAsync(ws + o_pr, op.data(), op.size() * sizeof(OzProb), hipMemcpyHostToDevice, ctx->stream));
-    HIP_CHECK(hipMemcpyAsync(ws + o_pre, pre.data(), pre.size() * sizeof(uint32_t), hipMemcpyHostToDevice, ctx->stream));
+    {
+        // problems and prefix tables: one upload of [o_pr, o_pre + prefix bytes)
+        const size_t span = o_pre - o_pr + pre.size() * sizeof(uint32_t);
+        char* host = (char*)meta_stage(ctx, span, "grouped split GEMM");
+        std::memcpy(host, op.data(), op.size() * sizeof(OzProb));
+        std::memcpy(host + (o_pre - o_pr), pre.data(), pre.size() * sizeof(uint32_t));
+        meta_upload(ctx, ws + o_pr, span);
+    }
     hipLaunchKernelGGL(k_zero_i32, dim3(ceil_div_u32(n_zero, 256)), dim3(256), 0, ctx->stream, flag, (int64_t)n_zero);
     check_launch("grouped ozaki zero");
     hipLaunchKernelGGL(k_gnorm, dim3((uint32_t)tn), dim3(256), 0, ctx->stream, dprobs, count, dpre + 0 * (count + 1),
@@ -1078,8 +1084,6 @@ void ozaki_grouped_f64(rstsr_hip_ctx* ctx, const std::vector<GemmProblem>& probs
     hipLaunchKernelGGL(k_gcrt, dim3((uint32_t)tc), dim3(256), 0, ctx->stream, dprobs, count, dpre + 3 * (count + 1), R,
                        (int64_t)sR, su.cc, eA, fB, alpha, beta, beta_zero ? 1 : 0, flag);
     check_launch("grouped ozaki crt");
-    // the host tables die with this call: wait for the copies
-    HIP_CHECK(hipStreamSynchronize(ctx->stream));
 }
 
 }  // namespace hip

@@ -247,6 +247,39 @@ def test_grouped_uneven(dev, oracle, dtype, mode):
                 assert np.max(np.abs(got - ref)) <= 1e-14 * np.max(np.abs(ref))
 
 
+@pytest.mark.parametrize("mode", ["plain", "ozaki"])
+def test_grouped_back_to_back_calls(dev, oracle, mode):
+    """Five grouped calls with different problem tables queued without a host wait between them:
+    the tables travel through two alternating pinned regions, each refilled only after its
+    previous copy ran; every call's outputs are checked afterwards."""
+    rng = np.random.default_rng(21)
+    calls = []
+    for c in range(5):
+        probs, hosts = [], []
+        for g in range(3 + 4 * c):
+            m, n, k = (int(x) for x in rng.integers(1, 24, size=3) * 8)
+            A, B = rng.normal(size=(m, k)), rng.normal(size=(k, n))
+            probs.append((rt.asarray(A, device=dev), rt.asarray(B, device=dev), rt.empty((m, n), np.float64, device=dev)))
+            hosts.append((A, B))
+        calls.append((probs, hosts))
+    for probs, _ in calls:
+        rt.gemm_grouped(probs, mode=mode)
+    for probs, hosts in calls:
+        for (A, B), (_, _, tc) in zip(hosts, probs):
+            ref = oracle.gemm_dot2(A, B)
+            assert np.max(np.abs(tc.to_numpy() - ref)) <= 1e-13 * max(np.max(np.abs(ref)), 1.0)
+
+
+def test_grouped_not_capturable(dev):
+    A = rt.asarray(np.ones((8, 8)), device=dev)
+    C = rt.empty((8, 8), np.float64, device=dev)
+    with pytest.raises(Exception, match="cannot be captured"):
+        with rt.capture(dev):
+            rt.gemm_grouped([(A, A, C)])
+    rt.gemm_grouped([(A, A, C)])  # the context works normally afterwards
+    assert np.array_equal(C.to_numpy(), np.full((8, 8), 8.0))
+
+
 def test_grouped_alpha_beta(dev, oracle):
     rng = np.random.default_rng(12)
     A = rng.normal(size=(24, 40))
