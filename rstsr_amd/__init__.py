@@ -626,12 +626,6 @@ def allclose(a, b, rtol=1e-5, atol=1e-8, equal_nan=False):
     return a._t.allclose(b._t, rtol, atol, equal_nan)
 
 
-def _elem_ptr(t):
-    import ctypes as C
-
-    return C.c_void_p(t._t._data_ptr() + t.offset * t.dtype.itemsize)
-
-
 def gemm_grouped(problems, mode="plain", alpha=1.0, beta=0.0):
     """C_g = alpha * A_g @ B_g + beta * C_g for a list of 2-D (A, B, C) tensors of uneven shapes.
 
@@ -642,16 +636,19 @@ def gemm_grouped(problems, mode="plain", alpha=1.0, beta=0.0):
     if not problems:
         return
     dt = problems[0][0].dtype
+    isz = np.dtype(dt).itemsize
     descs = (abi.GemmDesc * len(problems))()
     for i, (a, b, c) in enumerate(problems):
-        if a.ndim != 2 or b.ndim != 2 or c.ndim != 2:
+        # one layout read per operand (shape, stride, offset): a call builds hundreds of these
+        (sa, ta, oa), (sb, tb, ob), (sc, tc, oc) = a._t.layout, b._t.layout, c._t.layout
+        if len(sa) != 2 or len(sb) != 2 or len(sc) != 2:
             raise RSTSRError("InvalidLayout", "gemm_grouped takes 2-D operands")
-        m, k = a.shape
-        k2, n = b.shape
-        if k != k2 or c.shape != (m, n):
+        m, k = sa
+        k2, n = sb
+        if k != k2 or sc[0] != m or sc[1] != n:
             raise RSTSRError("InvalidLayout", "gemm_grouped shape mismatch")
-        descs[i] = abi.GemmDesc(m, n, k, _elem_ptr(a), a.stride[0], a.stride[1], _elem_ptr(b), b.stride[0],
-                                b.stride[1], _elem_ptr(c), c.stride[0], c.stride[1])
+        descs[i] = abi.GemmDesc(m, n, k, a._t._data_ptr() + oa * isz, ta[0], ta[1], b._t._data_ptr() + ob * isz,
+                                tb[0], tb[1], c._t._data_ptr() + oc * isz, tc[0], tc[1])
     al = np.array(alpha, dtype=dt)
     be = np.array(beta, dtype=dt)
     ctx = C.c_void_p(problems[0][0].device._ctx_ptr())
