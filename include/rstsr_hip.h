@@ -270,7 +270,9 @@ typedef struct {
  * the f64 split mode runs each of its stages (norms, residue split, one INT8 launch over all
  * moduli, CRT) as one launch over the whole group; other dtypes run one call per problem. No
  * trait in the reference takes a list of problems; the closest is DeviceFaer's batched loop
- * over matmul tasks (device_faer/matmul.rs:151-199). */
+ * over matmul tasks (device_faer/matmul.rs:151-199). The descriptor array is read before the
+ * call returns (the caller may free it); the products run on the context stream without the
+ * host waiting. Not available during graph capture (UnImplemented). */
 int rstsr_hip_gemm_grouped(rstsr_hip_ctx_t* ctx, rstsr_dtype_t t, rstsr_gemm_mode_t mode, size_t count,
                            const rstsr_gemm_desc_t* descs, const void* alpha, const void* beta);
 
