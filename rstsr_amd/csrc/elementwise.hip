@@ -443,6 +443,8 @@ int rstsr_hip_allclose(rstsr_hip_ctx_t* ctx, rstsr_dtype_t t, const void* a, con
     return guarded([&] {
         bind(ctx);
         if (!host_out) RSTSR_RAISE(InvalidValue, "null output");
+        // refused before anything is enqueued (its flag is cleared with a memset)
+        if (stream_capturing(ctx)) RSTSR_RAISE(UnImplemented, "allclose hands its value to the host and cannot be captured");
         Layout la = from_abi(la_), lb = from_abi(lb_);
         if (la.size() == 0 || lb.size() == 0) RSTSR_RAISE(InvalidValue, "zero-size array is not supported for allclose");
         auto bl = broadcast_layout(la, lb, FlagOrder::RowMajor);

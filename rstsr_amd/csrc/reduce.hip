@@ -1081,6 +1081,8 @@ static void reduce_all_common(rstsr_hip_ctx_t* ctx, rstsr_redop_t op, rstsr_dtyp
                               const rstsr_layout_t* la_, void* host_out, size_t out_bytes) {
     bind(ctx);
     if (!host_out) RSTSR_RAISE(InvalidValue, "null output");
+    if (stream_capturing(ctx))
+        RSTSR_RAISE(UnImplemented, "a reduction over all elements hands its value to the host and cannot be captured");
     Layout la = from_abi(la_);
     check_extent(a, la, dtype_size(t), "a");
     std::lock_guard<std::mutex> g(ctx->mu);

@@ -571,3 +571,22 @@ def test_zgemm_ozaki_four_products_with_beta(dev):
     ref = 1.5 * (A.astype(np.clongdouble) @ B.astype(np.clongdouble)) - 0.5 * C0
     err = float(np.max(np.abs(tc.to_numpy() - ref)) / np.max(np.abs(ref)))
     assert err <= 1e-14, err
+
+
+def test_host_value_entry_points_refuse_capture(dev):
+    """sum_all and allclose hand a value to the host: under capture they fail before enqueuing
+    anything, and the captured graph and the context stay usable."""
+    x = rt.asarray(np.arange(16.0), device=dev)
+    y = rt.asarray(np.arange(16.0), device=dev)
+    with pytest.raises(Exception, match="cannot be captured"):
+        with rt.capture(dev):
+            x.sum_all()
+    with pytest.raises(Exception, match="cannot be captured"):
+        with rt.capture(dev):
+            rt.allclose(x, y)
+    with rt.capture(dev) as g:
+        z = x + y
+    g.launch()
+    dev.synchronize()
+    assert np.array_equal(z.to_numpy(), 2 * np.arange(16.0))
+    assert x.sum_all() == 120.0 and rt.allclose(x, y)
