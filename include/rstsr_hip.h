@@ -144,6 +144,11 @@ const char* rstsr_hip_last_error(void);
 int rstsr_hip_synchronize(rstsr_hip_ctx_t* ctx);
 /* hipStream_t of the context, for callers that enqueue their own work on it. */
 int rstsr_hip_get_stream(rstsr_hip_ctx_t* ctx, void** stream);
+/* No counterpart in the reference (a CPU device has no queue): order ctx's stream after the
+ * work already queued on `other` (an event on other's stream, waited on by ctx's stream;
+ * during a capture on ctx, a host wait for other's stream). Call it before ctx's work reads
+ * or writes a buffer that another context on the same device last touched. */
+int rstsr_hip_stream_wait(rstsr_hip_ctx_t* ctx, rstsr_hip_ctx_t* other);
 int rstsr_hip_device_count(int* count);
 
 /* ---- storage (DeviceCreationAnyAPI storage/creation.rs:3-34, DeviceStorageAPI storage/device.rs:23-57) ---- */

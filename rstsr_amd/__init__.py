@@ -20,7 +20,7 @@ try:
     from . import _rstsr
 except ImportError as e:  # pragma: no cover - exercised only on a broken build
     raise ImportError(
-        "rstsr_amd native modules are not built; run `python -m rstsr_amd.build` "
+        "rstsr_amd native modules are not built; run `python rstsr_amd/build.py` "
         "(needs hipcc for gfx950): " + str(e)
     ) from e
 
@@ -688,6 +688,9 @@ class capture:
         if et is None:
             abi.check(rc)
             self.exec = g
+        elif rc == 0 and g:
+            # the block raised: nobody will launch this graph; free it and what it holds
+            abi.lib().rstsr_hip_graph_destroy(g)
         return False
 
     def launch(self):

@@ -44,7 +44,7 @@ def lib():
     if _lib is not None:
         return _lib
     if not os.path.exists(LIB_PATH):
-        raise RuntimeError("librstsr_hip.so is not built: run `python -m rstsr_amd.build`")
+        raise RuntimeError("librstsr_hip.so is not built: run `python rstsr_amd/build.py`")
     L = C.CDLL(LIB_PATH)
     vp, P = C.c_void_p, C.POINTER(Layout)
     sig = {
@@ -80,6 +80,7 @@ def lib():
         "rstsr_hip_graph_end": [vp, C.POINTER(C.c_void_p)],
         "rstsr_hip_graph_launch": [vp, vp],
         "rstsr_hip_graph_destroy": [vp],
+        "rstsr_hip_stream_wait": [vp, vp],
     }
     for name, args in sig.items():
         f = getattr(L, name)

@@ -1,7 +1,9 @@
 """Build the gfx950 device library, the C++ host front end and the CPU oracle, in tree.
 
-    python -m rstsr_amd.build            # everything
-    python -m rstsr_amd.build --lib      # librstsr_hip.so only
+    python rstsr_amd/build.py            # everything
+    python rstsr_amd/build.py --lib      # librstsr_hip.so only
+
+(run by path: the package itself refuses to import until the native modules exist)
 
 Outputs:
     rstsr_amd/librstsr_hip.so            C ABI of include/rstsr_hip.h (hipcc, --offload-arch=gfx950)

@@ -164,13 +164,14 @@ void check_extent_bytes(const void* ptr, size_t bytes, const char* name) {
 }
 
 static void* grow(rstsr_hip_ctx* ctx, Workspace& w, size_t bytes) {
+    const bool capturing = stream_capturing(ctx);
+    // a kernel recorded now keeps this buffer's address: the graph takes it at graph_end
+    if (capturing) w.captured = true;
     if (bytes <= w.bytes) return w.ptr;
     size_t nb = std::max<size_t>(bytes, w.bytes * 2);
     nb = (nb + 4095) & ~(size_t)4095;
-    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
-    HIP_CHECK(hipStreamIsCapturing(ctx->stream, &cs));
     if (w.ptr) {
-        if (cs != hipStreamCaptureStatusNone) {
+        if (capturing) {
             ctx->retired.push_back(w.ptr);
         } else {
             HIP_CHECK(hipStreamSynchronize(ctx->stream));
@@ -181,9 +182,16 @@ static void* grow(rstsr_hip_ctx* ctx, Workspace& w, size_t bytes) {
     }
     void* p = nullptr;
     hipError_t e = hipMalloc(&p, nb);
+    if (e != hipSuccess && !capturing) {
+        // freed caller buffers cached on this context may be what is missing
+        (void)hipGetLastError();
+        release_cache(ctx);
+        e = hipMalloc(&p, nb);
+    }
     if (e != hipSuccess) {
         (void)hipGetLastError();
-        throw Error(ErrorKind::MemoryError, "workspace hipMalloc failed: " + std::string(hipGetErrorString(e)));
+        throw Error(ErrorKind::MemoryError, "workspace hipMalloc of " + std::to_string(nb) +
+                                                " bytes failed: " + std::string(hipGetErrorString(e)));
     }
     w.ptr = p;
     w.bytes = nb;
@@ -227,6 +235,8 @@ static void par_memcpy(void* d, const void* s, size_t n) {
 // ones go through the runtime. Mode 1 forces the runtime path, mode 2 the
 // staged path (for measurement).
 static void host_transfer(rstsr_hip_ctx* ctx, void* dst, const void* src, size_t n, bool h2d, int mode = 0) {
+    // refused before anything is enqueued: a synchronising copy would invalidate the capture
+    if (stream_capturing(ctx)) RSTSR_RAISE(InvalidValue, "host transfers cannot be captured into a graph");
     const void* host = h2d ? src : dst;
     const bool staged = mode == 2 || (mode == 0 && n >= STAGE_MIN && !host_is_pinned(host));
     const hipMemcpyKind kind = h2d ? hipMemcpyHostToDevice : hipMemcpyDeviceToHost;
@@ -235,7 +245,6 @@ static void host_transfer(rstsr_hip_ctx* ctx, void* dst, const void* src, size_t
         HIP_CHECK(hipStreamSynchronize(ctx->stream));
         return;
     }
-    if (stream_capturing(ctx)) RSTSR_RAISE(InvalidValue, "host transfers cannot be captured into a graph");
     for (int i = 0; i < 2; ++i) {
         if (!ctx->stage[i]) HIP_CHECK(hipHostMalloc(&ctx->stage[i], STAGE_CHUNK, hipHostMallocDefault));
         if (!ctx->stage_ev[i]) HIP_CHECK(hipEventCreateWithFlags(&ctx->stage_ev[i], hipEventDisableTiming));
@@ -329,6 +338,8 @@ int tune_select(rstsr_hip_ctx* ctx, const std::string& key, int ncand, int heuri
             }
         }
     }
+    // the call's result comes from the configuration later calls will use
+    if (bi != ncand - 1) run(bi);
     std::lock_guard<std::mutex> g(g_tune_mu);
     tune_table()[key] = bi;
     if (const char* f = std::getenv("RSTSR_HIP_TUNE_CACHE")) {
@@ -442,6 +453,7 @@ int rstsr_hip_ctx_destroy(rstsr_hip_ctx_t* ctx) {
         for (void* p : ctx->deferred) HIP_CHECK(hipFree(p));
         for (auto& ev : ctx->events)
             if (ev) HIP_CHECK(hipEventDestroy(ev));
+        if (ctx->order_ev) HIP_CHECK(hipEventDestroy(ctx->order_ev));
         if (ctx->host_pinned) HIP_CHECK(hipHostFree(ctx->host_pinned));
         for (int i = 0; i < 2; ++i) {
             if (ctx->stage[i]) HIP_CHECK(hipHostFree(ctx->stage[i]));
@@ -462,6 +474,28 @@ int rstsr_hip_synchronize(rstsr_hip_ctx_t* ctx) {
     return guarded([&] {
         bind(ctx);
         HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+int rstsr_hip_stream_wait(rstsr_hip_ctx_t* ctx, rstsr_hip_ctx_t* other) {
+    return guarded([&] {
+        if (!ctx || !other) RSTSR_RAISE(InvalidValue, "null context");
+        if (ctx == other || ctx->stream == other->stream) return;
+        if (stream_capturing(ctx)) {
+            // an event recorded outside the capture cannot join the graph: wait on the host
+            HIP_CHECK(hipSetDevice(other->device));
+            HIP_CHECK(hipStreamSynchronize(other->stream));
+            bind(ctx);
+            return;
+        }
+        {
+            std::lock_guard<std::mutex> g(other->mu);
+            HIP_CHECK(hipSetDevice(other->device));
+            if (!other->order_ev) HIP_CHECK(hipEventCreateWithFlags(&other->order_ev, hipEventDisableTiming));
+            HIP_CHECK(hipEventRecord(other->order_ev, other->stream));
+            bind(ctx);
+            HIP_CHECK(hipStreamWaitEvent(ctx->stream, other->order_ev, 0));
+        }
     });
 }
 
@@ -657,24 +691,55 @@ int rstsr_hip_graph_begin(rstsr_hip_ctx_t* ctx) {
     });
 }
 
+// Buffers a capture leaves behind: caller buffers freed during it, workspaces replaced
+// during it, and the workspaces its kernels were handed (their addresses are baked into
+// the graph). The context forgets the workspaces; the next call starts fresh ones.
+static std::vector<void*> take_capture_buffers(rstsr_hip_ctx* ctx) {
+    std::lock_guard<std::mutex> g(ctx->mu);
+    std::vector<void*> held;
+    held.swap(ctx->deferred);
+    held.insert(held.end(), ctx->retired.begin(), ctx->retired.end());
+    ctx->retired.clear();
+    for (Workspace* w : {&ctx->ws, &ctx->aux, &ctx->zbuf}) {
+        if (!w->captured) continue;
+        if (w->ptr) held.push_back(w->ptr);
+        w->ptr = nullptr;
+        w->bytes = 0;
+        w->captured = false;
+    }
+    return held;
+}
+
+static void free_now(const std::vector<void*>& held) {
+    if (held.empty()) return;
+    HIP_CHECK(hipDeviceSynchronize());  // work queued before the capture may still use them
+    for (void* p : held) {
+        HIP_CHECK(hipFree(p));
+        std::lock_guard<std::mutex> lg(g_reg_mu);
+        blocks().erase(p);
+    }
+}
+
 int rstsr_hip_graph_end(rstsr_hip_ctx_t* ctx, void** graph_exec) {
     return guarded([&] {
         bind(ctx);
         if (!graph_exec) RSTSR_RAISE(InvalidValue, "null output");
+        *graph_exec = nullptr;
         hipGraph_t g = nullptr;
-        HIP_CHECK(hipStreamEndCapture(ctx->stream, &g));
+        const hipError_t ec = hipStreamEndCapture(ctx->stream, &g);
+        std::vector<void*> held = take_capture_buffers(ctx);
+        if (ec != hipSuccess) {
+            (void)hipGetLastError();
+            if (g) (void)hipGraphDestroy(g);
+            free_now(held);  // no graph uses them
+            throw Error(ErrorKind::DeviceError, std::string("hipStreamEndCapture: ") + hipGetErrorString(ec));
+        }
         hipGraphExec_t ex = nullptr;
         hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
         (void)hipGraphDestroy(g);
-        std::vector<void*> held;
-        held.swap(ctx->deferred);  // buffers freed during the capture
         if (e != hipSuccess) {
-            // no graph uses them
-            for (void* p : held) {
-                HIP_CHECK(hipFree(p));
-                std::lock_guard<std::mutex> lg(g_reg_mu);
-                blocks().erase(p);
-            }
+            (void)hipGetLastError();
+            free_now(held);
             throw Error(ErrorKind::DeviceError, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
         }
         if (!held.empty()) {

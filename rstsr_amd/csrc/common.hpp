@@ -57,6 +57,7 @@ inline int guarded(F&& body) {
 struct Workspace {
     void* ptr = nullptr;
     size_t bytes = 0;
+    bool captured = false;  // handed to a kernel while the stream was being captured
 };
 
 }  // namespace hip
@@ -71,8 +72,9 @@ struct rstsr_hip_ctx {
     rstsr::hip::Workspace ws;   // partials, residue planes, scalar results
     rstsr::hip::Workspace aux;  // small metadata (grouped GEMM descriptors), BLAS driver scratch
     rstsr::hip::Workspace zbuf;  // temporaries of the three-product complex split GEMM
-    std::vector<void*> retired;  // buffers replaced while a stream was being captured
+    std::vector<void*> retired;  // workspaces replaced while the stream was being captured
     std::vector<void*> deferred;  // caller buffers freed while the stream was being captured
+    hipEvent_t order_ev = nullptr;  // recorded on this stream when another context must wait for it
     void* host_pinned = nullptr;  // 4 KiB pinned staging for scalar results
     hipEvent_t events[16] = {};
     // two pinned chunks for host transfers of pageable memory (allocated on first use)
@@ -101,8 +103,10 @@ inline void bind(rstsr_hip_ctx* ctx) {
 
 // Return a device buffer of at least `bytes` owned by the context. Growth is
 // ordered after all work already enqueued on the context stream; during stream
-// capture the old buffer is retired (freed with the context) instead, so a
-// captured graph keeps valid addresses.
+// capture the old buffer is retired instead of freed. A workspace handed out
+// during a capture belongs to the graph from rstsr_hip_graph_end on (the graph's
+// kernels keep its address; it is freed with the graph), and the context starts
+// a fresh one, so later eager growth never frees memory a replay writes.
 void* workspace(rstsr_hip_ctx* ctx, size_t bytes);
 void* aux_buffer(rstsr_hip_ctx* ctx, size_t bytes);
 void* zbuf_buffer(rstsr_hip_ctx* ctx, size_t bytes);
@@ -122,7 +126,8 @@ void meta_upload(rstsr_hip_ctx* ctx, void* dst, size_t bytes);
 // Autotuner: run(c) performs the whole call with candidate c (all candidates give
 // identical results). The first call for `key` with can_time runs the heuristic
 // once untimed (operand warm-up), then times every candidate twice with events on
-// the context stream and keeps the fastest; later calls use the cached choice. No
+// the context stream and keeps the fastest, which then runs once more, so that
+// the tuning call's output is the chosen candidate's; later calls use the cached choice. No
 // timing while the stream is captured, when can_time is false (the output is
 // also an input) or under RSTSR_HIP_NO_TUNE: then the cached choice or the
 // heuristic runs. RSTSR_HIP_TUNE_CACHE=<file> loads and appends choices.

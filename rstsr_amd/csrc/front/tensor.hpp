@@ -76,6 +76,13 @@ class DeviceHIP {
     FlagOrder default_order() const { return order; }
     void set_default_order(FlagOrder o) { order = o; }
     void synchronize() const { check(rstsr_hip_synchronize(raw())); }
+    // Work this device queues next runs after what `o` has queued (same GPU, other
+    // stream): an operand another context just wrote is read whole, and one it still
+    // reads is not overwritten early.
+    void order_after(const DeviceHIP& o) const {
+        if (!same_device(o)) RSTSR_RAISE(DeviceMismatch, "operands live on different devices");
+        if (o.raw() != raw()) check(rstsr_hip_stream_wait(raw(), o.raw()));
+    }
 };
 
 struct Storage {
@@ -147,6 +154,7 @@ class Tensor {
 
     /* ---------- assignment ---------- */
     void assign(const Tensor& a) {
+        device().order_after(a.device());
         Layout la = broadcast_layout_to_shape(a.layout, layout.shape, device().default_order());
         AbiLayout x(layout), y(la);
         check(rstsr_hip_assign(device().raw(), dtype, ptr(), &x, a.dtype, a.ptr(), &y));
@@ -177,8 +185,8 @@ class Tensor {
 
     /* ---------- elementwise ---------- */
     Tensor binary(rstsr_binop_t op, const Tensor& b) const {
-        if (!device().same_device(b.device())) RSTSR_RAISE(DeviceMismatch, "operands live on different devices");
         if (dtype != b.dtype) RSTSR_RAISE(InvalidValue, "operands must share one dtype");
+        device().order_after(b.device());
         FlagOrder ord = device().default_order();
         auto bl = broadcast_layout(layout, b.layout, ord);
         Layout lc = get_layout_for_binary_op(bl.first, bl.second, ord);
@@ -199,6 +207,7 @@ class Tensor {
     }
     void binary_inplace(rstsr_binop_t op, const Tensor& b) {
         if (dtype != b.dtype) RSTSR_RAISE(InvalidValue, "operands must share one dtype");
+        device().order_after(b.device());
         Layout lb = broadcast_layout_to_shape(b.layout, layout.shape, device().default_order());
         AbiLayout xc(layout), xb(lb);
         check(rstsr_hip_op_muta_refb(device().raw(), op, dtype, ptr(), &xc, b.ptr(), &xb));
@@ -221,7 +230,7 @@ class Tensor {
     }
     // op_muta_refb into a caller-given output view (any layout of the same shape)
     void unary_to(rstsr_unop_t op, Tensor& out) const {
-        if (!device().same_device(out.device())) RSTSR_RAISE(DeviceMismatch, "operands live on different devices");
+        device().order_after(out.device());
         AbiLayout xc(out.layout), xa(layout);
         check(rstsr_hip_unary(device().raw(), op, out.dtype, out.ptr(), &xc, dtype, ptr(), &xa));
     }
@@ -272,6 +281,7 @@ class Tensor {
     }
     bool allclose(const Tensor& b, double rtol, double atol, bool equal_nan) const {
         if (dtype != b.dtype) RSTSR_RAISE(InvalidValue, "operands must share one dtype");
+        device().order_after(b.device());
         AbiLayout xa(layout), xb(b.layout);
         int ok = 0;
         check(rstsr_hip_allclose(device().raw(), dtype, ptr(), &xa, b.ptr(), &xb, rtol, atol, equal_nan ? 1 : 0, &ok));
@@ -280,8 +290,8 @@ class Tensor {
 
     /* ---------- matmul ---------- */
     Tensor matmul(const Tensor& b, rstsr_gemm_mode_t mode) const {
-        if (!device().same_device(b.device())) RSTSR_RAISE(DeviceMismatch, "operands live on different devices");
         if (dtype != b.dtype) RSTSR_RAISE(InvalidValue, "operands must share one dtype");
+        device().order_after(b.device());
         Layout lc = matmul_output_layout(layout, b.layout, FlagOrder::RowMajor);
         Tensor c = uninit(device(), lc, dtype);
         AbiLayout xc(lc), xa(layout), xb(b.layout);

@@ -267,3 +267,58 @@ def test_moment_empty_values(dev):
     assert np.all(np.isnan(e.mean_axes([1]).to_numpy()))
     assert np.isnan(e.std_all())
     assert e.prod_all() == 1.0
+
+
+# A broadcast (stride-0) axis among the reduced ones is summed as repeated additions of the same
+# element (cpu_rayon/reduction.rs:189-193); max/argmax see ties along it (first index wins).
+BCAST = [
+    ((5, 6, 7), (7, 0, 1), [1]),
+    ((5, 6, 7), (7, 0, 1), [0, 1]),
+    ((5, 6, 7), (7, 0, 1), [1, 2]),
+    ((5, 6, 7), (7, 0, 1), [0, 1, 2]),
+    ((6, 7, 5), (0, 1, 7), [0]),           # stride-0 reduced axis first, kept unit axis
+    ((6, 7, 5), (0, 1, 7), [2, 0]),
+    ((4096, 35), (0, 1), [0]),             # long stride-0 reduced axis (column kernel, split)
+    ((35, 4096), (1, 0), [1]),             # long stride-0 reduced axis, the kept axis strided
+]
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32, np.int32])
+@pytest.mark.parametrize("op", [SUM, MAX, MIN, ARGMAX, ARGMIN])
+@pytest.mark.parametrize("ci", range(len(BCAST)))
+def test_reduce_axes_over_broadcast_axis(dev, oracle, dtype, op, ci):
+    from gpu_util import pair
+
+    shape, stride, axes = BCAST[ci]
+    rng = np.random.default_rng(1000 + ci * 17 + op)
+    n = 35
+    base = rand(rng, n, dtype)
+    tx, vx = pair(dev, base, shape, stride)
+    got = getattr(tx, METH[op])(axes).to_numpy()
+    ref = oracle_reduce(oracle, op, vx, axes, dtype)
+    assert got.shape == ref.shape
+    if op == SUM and np.dtype(dtype).kind == "f":
+        check_sum(got, ref, float(np.abs(vx.array()).sum()), dtype)
+        # repeated additions of one value: the count times the value, rounded once
+        if axes == [1] and shape == (5, 6, 7):
+            exact = 6 * base.reshape(5, 7).astype(np.float64)
+            assert np.all(np.abs(got - exact) <= 2 * np.finfo(dtype).eps * np.abs(exact))
+    else:
+        assert bits_equal(got, ref)
+    if op in (ARGMAX, ARGMIN) and axes == [1] and shape == (5, 6, 7):
+        assert np.all(got == 0)  # every position along the broadcast axis ties: the first wins
+
+
+@pytest.mark.parametrize("op", [SUM, MAX, ARGMAX])
+def test_reduce_all_over_broadcast_view(dev, oracle, op):
+    from gpu_util import pair
+
+    rng = np.random.default_rng(31 + op)
+    base = rng.normal(size=9)
+    tx, vx = pair(dev, base, (4, 9, 1000), (0, 1, 0))
+    got = getattr(tx, ALL[op])()
+    ref = oracle.reduce_all(op, vx)
+    if op == SUM:
+        assert abs(got - ref) <= 2 * 2.0 ** -52 * abs(ref) + 1e-30 * np.abs(base).sum() * 4000
+    else:
+        assert got == ref

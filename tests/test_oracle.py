@@ -194,6 +194,82 @@ def test_complex_mul_matches_python(oracle):
         assert g.real == re and g.imag == im
 
 
+@pytest.mark.parametrize("op", [ADD, SUB])
+def test_complex_add_sub_componentwise_ieee(oracle, op):
+    """num-complex add/sub are componentwise IEEE operations: c128 against Python's complex
+    arithmetic, c64 against NumPy's componentwise float32 add/sub, bit for bit, on strided views
+    with a broadcast operand and signed zeros, infinities and NaN."""
+    rng = np.random.default_rng(40 + op)
+    re = rng.normal(size=(7, 9))
+    im = rng.normal(size=(7, 9))
+    re[0, :4] = [0.0, -0.0, np.inf, -np.inf]
+    im[1, :4] = [-0.0, 0.0, np.nan, np.inf]
+    z = np.empty(re.shape, np.complex128)
+    z.real, z.imag = re, im                         # no 0 * inf products
+    a = z.T[::2]                                    # transposed, stepped
+    b = (rng.normal(size=7) - 1j * rng.normal(size=7))[None, :]  # broadcast row
+    got = o_binary(oracle, op, a, b)
+    f = (lambda x, y: x + y) if op == ADD else (lambda x, y: x - y)
+    A, B = np.broadcast_arrays(a, b)
+    ref = np.array([[f(complex(x), complex(y)) for x, y in zip(ra, rb)] for ra, rb in zip(A, B)])
+    assert np.array_equal(got.view(np.uint64), ref.view(np.uint64))
+    a32, b32 = a.astype(np.complex64), b.astype(np.complex64)
+    got32 = o_binary(oracle, op, a32, b32)
+    A32, B32 = np.broadcast_arrays(a32, b32)
+    g = np.add if op == ADD else np.subtract
+    ref32 = np.empty(A32.shape, np.complex64)
+    ref32.real = g(A32.real, B32.real)
+    ref32.imag = g(A32.imag, B32.imag)
+    assert np.array_equal(got32.view(np.uint32), ref32.view(np.uint32))
+
+
+def test_complex_mul_python_builtin(oracle):
+    """c128 multiply against CPython's own complex product (an independent C implementation of
+    (ac - bd, ad + bc) with no fused multiply-add), bit for bit."""
+    rng = np.random.default_rng(43)
+    a = (rng.normal(size=500) + 1j * rng.normal(size=500)) * np.exp2(rng.integers(-30, 30, size=500))
+    b = (rng.normal(size=500) + 1j * rng.normal(size=500)) * np.exp2(rng.integers(-30, 30, size=500))
+    got = o_binary(oracle, MUL, a, b)
+    ref = np.array([complex(x) * complex(y) for x, y in zip(a, b)])
+    assert np.array_equal(got.view(np.uint64), ref.view(np.uint64))
+
+
+@pytest.mark.parametrize("dtype", [np.complex128, np.complex64])
+def test_complex_mul_exact_identities(oracle, dtype):
+    """Products the mathematics fixes exactly: Gaussian integers (exact in both precisions, checked
+    against Python integers), multiplication by i, by a real number, and z * conj(z) = |z|^2 + 0i
+    (imaginary part a(-b) + ba = 0 exactly when nothing is fused)."""
+    rng = np.random.default_rng(44)
+    p = rng.integers(-2000, 2000, size=(4, 200))
+    a = (p[0] + 1j * p[1]).astype(dtype)
+    b = (p[2] + 1j * p[3]).astype(dtype)
+    got = o_binary(oracle, MUL, a, b)
+    assert np.array_equal(got.real, (p[0] * p[2] - p[1] * p[3]).astype(got.real.dtype))
+    assert np.array_equal(got.imag, (p[0] * p[3] + p[1] * p[2]).astype(got.real.dtype))
+    z = (rng.normal(size=300) + 1j * rng.normal(size=300)).astype(dtype)
+    gi = o_binary(oracle, MUL, z, np.full_like(z, 1j))
+    assert np.array_equal(gi.real, -z.imag) and np.array_equal(gi.imag, z.real)
+    r = rng.normal(size=300).astype(z.real.dtype)
+    gr = o_binary(oracle, MUL, z, r.astype(dtype))
+    assert np.array_equal(gr.real, z.real * r) and np.array_equal(gr.imag, z.imag * r)
+    gc = o_binary(oracle, MUL, z, np.conj(z))
+    assert np.all(gc.imag == 0)
+    rt_ = z.real.dtype.type
+    assert np.array_equal(gc.real, np.array([rt_(x.real) * rt_(x.real) + rt_(x.imag) * rt_(x.imag) for x in z]))
+
+
+def test_complex_mul_pin_bites(oracle):
+    """The identities above fail for the plausible mistakes: a conjugated operand, swapped
+    components, or the sign of bd flipped."""
+    rng = np.random.default_rng(45)
+    p = rng.integers(-50, 50, size=(4, 64))
+    a = p[0] + 1j * p[1]
+    b = p[2] + 1j * p[3]
+    good = o_binary(oracle, MUL, a, b)
+    for wrong in (a * np.conj(b), (a * b).imag + 1j * (a * b).real, (p[0] * p[2] + p[1] * p[3]) + 1j * (a * b).imag):
+        assert not np.array_equal(good, wrong)
+
+
 def test_assign_matches_numpy_copy(oracle):
     rng = np.random.default_rng(11)
     base = rng.normal(size=4 * 5 * 6)
