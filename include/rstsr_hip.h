@@ -223,6 +223,17 @@ int rstsr_hip_reduce_out_layout(const rstsr_layout_t* la, size_t naxes, const pt
 int rstsr_hip_reduce_axes(rstsr_hip_ctx_t* ctx, rstsr_redop_t op, rstsr_dtype_t t, const void* a,
                           const rstsr_layout_t* la, size_t naxes, const ptrdiff_t* axes, void* out,
                           const rstsr_layout_t* lo);
+/* No counterpart in the reference: rstsr_hip_reduce_axes (same arguments and results) for
+ * callers that cannot allocate device memory (code under hipGraph stream capture, tight
+ * serving loops, harnesses that forbid hipMalloc). It calls no hipMalloc/hipFree: the
+ * partials of split reductions live in a fixed 8 MiB device array per GPU, resolved when
+ * the first context on that GPU is created; a plan needing more fails with
+ * RSTSR_ERR_VALUE_OUT_OF_RANGE before anything is enqueued. Every call on one GPU shares
+ * that array, so calls that overlap on one GPU (other threads, other contexts' streams)
+ * race; calls ordered on one stream do not. */
+int rstsr_hip_reduce_axes_noalloc(rstsr_hip_ctx_t* ctx, rstsr_redop_t op, rstsr_dtype_t t, const void* a,
+                                  const rstsr_layout_t* la, size_t naxes, const ptrdiff_t* axes, void* out,
+                                  const rstsr_layout_t* lo);
 /* argmax_axes / argmin_axes: u64 indices raveled row-major over the reduced axes in the given
  * order (cpu_serial/reduction.rs:553-580), into `out` laid out as `lo`. */
 int rstsr_hip_argreduce_axes(rstsr_hip_ctx_t* ctx, rstsr_redop_t op, rstsr_dtype_t t, const void* a,

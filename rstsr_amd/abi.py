@@ -52,6 +52,7 @@ def lib():
         "rstsr_hip_assign": [vp, C.c_int, vp, P, C.c_int, vp, P],
         "rstsr_hip_assign_arbitary": [vp, C.c_int, vp, P, C.c_int, vp, P],
         "rstsr_hip_reduce_axes": [vp, C.c_int, C.c_int, vp, P, C.c_size_t, C.POINTER(C.c_ssize_t), vp, P],
+        "rstsr_hip_reduce_axes_noalloc": [vp, C.c_int, C.c_int, vp, P, C.c_size_t, C.POINTER(C.c_ssize_t), vp, P],
         "rstsr_hip_argreduce_axes": [vp, C.c_int, C.c_int, vp, P, C.c_size_t, C.POINTER(C.c_ssize_t), vp, P],
         "rstsr_hip_reduce_all": [vp, C.c_int, C.c_int, vp, P, vp],
         "rstsr_hip_unary": [vp, C.c_int, C.c_int, vp, P, C.c_int, vp, P],

@@ -413,6 +413,7 @@ int rstsr_hip_ctx_create(int device_id, rstsr_hip_ctx_t** out) {
             hipDeviceProp_t prop;
             HIP_CHECK(hipGetDeviceProperties(&prop, device_id));
             ctx->num_cu = prop.multiProcessorCount;
+            reduce_scratch_setup(device_id);
             ctx->l2_bytes = prop.l2CacheSize;
             HIP_CHECK(hipHostMalloc(&ctx->host_pinned, 4096, hipHostMallocDefault));
             for (auto& ev : ctx->events) HIP_CHECK(hipEventCreate(&ev));

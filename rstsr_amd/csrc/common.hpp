@@ -111,6 +111,10 @@ void* workspace(rstsr_hip_ctx* ctx, size_t bytes);
 void* aux_buffer(rstsr_hip_ctx* ctx, size_t bytes);
 void* zbuf_buffer(rstsr_hip_ctx* ctx, size_t bytes);
 
+// Resolve the calling device's fixed reduction scratch (reduce.hip) once per device; run by
+// rstsr_hip_ctx_create with the context's device current.
+void reduce_scratch_setup(int device);
+
 // True while the context stream is being captured into a graph.
 bool stream_capturing(rstsr_hip_ctx* ctx);
 

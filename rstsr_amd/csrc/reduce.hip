@@ -780,12 +780,31 @@ static RedSetup reduce_plan(rstsr_hip_ctx* ctx, const Layout& la, const std::vec
     return s;
 }
 
+// Where a split reduction keeps its partials: the context workspace (grown on demand), or a
+// fixed caller-supplied device region of `cap` bytes (no allocation; a plan that needs more
+// is refused before anything is enqueued).
+struct PartBuf {
+    void* ptr = nullptr;
+    size_t cap = 0;
+};
+
 template <class A, class T>
-static void run_reduce(rstsr_hip_ctx* ctx, const RedSetup& s, const T* a, typename A::Out* out) {
+static void run_reduce(rstsr_hip_ctx* ctx, const RedSetup& s, const T* a, typename A::Out* out, PartBuf fixed) {
     const RedPlan& p = s.p;
     using S = typename A::S;
     S* part = nullptr;
-    if (p.nsplit > 1) part = (S*)workspace(ctx, (size_t)p.M * p.nsplit * sizeof(S));
+    if (p.nsplit > 1) {
+        const size_t need = (size_t)p.M * p.nsplit * sizeof(S);
+        if (fixed.ptr) {
+            if (need > fixed.cap)
+                RSTSR_RAISE(ValueOutOfRange, "reduction partials need " + std::to_string(need) +
+                                                 " bytes, more than the fixed scratch of " +
+                                                 std::to_string(fixed.cap) + " bytes");
+            part = (S*)fixed.ptr;
+        } else {
+            part = (S*)workspace(ctx, need);
+        }
+    }
     // 16-byte loads when every address a thread touches is 16-byte aligned
     constexpr int V16 = (std::is_arithmetic<T>::value && (sizeof(T) == 8 || sizeof(T) == 4)) ? (int)(16 / sizeof(T)) : 1;
     if (s.kind == RedKind::Row) {
@@ -837,11 +856,11 @@ __global__ void k_fill_value(T* out, RedPlan p, T v) {
 
 // Dispatch (op, dtype) to an accumulator and run it.
 static void reduce_dispatch(rstsr_hip_ctx* ctx, rstsr_redop_t op, rstsr_dtype_t t, const void* a, const RedSetup& s,
-                            void* out) {
+                            void* out, PartBuf fixed = PartBuf{}) {
     auto go = [&](auto acc_tag, auto val_tag) {
         using A = decltype(acc_tag);
         using T = decltype(val_tag);
-        run_reduce<A, T>(ctx, s, (const T*)a, (typename A::Out*)out);
+        run_reduce<A, T>(ctx, s, (const T*)a, (typename A::Out*)out, fixed);
     };
     switch (op) {
         case RSTSR_RED_SUM:
@@ -969,7 +988,8 @@ static void empty_value(rstsr_redop_t op, rstsr_dtype_t t, void* v) {
 }
 
 static void reduce_axes_impl(rstsr_hip_ctx* ctx, rstsr_redop_t op, rstsr_dtype_t t, const void* a, const Layout& la,
-                             const std::vector<ptrdiff_t>& axes, void* out, const Layout& lo) {
+                             const std::vector<ptrdiff_t>& axes, void* out, const Layout& lo,
+                             PartBuf fixed = PartBuf{}) {
     if (!empty_value_op(op) && la.size() == 0) {
         if (op == RSTSR_RED_MAX || op == RSTSR_RED_MIN)
             RSTSR_RAISE(InvalidValue, "zero-size array is not supported for max/min");
@@ -1028,7 +1048,41 @@ static void reduce_axes_impl(rstsr_hip_ctx* ctx, rstsr_redop_t op, rstsr_dtype_t
         }
         return;
     }
-    reduce_dispatch(ctx, op, t, a, s, out);
+    reduce_dispatch(ctx, op, t, a, s, out, fixed);
+}
+
+/* ======================= fixed scratch (no-allocation entry point) ======================= */
+
+// Partials of split reductions for rstsr_hip_reduce_axes_noalloc. Each GPU the library runs
+// on holds its own copy until the process ends. Its host-side name is not a device address:
+// host code uses only the per-device address hipGetSymbolAddress returns (scratch_addr).
+// On 256 CUs the largest plan needs 131072 partials of at most 48 bytes (6 MiB).
+constexpr size_t kNoallocScratchBytes = 8u << 20;
+static __device__ __align__(256) unsigned char scratch[kNoallocScratchBytes];
+
+static std::mutex g_scratch_mu;
+static std::map<int, void*>& scratch_table() {  // device -> address of its scratch; g_scratch_mu
+    static std::map<int, void*> t;
+    return t;
+}
+
+// Resolve this device's scratch address once per device, with that device current.
+void reduce_scratch_setup(int device) {
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    if (scratch_table().count(device)) return;
+    HIP_CHECK(hipSetDevice(device));
+    void* ptr = nullptr;
+    hipError_t e = hipGetSymbolAddress((void**)&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+        RSTSR_RAISE(DeviceError, std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e));
+    scratch_table()[device] = ptr;
+}
+
+static void* scratch_addr(int device) {
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    auto it = scratch_table().find(device);
+    if (it == scratch_table().end()) RSTSR_RAISE(DeviceError, "reduction scratch was not set up for this device");
+    return it->second;
 }
 
 }  // namespace hip
@@ -1058,6 +1112,23 @@ int rstsr_hip_reduce_axes(rstsr_hip_ctx_t* ctx, rstsr_redop_t op, rstsr_dtype_t 
         check_extent(out, lo, out_size_of(op, t), "out");
         std::lock_guard<std::mutex> g(ctx->mu);
         reduce_axes_impl(ctx, op, t, a, la, ax, out, lo);
+    });
+}
+
+int rstsr_hip_reduce_axes_noalloc(rstsr_hip_ctx_t* ctx, rstsr_redop_t op, rstsr_dtype_t t, const void* a,
+                                  const rstsr_layout_t* la_, size_t naxes, const ptrdiff_t* axes, void* out,
+                                  const rstsr_layout_t* lo_) {
+    return guarded([&] {
+        bind(ctx);
+        check_op(op, false);
+        Layout la = from_abi(la_), lo = from_abi(lo_);
+        if (naxes && !axes) RSTSR_RAISE(InvalidValue, "null axes");
+        std::vector<ptrdiff_t> ax(axes, axes + naxes);
+        check_extent(a, la, dtype_size(t), "a");
+        check_extent(out, lo, out_size_of(op, t), "out");
+        PartBuf fixed{scratch_addr(ctx->device), kNoallocScratchBytes};
+        std::lock_guard<std::mutex> g(ctx->mu);
+        reduce_axes_impl(ctx, op, t, a, la, ax, out, lo, fixed);
     });
 }
 
