@@ -49,8 +49,27 @@ typedef enum {
     RSTSR_BOOL = 7
 } rstsr_dtype_t;
 
-/* Binary arithmetic (rstsr-core/src/operators/ops/op_ternary_arithmetic.rs:3-56). */
-typedef enum { RSTSR_OP_ADD = 0, RSTSR_OP_SUB = 1, RSTSR_OP_MUL = 2, RSTSR_OP_DIV = 3 } rstsr_binop_t;
+/* Binary operators (rstsr-core/src/operators/ops/op_ternary_arithmetic.rs:3-56; the device impls
+ * of feature_rayon/auto_impl/op_ternary_arithmetic.rs:3-14), operands of one dtype:
+ *  - ADD, SUB, MUL, DIV, REM: f32, f64, i32, i64, c64, c128. Integers wrap; integer / and %
+ *    truncate toward zero; float % is fmod; complex / and % are num-complex 0.4's formulas.
+ *    Rust panics on an integer zero divisor and on MIN / -1 (MIN % -1); the device gives 0,
+ *    the wrapped MIN and 0 for those instead.
+ *  - BITAND, BITOR, BITXOR: i32, i64, bool.  SHL, SHR: i32, i64, the shift amount taken
+ *    modulo the bit width (Rust release semantics), SHR arithmetic.
+ * Other (operator, dtype) pairs have no impl in the reference and fail with UnImplemented. */
+typedef enum {
+    RSTSR_OP_ADD = 0,
+    RSTSR_OP_SUB = 1,
+    RSTSR_OP_MUL = 2,
+    RSTSR_OP_DIV = 3,
+    RSTSR_OP_REM = 4,
+    RSTSR_OP_BITAND = 5,
+    RSTSR_OP_BITOR = 6,
+    RSTSR_OP_BITXOR = 7,
+    RSTSR_OP_SHL = 8,
+    RSTSR_OP_SHR = 9
+} rstsr_binop_t;
 
 /* Reductions (rstsr-core/src/operators/reduction.rs:3-95). */
 typedef enum {
@@ -65,7 +84,13 @@ typedef enum {
     RSTSR_RED_MEAN = 6,
     RSTSR_RED_VAR = 7,
     RSTSR_RED_STD = 8,
-    RSTSR_RED_L2NORM = 9
+    RSTSR_RED_L2NORM = 9,
+    /* all / any of a bool array -> bool (auto_impl/reduction.rs:466-519); count_nonzero of any
+     * dtype -> usize (:521-551). RSTSR_RED_SUM of a bool array is OpSumBoolAPI (:678-715): the
+     * number of true elements as a usize. */
+    RSTSR_RED_ALL = 10,
+    RSTSR_RED_ANY = 11,
+    RSTSR_RED_COUNT_NONZERO = 12
 } rstsr_redop_t;
 
 /* Unary elementwise operations: the 35 traits of rstsr-core/src/operators/ops/op_binary_common.rs:3-39
@@ -219,7 +244,9 @@ int rstsr_hip_argreduce_all(rstsr_hip_ctx_t* ctx, rstsr_redop_t op, rstsr_dtype_
  * layout_for_array_copy(rest, K)); shape_out/stride_out need room for la->ndim entries. */
 int rstsr_hip_reduce_out_layout(const rstsr_layout_t* la, size_t naxes, const ptrdiff_t* axes, size_t* ndim_out,
                                 size_t* shape_out, ptrdiff_t* stride_out);
-/* sum_axes / max_axes / min_axes into caller-allocated `out` laid out as `lo` (same dtype t). */
+/* sum_axes / max_axes / min_axes / ... into caller-allocated `out` laid out as `lo`; the output
+ * element is T, except T::Real for var/std/l2_norm of complex, bool for all/any and u64 (usize)
+ * for count_nonzero and the sum of a bool array. */
 int rstsr_hip_reduce_axes(rstsr_hip_ctx_t* ctx, rstsr_redop_t op, rstsr_dtype_t t, const void* a,
                           const rstsr_layout_t* la, size_t naxes, const ptrdiff_t* axes, void* out,
                           const rstsr_layout_t* lo);
@@ -261,7 +288,7 @@ int rstsr_hip_unary_inplace(rstsr_hip_ctx_t* ctx, rstsr_unop_t op, rstsr_dtype_t
 /* ---- matmul (DeviceMatMulAPI, operators/matmul.rs:7-25; device_faer/matmul.rs:204-239) ---- */
 /* c = alpha * a % b + beta * c with rstsr's row-major matmul rules 1-7 (layout/matmul.rs:9-29).
  * beta == 0 never reads c. alpha/beta point to host scalars of dtype t. mode selects the arithmetic
- * for f64/c128; err_out (nullable) receives nothing and is reserved for per-call error reports. */
+ * for f64/c128 (other dtypes ignore it). */
 int rstsr_hip_matmul(rstsr_hip_ctx_t* ctx, rstsr_dtype_t t, void* c, const rstsr_layout_t* lc, const void* a,
                      const rstsr_layout_t* la, const void* b, const rstsr_layout_t* lb, const void* alpha,
                      const void* beta, rstsr_gemm_mode_t mode);

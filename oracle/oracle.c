@@ -38,9 +38,12 @@ typedef struct {
     size_t offset;
 } olayout;
 
-enum { F32 = 0, F64 = 1, I32 = 2, I64 = 3, C64 = 4, C128 = 5, U64 = 6 };
-enum { OP_ADD = 0, OP_SUB = 1, OP_MUL = 2, OP_DIV = 3 };
-enum { R_SUM = 0, R_MAX = 1, R_MIN = 2, R_ARGMAX = 3, R_ARGMIN = 4, R_PROD = 5, R_MEAN = 6, R_VAR = 7, R_STD = 8, R_L2 = 9 };
+enum { F32 = 0, F64 = 1, I32 = 2, I64 = 3, C64 = 4, C128 = 5, U64 = 6, BOOL = 7 };
+/* operators of auto_impl/op_ternary_arithmetic.rs:3-14: Add Sub Mul Div Rem BitAnd BitOr BitXor Shl Shr */
+enum { OP_ADD = 0, OP_SUB = 1, OP_MUL = 2, OP_DIV = 3, OP_REM = 4, OP_AND = 5, OP_OR = 6, OP_XOR = 7, OP_SHL = 8,
+       OP_SHR = 9 };
+enum { R_SUM = 0, R_MAX = 1, R_MIN = 2, R_ARGMAX = 3, R_ARGMIN = 4, R_PROD = 5, R_MEAN = 6, R_VAR = 7, R_STD = 8, R_L2 = 9,
+       R_ALL = 10, R_ANY = 11, R_COUNT = 12 };
 
 typedef struct { float re, im; } c64_t;
 typedef struct { double re, im; } c128_t;
@@ -94,38 +97,95 @@ static c64_t cmul64(c64_t a, c64_t b) {
     return r;
 }
 
-/* apply op on element pointers; integers wrap (Rust release builds) */
+/* num-complex 0.4 Div: ((a.re c + a.im d) / n, (a.im c - a.re d) / n), n = c*c + d*d; and Rem:
+ * q = a / b, q0 = (q.re - q.re % 1, q.im - q.im % 1) (q truncated toward zero), a - b * q0. */
+static c128_t cdiv128(c128_t a, c128_t b) {
+    double n = b.re * b.re + b.im * b.im;
+    c128_t r;
+    r.re = (a.re * b.re + a.im * b.im) / n;
+    r.im = (a.im * b.re - a.re * b.im) / n;
+    return r;
+}
+static c64_t cdiv64(c64_t a, c64_t b) {
+    float n = b.re * b.re + b.im * b.im;
+    c64_t r;
+    r.re = (a.re * b.re + a.im * b.im) / n;
+    r.im = (a.im * b.re - a.re * b.im) / n;
+    return r;
+}
+static c128_t crem128(c128_t a, c128_t b) {
+    c128_t q = cdiv128(a, b), q0, p, r;
+    q0.re = q.re - fmod(q.re, 1.0);
+    q0.im = q.im - fmod(q.im, 1.0);
+    p = cmul128(b, q0);
+    r.re = a.re - p.re;
+    r.im = a.im - p.im;
+    return r;
+}
+static c64_t crem64(c64_t a, c64_t b) {
+    c64_t q = cdiv64(a, b), q0, p, r;
+    q0.re = q.re - fmodf(q.re, 1.0f);
+    q0.im = q.im - fmodf(q.im, 1.0f);
+    p = cmul64(b, q0);
+    r.re = a.re - p.re;
+    r.im = a.im - p.im;
+    return r;
+}
+
+/* Integer ops as Rust release builds compute them: add/sub/mul wrap; / and % truncate toward
+ * zero; << and >> take the shift amount modulo the bit width, >> is arithmetic. Rust panics on
+ * a zero divisor and on MIN / -1 (MIN % -1); the values given here for those cases (0, and the
+ * wrapped MIN / -1 = MIN, MIN % -1 = 0) are the device's documented substitutes. */
+#define INT_OP(T, U, BITS)                                                              \
+    static T int_op_##T(int op, T x, T y) {                                             \
+        U ux = (U)x, uy = (U)y;                                                         \
+        switch (op) {                                                                   \
+            case OP_ADD: return (T)(ux + uy);                                           \
+            case OP_SUB: return (T)(ux - uy);                                           \
+            case OP_MUL: return (T)(ux * uy);                                           \
+            case OP_DIV: return y == 0 ? (T)0 : y == -1 ? (T)((U)0 - ux) : (T)(x / y);  \
+            case OP_REM: return (y == 0 || y == -1) ? (T)0 : (T)(x % y);                \
+            case OP_AND: return (T)(ux & uy);                                           \
+            case OP_OR: return (T)(ux | uy);                                            \
+            case OP_XOR: return (T)(ux ^ uy);                                           \
+            case OP_SHL: return (T)(ux << (uy & (BITS - 1)));                           \
+            default: return (T)(x >> (uy & (BITS - 1)));                                \
+        }                                                                               \
+    }
+INT_OP(int32_t, uint32_t, 32)
+INT_OP(int64_t, uint64_t, 64)
+
+/* apply op on element pointers (operands of one dtype; every combination the reference's
+ * types implement: arithmetic for numbers, Rem with fmod for floats, bit ops for integers and
+ * bool, shifts for integers) */
 static void apply(int op, int t, void* c, const void* a, const void* b) {
     switch (t) {
         case F32: {
             float x = *(const float*)a, y = *(const float*)b, r;
-            r = op == OP_ADD ? x + y : op == OP_SUB ? x - y : op == OP_MUL ? x * y : x / y;
+            r = op == OP_ADD ? x + y : op == OP_SUB ? x - y : op == OP_MUL ? x * y : op == OP_DIV ? x / y : fmodf(x, y);
             *(float*)c = r;
             break;
         }
         case F64: {
             double x = *(const double*)a, y = *(const double*)b, r;
-            r = op == OP_ADD ? x + y : op == OP_SUB ? x - y : op == OP_MUL ? x * y : x / y;
+            r = op == OP_ADD ? x + y : op == OP_SUB ? x - y : op == OP_MUL ? x * y : op == OP_DIV ? x / y : fmod(x, y);
             *(double*)c = r;
             break;
         }
-        case I32: {
-            uint32_t x = (uint32_t) * (const int32_t*)a, y = (uint32_t) * (const int32_t*)b, r;
-            r = op == OP_ADD ? x + y : op == OP_SUB ? x - y : x * y;
-            *(int32_t*)c = (int32_t)r;
-            break;
-        }
-        case I64: {
-            uint64_t x = (uint64_t) * (const int64_t*)a, y = (uint64_t) * (const int64_t*)b, r;
-            r = op == OP_ADD ? x + y : op == OP_SUB ? x - y : x * y;
-            *(int64_t*)c = (int64_t)r;
+        case I32: *(int32_t*)c = int_op_int32_t(op, *(const int32_t*)a, *(const int32_t*)b); break;
+        case I64: *(int64_t*)c = int_op_int64_t(op, *(const int64_t*)a, *(const int64_t*)b); break;
+        case BOOL: {
+            uint8_t x = *(const uint8_t*)a != 0, y = *(const uint8_t*)b != 0;
+            *(uint8_t*)c = op == OP_AND ? (x & y) : op == OP_OR ? (x | y) : (x ^ y);
             break;
         }
         case C64: {
             c64_t x = *(const c64_t*)a, y = *(const c64_t*)b, r;
             if (op == OP_MUL) r = cmul64(x, y);
             else if (op == OP_ADD) { r.re = x.re + y.re; r.im = x.im + y.im; }
-            else { r.re = x.re - y.re; r.im = x.im - y.im; }
+            else if (op == OP_SUB) { r.re = x.re - y.re; r.im = x.im - y.im; }
+            else if (op == OP_DIV) r = cdiv64(x, y);
+            else r = crem64(x, y);
             *(c64_t*)c = r;
             break;
         }
@@ -133,7 +193,9 @@ static void apply(int op, int t, void* c, const void* a, const void* b) {
             c128_t x = *(const c128_t*)a, y = *(const c128_t*)b, r;
             if (op == OP_MUL) r = cmul128(x, y);
             else if (op == OP_ADD) { r.re = x.re + y.re; r.im = x.im + y.im; }
-            else { r.re = x.re - y.re; r.im = x.im - y.im; }
+            else if (op == OP_SUB) { r.re = x.re - y.re; r.im = x.im - y.im; }
+            else if (op == OP_DIV) r = cdiv128(x, y);
+            else r = crem128(x, y);
             *(c128_t*)c = r;
             break;
         }
@@ -149,6 +211,7 @@ static size_t esize(int t) {
         case C64: return 8;
         case C128: return 16;
         case U64: return 8;
+        case BOOL: return 1;
     }
     return 0;
 }
@@ -288,6 +351,39 @@ static void reduce_one(int op, int t, const void* a, const olayout* lr, void* ou
     size_t n = lsize(lr), es = esize(t);
     odo_t o;
     odo_init(&o, lr);
+    if (op == R_ALL || op == R_ANY) {
+        /* all / any (feature_rayon/auto_impl/reduction.rs:466-519): fold && from true, || from false */
+        int acc = op == R_ALL;
+        for (size_t i = 0; i < n; ++i) {
+            int x = *((const uint8_t*)a + o.off) != 0;
+            acc = op == R_ALL ? (acc && x) : (acc || x);
+            odo_next(&o, lr);
+        }
+        *(uint8_t*)out = (uint8_t)acc;
+        return;
+    }
+    if (op == R_COUNT || (op == R_SUM && t == BOOL)) {
+        /* count_nonzero (:521-551): elements != zero (complex: either part) into a usize; the sum
+         * of a bool array (OpSumBoolAPI :678-715) counts the true elements */
+        uint64_t cnt = 0;
+        for (size_t i = 0; i < n; ++i) {
+            const char* p = (const char*)a + o.off * (ptrdiff_t)es;
+            int nz;
+            switch (t) {
+                case F32: nz = *(const float*)p != 0.0f; break;
+                case F64: nz = *(const double*)p != 0.0; break;
+                case I32: nz = *(const int32_t*)p != 0; break;
+                case I64: nz = *(const int64_t*)p != 0; break;
+                case C64: nz = ((const float*)p)[0] != 0.0f || ((const float*)p)[1] != 0.0f; break;
+                case C128: nz = ((const double*)p)[0] != 0.0 || ((const double*)p)[1] != 0.0; break;
+                default: nz = *(const uint8_t*)p != 0; break;
+            }
+            cnt += (uint64_t)nz;
+            odo_next(&o, lr);
+        }
+        *(uint64_t*)out = cnt;
+        return;
+    }
     if (op == R_SUM) {
         if (t == F32 || t == F64) {
             dd_t acc = {0, 0};
@@ -424,6 +520,8 @@ int o_reduce_axes(int op, int t, const void* a, const olayout* la, size_t naxes,
     size_t m = lsize(&lk);
     size_t eo = (op == R_ARGMAX || op == R_ARGMIN) ? 8 : esize(t);
     if ((op == R_VAR || op == R_STD || op == R_L2) && (t == C64 || t == C128)) eo = esize(t) / 2;
+    if (op == R_ALL || op == R_ANY) eo = 1;
+    if (op == R_COUNT || (op == R_SUM && t == BOOL)) eo = 8;
     olayout lr = {nr, rs, rt, 0};
     if (lsize(&lr) == 0 && (op == R_MAX || op == R_MIN || op == R_ARGMAX || op == R_ARGMIN)) return 2;
     odo_t ok, oo;

@@ -42,9 +42,11 @@ _NP2CODE = {
 }
 _CODE2NP = {v: k for k, v in _NP2CODE.items()}
 
-OP_ADD, OP_SUB, OP_MUL, OP_DIV = 0, 1, 2, 3
+OP_ADD, OP_SUB, OP_MUL, OP_DIV, OP_REM = 0, 1, 2, 3, 4
+OP_BITAND, OP_BITOR, OP_BITXOR, OP_SHL, OP_SHR = 5, 6, 7, 8, 9
 RED_SUM, RED_MAX, RED_MIN, RED_ARGMAX, RED_ARGMIN = 0, 1, 2, 3, 4
 RED_PROD, RED_MEAN, RED_VAR, RED_STD, RED_L2NORM = 5, 6, 7, 8, 9
+RED_ALL, RED_ANY, RED_COUNT_NONZERO = 10, 11, 12
 _REAL_OF = {np.dtype(np.complex64): np.dtype(np.float32), np.dtype(np.complex128): np.dtype(np.float64)}
 GEMM_PLAIN, GEMM_OZAKI = 0, 1
 
@@ -103,12 +105,17 @@ def _axes(axes, ndim):
 
 
 class Tensor:
-    """A strided view of device storage (rstsr-core/src/tensorbase.rs:5-24)."""
+    """A strided view of device storage (rstsr-core/src/tensorbase.rs:5-24).
 
-    __slots__ = ("_t",)
+    ``is_owned()`` follows the reference's ownership kinds: results of computations and copies
+    own their storage; views (slices, transposes, reshapes and conversions that needed no copy)
+    borrow it (TensorView / TensorCow::Borrowed)."""
 
-    def __init__(self, t):
+    __slots__ = ("_t", "_own")
+
+    def __init__(self, t, own=True):
         self._t = t
+        self._own = own
 
     # ---------------- properties ----------------
     @property
@@ -183,11 +190,22 @@ class Tensor:
 
     # ---------------- views (layout only) ----------------
     def _view(self, layout):
-        return Tensor(self._t._with_layout(layout))
+        return Tensor(self._t._with_layout(layout), own=False)
+
+    def _order(self):
+        return self._t.device.default_order
 
     def transpose(self, axes=None):
+        """transpose / permute_dims (tensor/manipulation/transpose.rs): a view with permuted axes;
+        the axes must be a permutation of 0..ndim (negative values count from the end)."""
         nd = self.ndim
-        axes = list(range(nd))[::-1] if axes is None else [a % nd if a < 0 else a for a in axes]
+        if axes is None:
+            axes = list(range(nd))[::-1]
+        else:
+            axes = [int(a) for a in axes]
+            if any(a < -nd or a >= nd for a in axes):
+                raise RSTSRError("InvalidLayout", "transpose: axis out of range")
+            axes = [a + nd if a < 0 else a for a in axes]
         if sorted(axes) != list(range(nd)):
             raise RSTSRError("InvalidLayout", "transpose: invalid permutation")
         sh, st, off = self.layout
@@ -204,6 +222,8 @@ class Tensor:
 
     def swapaxes(self, a, b):
         nd = self.ndim
+        if not (-nd <= a < nd and -nd <= b < nd):
+            raise RSTSRError("InvalidLayout", f"swapaxes: axes ({a}, {b}) out of range for {nd} dimensions")
         a, b = a % nd, b % nd
         p = list(range(nd))
         p[a], p[b] = p[b], p[a]
@@ -214,6 +234,10 @@ class Tensor:
 
     def slice(self, axis, start=None, stop=None, step=None):
         return self._view(_rstsr.dim_narrow(self.layout, axis, start, stop, 1 if step is None else step))
+
+    def i(self, idx):
+        """rstsr's .i(...) indexing: same as [...]."""
+        return self[idx]
 
     def __getitem__(self, idx):
         if not isinstance(idx, tuple):
@@ -245,43 +269,106 @@ class Tensor:
         st = [s for k, s in enumerate(st) if k not in drop]
         return self._view((sh, st, off))
 
-    def reshape(self, shape):
-        """into_shape: a view when the layout is c-contiguous, else a c-contiguous copy first."""
-        shape = [int(s) for s in shape]
+    def _reshape(self, shape, order=None):
+        """(storage view or copy, new layout, copied?) of a reshape in `order` (default: the device's
+        default order, tensor/manipulation/reshape.rs:126): elements pair up in C order for RowMajor
+        and F order for ColMajor; a view when the layout is contiguous in that order, else a copy
+        into a contiguous layout first."""
+        shape = [int(x) for x in shape]
+        if shape.count(-1) > 1:
+            raise RSTSRError("InvalidLayout", "reshape: at most one -1")
         if -1 in shape:
-            known = int(np.prod([s for s in shape if s != -1]))
+            known = int(np.prod([x for x in shape if x != -1]))
             shape[shape.index(-1)] = self.size // max(known, 1)
-        if int(np.prod(shape)) != self.size:
+        if int(np.prod(shape)) != self.size or any(x < 0 for x in shape):
             raise RSTSRError("InvalidLayout", "reshape: element counts differ")
-        src = self if _rstsr.c_contig(self.layout) else self.to_contig()
-        st = _rstsr.c_layout(shape)[1]
-        return Tensor(src._t._with_layout((shape, st, src.offset)))
+        order = order or self._order()
+        fo = order in ("F", "ColMajor")
+        contig = _rstsr.f_contig(self.layout) if fo else _rstsr.c_contig(self.layout)
+        src, copied = (self, False) if contig else (self.to_contig("F" if fo else "C"), True)
+        st = (_rstsr.f_layout(shape) if fo else _rstsr.c_layout(shape))[1]
+        return src, (shape, st, src.offset), copied
 
-    into_shape = reshape
+    def reshape(self, shape, order=None):
+        """reshape (borrowing): a view when no copy is needed, else an owned copy."""
+        src, lay, copied = self._reshape(shape, order)
+        return Tensor(src._t._with_layout(lay), own=copied)
+
+    def into_shape(self, shape, order=None):
+        """into_shape (consuming): the result owns its storage when this tensor did, or when copied."""
+        src, lay, copied = self._reshape(shape, order)
+        return Tensor(src._t._with_layout(lay), own=copied or self._own)
 
     def broadcast_to(self, shape):
-        sh, st, off = self.layout
-        shape = list(shape)
-        n = len(shape)
-        if n < len(sh):
-            raise RSTSRError("InvalidLayout", "broadcast_to: rank too small")
-        nst = [0] * n
-        for i in range(len(sh)):
-            j = n - len(sh) + i
-            if sh[i] == shape[j]:
-                nst[j] = st[i]
-            elif sh[i] == 1:
-                nst[j] = 0
-            else:
-                raise RSTSRError("InvalidLayout", "Broadcasting failed.")
-        return self._view((shape, nst, off))
+        """broadcast_to (tensor/manipulation/broadcast.rs:281): a view broadcast to `shape` under
+        the device's default order (ColMajor aligns the leading axes)."""
+        lay = _rstsr.broadcast_layout_to_shape(self.layout, [int(x) for x in shape], self._order())
+        return self._view(lay)
+
+    # ---------------- contiguity (layout/layoutbase.rs) ----------------
+    def c_contig(self):
+        return _rstsr.c_contig(self.layout)
+
+    def f_contig(self):
+        return _rstsr.f_contig(self.layout)
+
+    def c_prefer(self):
+        return _rstsr.c_prefer(self.layout)
+
+    def f_prefer(self):
+        return _rstsr.f_prefer(self.layout)
+
+    def is_owned(self):
+        return self._own
+
+    def as_ptr(self):
+        """Address of the storage (raw().as_ptr()): equal for views of one storage."""
+        return self._t._data_ptr()
+
+    def view(self):
+        return Tensor(self._t, own=False)
+
+    def view_owned(self):
+        return Tensor(self._t, own=True)
+
+    def to_owned(self):
+        """An owned copy laid out as layout_for_array_copy(self, K)."""
+        return Tensor(self._t.to_layout(_rstsr.layout_for_array_copy(self.layout, "K")))
+
+    def iter(self):
+        """Elements in the device's default order (iterator_elem.rs:74-80: C for RowMajor, F for
+        ColMajor)."""
+        a = self.to_numpy()
+        return iter((a.T if self._order() == "ColMajor" else a).reshape(-1).tolist())
 
     # ---------------- copies ----------------
+    def _cow(self, t):
+        """TensorCow: borrowed when the conversion returned this very storage and layout."""
+        same = t._data_ptr() == self._t._data_ptr() and t.layout == self.layout
+        return Tensor(t, own=not same)
+
     def to_contig(self, order="C"):
-        return Tensor(self._t.to_contig(order))
+        """to_contig (tensor/manipulation/to_contig.rs): a view when the layout already equals the
+        contiguous layout of `order` (offset 0), else an owned contiguous copy."""
+        return self._cow(self._t.to_contig(order))
+
+    change_contig = to_contig
+    to_contig_f = to_contig
+
+    def into_contig(self, order="C"):
+        r = self.to_contig(order)
+        return Tensor(r._t, own=True)
+
+    def to_prefer(self, order="C"):
+        """to_prefer (to_contig.rs:229-243): a view when the layout is c_prefer (RowMajor) /
+        f_prefer (ColMajor), else a contiguous copy in that order."""
+        fo = order in ("F", "ColMajor")
+        if (fo and self.f_prefer()) or (not fo and self.c_prefer()):
+            return Tensor(self._t, own=False)
+        return Tensor(self._t.to_contig("F" if fo else "C"), own=True)
 
     def to_layout(self, layout):
-        return Tensor(self._t.to_layout(layout))
+        return self._cow(self._t.to_layout(layout))
 
     def astype(self, dtype):
         return Tensor(self._t.astype(_code(dtype)))
@@ -342,6 +429,59 @@ class Tensor:
     def __itruediv__(self, o):
         return self._ibin(OP_DIV, o)
 
+    # bit operators (integers and bool) and shifts (integers): operators/ops/op_ternary_arithmetic.rs
+    def __and__(self, o):
+        return self._bin(OP_BITAND, o)
+
+    def __rand__(self, o):
+        return self._bin(OP_BITAND, o, left=True)
+
+    def __or__(self, o):
+        return self._bin(OP_BITOR, o)
+
+    def __ror__(self, o):
+        return self._bin(OP_BITOR, o, left=True)
+
+    def __xor__(self, o):
+        return self._bin(OP_BITXOR, o)
+
+    def __rxor__(self, o):
+        return self._bin(OP_BITXOR, o, left=True)
+
+    def __lshift__(self, o):
+        return self._bin(OP_SHL, o)
+
+    def __rlshift__(self, o):
+        return self._bin(OP_SHL, o, left=True)
+
+    def __rshift__(self, o):
+        return self._bin(OP_SHR, o)
+
+    def __rrshift__(self, o):
+        return self._bin(OP_SHR, o, left=True)
+
+    def __iand__(self, o):
+        return self._ibin(OP_BITAND, o)
+
+    def __ior__(self, o):
+        return self._ibin(OP_BITOR, o)
+
+    def __ixor__(self, o):
+        return self._ibin(OP_BITXOR, o)
+
+    def __ilshift__(self, o):
+        return self._ibin(OP_SHL, o)
+
+    def __irshift__(self, o):
+        return self._ibin(OP_SHR, o)
+
+    def rem(self, o):
+        """Elementwise remainder (rt::rem; `%` is matmul in rstsr)."""
+        return self._bin(OP_REM, o)
+
+    def rem_(self, o):
+        return self._ibin(OP_REM, o)
+
     def __mod__(self, o):
         return matmul(self, o)
 
@@ -351,8 +491,10 @@ class Tensor:
     # ---------------- reductions ----------------
     def _red_all(self, op):
         raw = self._t.reduce_all(op)
-        if op in (RED_ARGMAX, RED_ARGMIN):
+        if op in (RED_ARGMAX, RED_ARGMIN, RED_COUNT_NONZERO) or (op == RED_SUM and np.dtype(self.dtype) == np.bool_):
             return int(np.frombuffer(raw, dtype=np.uint64)[0])
+        if op in (RED_ALL, RED_ANY):
+            return bool(np.frombuffer(raw, dtype=np.uint8)[0])
         dt = self.dtype
         if op in (RED_VAR, RED_STD, RED_L2NORM):
             dt = _REAL_OF.get(np.dtype(dt), dt)
@@ -422,6 +564,26 @@ class Tensor:
 
     def l2_norm_axes(self, axes=None):
         return self._red_axes(RED_L2NORM, axes)
+
+    # bool folds and counts (operators/reduction.rs: OpAllAPI, OpAnyAPI, OpCountNonZeroAPI); the
+    # sum of a bool array counts its true elements (OpSumBoolAPI)
+    def all_all(self):
+        return self._red_all(RED_ALL)
+
+    def any_all(self):
+        return self._red_all(RED_ANY)
+
+    def count_nonzero_all(self):
+        return self._red_all(RED_COUNT_NONZERO)
+
+    def all_axes(self, axes=None):
+        return self._red_axes(RED_ALL, axes)
+
+    def any_axes(self, axes=None):
+        return self._red_axes(RED_ANY, axes)
+
+    def count_nonzero_axes(self, axes=None):
+        return self._red_axes(RED_COUNT_NONZERO, axes)
 
     sum = sum_axes
 
@@ -578,16 +740,38 @@ def format_tensor(vec, shape, stride, offset, max_print=None, min_print=None, fm
     return "".join(out)
 
 
-def asarray(x, device=None, dtype=None):
-    """rt::asarray((vec, shape, &device)) — upload host data as a c-contiguous tensor."""
+def _is_f(dev):
+    return dev.default_order == "ColMajor"
+
+
+def asarray(x, device=None, dtype=None, shape=None):
+    """rt::asarray: host data uploaded with the device's default order (tensor/asarray.rs:459-470):
+    a c-contiguous layout for RowMajor, an f-contiguous one for ColMajor, the same logical
+    values either way. With `shape`, x is a flat vector laid out in that order, as
+    rt::asarray((vec, shape, &device))."""
     dev = device or default_device()
-    a = np.ascontiguousarray(np.asarray(x, dtype=dtype))
+    a = np.asarray(x, dtype=dtype)
+    if shape is not None:
+        shape = [int(n) for n in shape]
+        a = a.reshape(-1).reshape(shape, order="F" if _is_f(dev) else "C")
+    if _is_f(dev) and a.ndim > 1:
+        # F-contiguous data of shape S is c-contiguous data of reversed(S) with its axes reversed
+        at = np.ascontiguousarray(a.T)
+        return Tensor(_rstsr.from_host(dev, at.tobytes(), list(at.shape), _code(at.dtype))).transpose().view_owned()
+    a = np.asarray(a, order="C")  # (ascontiguousarray would turn a 0-d array into 1-d)
     return Tensor(_rstsr.from_host(dev, a.tobytes(), list(a.shape), _code(a.dtype)))
+
+
+def tensor_from_nested(x, device=None, dtype=None):
+    """rt::tensor_from_nested!(...): a nested list as a tensor of the device."""
+    return asarray(x, device=device, dtype=dtype)
 
 
 def empty(shape, dtype=np.float64, device=None):
     dev = device or default_device()
-    return Tensor(_rstsr.empty(dev, _rstsr.c_layout(list(shape)), _code(dtype)))
+    shape = [int(n) for n in shape]
+    lay = _rstsr.f_layout(shape) if _is_f(dev) else _rstsr.c_layout(shape)
+    return Tensor(_rstsr.empty(dev, lay, _code(dtype)))
 
 
 def full(shape, value, dtype=np.float64, device=None):
@@ -614,6 +798,85 @@ def arange(n, device=None, dtype=None):
 def linspace(start, stop, num, device=None):
     """rt::linspace((start, stop, n)) as NumPy computes it."""
     return asarray(np.linspace(start, stop, num), device=device)
+
+
+def transpose(a, axes=None):
+    return a.transpose(axes)
+
+
+permute_dims = transpose
+
+
+def swapaxes(a, i, j):
+    return a.swapaxes(i, j)
+
+
+def to_contig(a, order="C"):
+    return a.to_contig(order)
+
+
+def to_prefer(a, order="C"):
+    return a.to_prefer(order)
+
+
+def broadcast_to(a, shape):
+    return a.broadcast_to(shape)
+
+
+def broadcast_arrays(tensors):
+    """rt::broadcast_arrays (tensor/manipulation/broadcast.rs:240-265): views of every tensor at
+    the common broadcast shape, under the first tensor's device order."""
+    if not tensors:
+        return []
+    order = tensors[0]._order()
+    lay = tensors[0].layout
+    for t in tensors[1:]:
+        lay = _rstsr.broadcast_layout(lay, t.layout, order)[0]
+    shape = list(lay[0])
+    return [t._view(_rstsr.broadcast_layout_to_shape(t.layout, shape, order)) for t in tensors]
+
+
+def add(a, b):
+    return a + b
+
+
+def sub(a, b):
+    return a - b
+
+
+def mul(a, b):
+    return a * b
+
+
+def div(a, b):
+    return a / b
+
+
+def rem(a, b):
+    """rt::rem: elementwise remainder (Rust's %: truncated for integers, fmod for floats)."""
+    if isinstance(a, Tensor):
+        return a._bin(OP_REM, b)
+    return b._bin(OP_REM, a, left=True)
+
+
+def bitand(a, b):
+    return a & b
+
+
+def bitor(a, b):
+    return a | b
+
+
+def bitxor(a, b):
+    return a ^ b
+
+
+def shl(a, b):
+    return a << b
+
+
+def shr(a, b):
+    return a >> b
 
 
 def matmul(a, b, mode="plain"):

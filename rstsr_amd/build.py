@@ -9,6 +9,7 @@ Outputs:
     rstsr_amd/librstsr_hip.so            C ABI of include/rstsr_hip.h (hipcc, --offload-arch=gfx950)
     rstsr_amd/_rstsr<ext>.so             pybind11 module over the C++ front end (links the library)
     oracle/liboracle.so                  CPU restatement used only by tests and benchmarks
+    tests/gpu_check/libdot2check.so      GPU Dot2 GEMM checker used only by tests and benchmarks
 """
 import os
 import subprocess
@@ -107,6 +108,22 @@ def build_front(force=True):
     return out
 
 
+CHECKER_SRC = os.path.join(ROOT, "tests", "gpu_check", "dot2_check.hip")
+CHECKER_LIB = os.path.join(ROOT, "tests", "gpu_check", "libdot2check.so")
+
+
+def build_checker(force=True):
+    """GPU Dot2 GEMM checker (test infrastructure for the GEMM error measurements)."""
+    if not os.path.exists(CHECKER_SRC):
+        return None
+    if not force and not _newer(CHECKER_LIB, [CHECKER_SRC]):
+        return CHECKER_LIB
+    tmp = CHECKER_LIB + ".tmp"
+    _run([HIPCC] + HIP_FLAGS + ["-ffp-contract=off", "-shared", "-o", tmp, CHECKER_SRC])
+    os.replace(tmp, CHECKER_LIB)
+    return CHECKER_LIB
+
+
 def build_oracle(force=True):
     return _run(["make", "-C", os.path.join(ROOT, "oracle")] + (["-B"] if force else []))
 
@@ -115,6 +132,7 @@ def build_all(force=True):
     build_lib(force=force)
     build_front(force=force)
     build_oracle(force=force)
+    build_checker(force=force)
 
 
 if __name__ == "__main__":
@@ -125,6 +143,8 @@ if __name__ == "__main__":
         print(build_front())
     elif "--oracle" in args:
         print(build_oracle())
+    elif "--checker" in args:
+        print(build_checker())
     else:
         build_all(force="--incremental" not in args)
         print("built", LIB)

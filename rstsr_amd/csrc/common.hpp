@@ -217,8 +217,9 @@ __device__ __forceinline__ T ld_nt(const T* p) {
             __builtin_memcpy((char*)&v + 8 * i, &w, 8);
         }
         return v;
+    } else if constexpr (sizeof(T) % 4 != 0 || alignof(T) < 4) {
+        return *p;  // byte-sized elements (bool): plain load
     } else {
-        static_assert(sizeof(T) % 4 == 0 && alignof(T) >= 4, "streaming load of an odd-sized element");
         T v;
         const unsigned* q = (const unsigned*)p;
         unsigned* d = (unsigned*)&v;

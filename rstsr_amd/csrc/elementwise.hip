@@ -59,12 +59,92 @@ __device__ __forceinline__ c64 op_mul(c64 a, c64 b) {
     return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
 }
 
+// num-complex Div: ((a.re c + a.im d) / n, (a.im c - a.re d) / n) with n = c*c + d*d
+template <>
+__device__ __forceinline__ c128 op_div(c128 a, c128 b) {
+    double n = b.re * b.re + b.im * b.im;
+    return {(a.re * b.re + a.im * b.im) / n, (a.im * b.re - a.re * b.im) / n};
+}
+template <>
+__device__ __forceinline__ c64 op_div(c64 a, c64 b) {
+    float n = b.re * b.re + b.im * b.im;
+    return {(a.re * b.re + a.im * b.im) / n, (a.im * b.re - a.re * b.im) / n};
+}
+
+// Integer / and % truncate toward zero, as Rust's. Rust panics on a zero divisor and on
+// MIN / -1 (MIN % -1); a kernel cannot stop there, so the device gives 0 for a zero divisor,
+// the wrapped MIN for MIN / -1 and 0 for MIN % -1.
+template <class T>
+__device__ __forceinline__ T int_div(T a, T b) {
+    using U = typename std::make_unsigned<T>::type;
+    if (b == 0) return 0;
+    if (b == -1) return (T)((U)0 - (U)a);
+    return a / b;
+}
+template <class T>
+__device__ __forceinline__ T int_rem(T a, T b) {
+    return (b == 0 || b == -1) ? (T)0 : (T)(a % b);
+}
+
+// Rem: integers as above; floats fmod (Rust's f64 % f64); complex as num-complex:
+// q = a / b, q0 = (q.re - q.re % 1, q.im - q.im % 1), a - b * q0
+template <class T>
+__device__ __forceinline__ T op_rem(T a, T b) {
+    if constexpr (std::is_same<T, double>::value) return fmod(a, b);
+    else if constexpr (std::is_same<T, float>::value) return fmodf(a, b);
+    else if constexpr (std::is_same<T, c128>::value) {
+        c128 q = op_div<c128>(a, b);
+        c128 q0{q.re - fmod(q.re, 1.0), q.im - fmod(q.im, 1.0)};
+        return op_sub<c128>(a, op_mul<c128>(b, q0));
+    } else if constexpr (std::is_same<T, c64>::value) {
+        c64 q = op_div<c64>(a, b);
+        c64 q0{q.re - fmodf(q.re, 1.0f), q.im - fmodf(q.im, 1.0f)};
+        return op_sub<c64>(a, op_mul<c64>(b, q0));
+    } else {
+        return int_rem<T>(a, b);
+    }
+}
+
+// Shifts as Rust release builds: the amount is taken modulo the bit width; >> of a signed
+// integer is arithmetic. Bit ops on bool are logical.
+template <class T>
+__device__ __forceinline__ T op_shl(T a, T b) {
+    using U = typename std::make_unsigned<T>::type;
+    return (T)((U)a << ((U)b & (U)(8 * sizeof(T) - 1)));
+}
+template <class T>
+__device__ __forceinline__ T op_shr(T a, T b) {
+    using U = typename std::make_unsigned<T>::type;
+    return (T)(a >> ((U)b & (U)(8 * sizeof(T) - 1)));
+}
+
 template <int OP, class T>
 __device__ __forceinline__ T apply_op(T a, T b) {
     if constexpr (OP == RSTSR_OP_ADD) return op_add<T>(a, b);
     if constexpr (OP == RSTSR_OP_SUB) return op_sub<T>(a, b);
     if constexpr (OP == RSTSR_OP_MUL) return op_mul<T>(a, b);
-    if constexpr (OP == RSTSR_OP_DIV) return op_div<T>(a, b);
+    if constexpr (OP == RSTSR_OP_DIV) {
+        if constexpr (std::is_integral<T>::value) return int_div<T>(a, b);
+        else return op_div<T>(a, b);
+    }
+    if constexpr (OP == RSTSR_OP_REM) return op_rem<T>(a, b);
+    if constexpr (OP == RSTSR_OP_BITAND) return (T)(a & b);
+    if constexpr (OP == RSTSR_OP_BITOR) return (T)(a | b);
+    if constexpr (OP == RSTSR_OP_BITXOR) return (T)(a ^ b);
+    if constexpr (OP == RSTSR_OP_SHL) return op_shl<T>(a, b);
+    if constexpr (OP == RSTSR_OP_SHR) return op_shr<T>(a, b);
+}
+
+// Which (op, element type) pairs exist in the reference (the std / num-complex operator impls
+// behind auto_impl/op_ternary_arithmetic.rs:3-14): arithmetic and Rem for every number type,
+// bit ops for integers and bool, shifts for integers.
+template <int OP, class T>
+constexpr bool op_defined() {
+    constexpr bool num = !std::is_same<T, bool>::value;
+    constexpr bool integ = std::is_integral<T>::value && num;
+    if constexpr (OP <= RSTSR_OP_REM) return num;
+    else if constexpr (OP <= RSTSR_OP_BITXOR) return std::is_integral<T>::value;
+    else return integ;
 }
 
 // value conversion for assign with promotion (Rust `as`-free DTypeCastAPI subset)
@@ -190,7 +270,7 @@ struct CloseBody {
 static void binary_impl(rstsr_hip_ctx* ctx, rstsr_binop_t op, rstsr_dtype_t t, void* c, const rstsr_layout_t* lc_,
                         const void* a, const rstsr_layout_t* la_, const void* b, const rstsr_layout_t* lb_) {
     bind(ctx);
-    if ((int)op < 0 || (int)op > 3) RSTSR_RAISE(InvalidValue, "unknown binary op");
+    if ((int)op < 0 || (int)op > 9) RSTSR_RAISE(InvalidValue, "unknown binary op");
     Layout lc = from_abi(lc_);
     bool has_a = la_ != nullptr, has_b = lb_ != nullptr;
     Layout la = has_a ? from_abi(la_) : lc;
@@ -202,37 +282,32 @@ static void binary_impl(rstsr_hip_ctx* ctx, rstsr_binop_t op, rstsr_dtype_t t, v
     check_extent(c, lc, es, "c");
     if (has_a) check_extent(a, la, es, "a");
     if (has_b) check_extent(b, lb, es, "b");
-    if (op == RSTSR_OP_DIV && !(t == RSTSR_F32 || t == RSTSR_F64))
-        RSTSR_RAISE(UnImplemented, "division is provided for f32/f64 only");
     const Stride* s[3] = {&lc.stride, has_a ? &la.stride : nullptr, has_b ? &lb.stride : nullptr};
     size_t off[3] = {lc.offset, has_a ? la.offset : 0, has_b ? lb.offset : 0};
     Dims d = collapse3(lc.shape, s, off);
     EwChoice ch = choose(d, has_a, has_b);
     // tile-order tuning reruns the op: only when the output is not also an input
     const bool can_time = (!has_a || !same_buffer(c, a)) && (!has_b || !same_buffer(c, b));
-    dispatch_dtype(t, [&](auto tag) {
+    auto by_type = [&](auto tag) {
         using T = decltype(tag);
         auto run = [&](auto opc) {
             constexpr int OP = decltype(opc)::value;
-            if constexpr (OP == RSTSR_OP_DIV && !std::is_floating_point<T>::value) {
-                RSTSR_RAISE(UnImplemented, "division is provided for f32/f64 only");
+            if constexpr (!op_defined<OP, T>()) {
+                RSTSR_RAISE(UnImplemented, "this binary operator is not defined for this dtype");
             } else {
+                // add/sub/mul and floating division are the hot path: every kernel form, streaming
+                // variants included; the other operators run the generic and row kernels
+                constexpr bool HOT = OP <= RSTSR_OP_MUL || (OP == RSTSR_OP_DIV && !std::is_integral<T>::value);
                 T sa{}, sb{};
                 if (!has_a) std::memcpy(&sa, a, sizeof(T));
                 if (!has_b) std::memcpy(&sb, b, sizeof(T));
-                if (has_a && has_b) {
-                    BinBody<OP, T, false, false> body{(T*)c, (const T*)a, (const T*)b, sa, sb};
-                    launch_ew(ctx, body, d, ch, can_time);
-                } else if (has_a) {
-                    BinBody<OP, T, false, true> body{(T*)c, (const T*)a, nullptr, sa, sb};
-                    launch_ew(ctx, body, d, ch, can_time);
-                } else if (has_b) {
-                    BinBody<OP, T, true, false> body{(T*)c, nullptr, (const T*)b, sa, sb};
-                    launch_ew(ctx, body, d, ch, can_time);
-                } else {
-                    BinBody<OP, T, true, true> body{(T*)c, nullptr, nullptr, sa, sb};
-                    launch_ew(ctx, body, d, ch);
-                }
+                auto go = [&](auto body, bool timed) {
+                    launch_ew<decltype(body), HOT, HOT>(ctx, body, d, ch, timed);
+                };
+                if (has_a && has_b) go(BinBody<OP, T, false, false>{(T*)c, (const T*)a, (const T*)b, sa, sb}, can_time);
+                else if (has_a) go(BinBody<OP, T, false, true>{(T*)c, (const T*)a, nullptr, sa, sb}, can_time);
+                else if (has_b) go(BinBody<OP, T, true, false>{(T*)c, nullptr, (const T*)b, sa, sb}, can_time);
+                else go(BinBody<OP, T, true, true>{(T*)c, nullptr, nullptr, sa, sb}, false);
             }
         };
         switch (op) {
@@ -240,8 +315,16 @@ static void binary_impl(rstsr_hip_ctx* ctx, rstsr_binop_t op, rstsr_dtype_t t, v
             case RSTSR_OP_SUB: run(std::integral_constant<int, RSTSR_OP_SUB>{}); break;
             case RSTSR_OP_MUL: run(std::integral_constant<int, RSTSR_OP_MUL>{}); break;
             case RSTSR_OP_DIV: run(std::integral_constant<int, RSTSR_OP_DIV>{}); break;
+            case RSTSR_OP_REM: run(std::integral_constant<int, RSTSR_OP_REM>{}); break;
+            case RSTSR_OP_BITAND: run(std::integral_constant<int, RSTSR_OP_BITAND>{}); break;
+            case RSTSR_OP_BITOR: run(std::integral_constant<int, RSTSR_OP_BITOR>{}); break;
+            case RSTSR_OP_BITXOR: run(std::integral_constant<int, RSTSR_OP_BITXOR>{}); break;
+            case RSTSR_OP_SHL: run(std::integral_constant<int, RSTSR_OP_SHL>{}); break;
+            case RSTSR_OP_SHR: run(std::integral_constant<int, RSTSR_OP_SHR>{}); break;
         }
-    });
+    };
+    if (t == RSTSR_BOOL) by_type(bool{});
+    else dispatch_dtype(t, by_type);
 }
 
 /* ---------------- assign / copy ---------------- */

@@ -404,7 +404,9 @@ static void with_ntm(int v, F&& f) {
 // Streaming hints (compile-time kernel variants): loads of operands read once,
 // stores of outputs of at least EW_NT_STORE_MIN bytes. Bodies with NT_JOINT
 // take both hints or neither (fewer variants).
-template <class Body, bool TILED = true>
+// NTV = false: only the plain (hint-free) variant of the row kernel is instantiated, for bodies
+// off the hot path.
+template <class Body, bool TILED = true, bool NTV = true>
 static void launch_ew(rstsr_hip_ctx* ctx, const Body& body, Dims d, EwChoice ch, bool can_time = false) {
     constexpr int OPS = Body::OPS;
     if (!TILED && ch.kind == EwKind::Tiled) ch.kind = EwKind::Generic;
@@ -454,7 +456,9 @@ static void launch_ew(rstsr_hip_ctx* ctx, const Body& body, Dims d, EwChoice ch,
                 hipLaunchKernelGGL((k_ew_rows<Body, decltype(ntm)::value>), dim3((uint32_t)nb), dim3(EW_BLOCK), 0,
                                    ctx->stream, body, p, nchunk, FastDiv(nchunk));
             };
-            if constexpr (Body::NT_JOINT) {
+            if constexpr (!NTV) {
+                go(std::integral_constant<int, 0>{});
+            } else if constexpr (Body::NT_JOINT) {
                 if (rows_ntm) go(std::integral_constant<int, OPS | 4>{});
                 else go(std::integral_constant<int, 0>{});
             } else {

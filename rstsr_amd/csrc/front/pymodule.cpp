@@ -52,6 +52,13 @@ PYBIND11_MODULE(_rstsr, m) {
         .def("synchronize", &DeviceHIP::synchronize)
         .def("same_device", &DeviceHIP::same_device)
         .def_property_readonly("device_id", [](const DeviceHIP& d) { return d.ctx->device; })
+        // DeviceBaseAPI (storage/device.rs:3-7): a clone shares the GPU context and carries its own
+        // default order; tensors keep the order of the device value they were created with
+        .def("clone", [](const DeviceHIP& d) { return d; })
+        .def("set_default_order", [](DeviceHIP& d, const std::string& o) { d.set_default_order(flag_order(o)); })
+        .def_property_readonly("default_order", [](const DeviceHIP& d) {
+            return std::string(d.default_order() == FlagOrder::RowMajor ? "RowMajor" : "ColMajor");
+        })
         .def("_ctx_ptr", [](const DeviceHIP& d) { return (uintptr_t)d.raw(); });
 
     py::class_<Tensor>(m, "Tensor")
@@ -134,6 +141,9 @@ PYBIND11_MODULE(_rstsr, m) {
     m.def("broadcast_layout", [](const PyLayout& a, const PyLayout& b, const std::string& o) {
         auto r = broadcast_layout(L(a), L(b), flag_order(o));
         return py::make_tuple(P(r.first), P(r.second));
+    });
+    m.def("broadcast_layout_to_shape", [](const PyLayout& l, const std::vector<size_t>& shape, const std::string& o) {
+        return P(broadcast_layout_to_shape(L(l), shape, flag_order(o)));
     });
     m.def("matmul_output_layout", [](const PyLayout& a, const PyLayout& b, const std::string& o) {
         return P(matmul_output_layout(L(a), L(b), flag_order(o)));

@@ -163,12 +163,17 @@ class Tensor {
         AbiLayout x(layout);
         check(rstsr_hip_fill(device().raw(), dtype, ptr(), &x, value));
     }
-    // change_layout_f: no copy when the layout is unchanged
+    // change_layout_f (tensor/manipulation/to_layout.rs:8-38): no copy when the layout is
+    // unchanged, else assign_arbitary into the new layout. assign_arbitary pairs elements in
+    // the device's default order (cpu_rayon/assignment.rs:59-70): C order for RowMajor, F order
+    // for ColMajor, which is C order over both layouts with their axes reversed.
     Tensor to_layout(const Layout& lnew) const {
         if (lnew.size() != size()) RSTSR_RAISE(InvalidLayout, "to_layout: element counts differ");
         if (lnew == layout) return *this;
         Tensor r = uninit(device(), lnew, dtype);
-        AbiLayout x(lnew), y(layout);
+        const bool f = device().default_order() == FlagOrder::ColMajor;
+        Layout lc = f ? lnew.reverse_axes() : lnew, la = f ? layout.reverse_axes() : layout;
+        AbiLayout x(lc), y(la);
         check(rstsr_hip_assign_arbitary(device().raw(), dtype, r.ptr(), &x, dtype, ptr(), &y));
         return r;
     }
@@ -249,6 +254,8 @@ class Tensor {
     // type for var/std/l2_norm of complex (T::Real), else T.
     rstsr_dtype_t reduce_out_dtype(rstsr_redop_t op) const {
         if (op == RSTSR_RED_ARGMAX || op == RSTSR_RED_ARGMIN) return RSTSR_U64;
+        if (op == RSTSR_RED_ALL || op == RSTSR_RED_ANY) return RSTSR_BOOL;
+        if (op == RSTSR_RED_COUNT_NONZERO || (op == RSTSR_RED_SUM && dtype == RSTSR_BOOL)) return RSTSR_U64;
         if (op == RSTSR_RED_VAR || op == RSTSR_RED_STD || op == RSTSR_RED_L2NORM) {
             if (dtype == RSTSR_C64) return RSTSR_F32;
             if (dtype == RSTSR_C128) return RSTSR_F64;
@@ -289,12 +296,22 @@ class Tensor {
     }
 
     /* ---------- matmul ---------- */
+    // The output layout follows the device's default order (layout/matmul.rs:423-463); the
+    // device call for ColMajor reverses every layout's axes and swaps a and b, as DeviceFaer's
+    // matmul does (device_faer/matmul.rs:227-237).
     Tensor matmul(const Tensor& b, rstsr_gemm_mode_t mode) const {
         if (dtype != b.dtype) RSTSR_RAISE(InvalidValue, "operands must share one dtype");
         device().order_after(b.device());
-        Layout lc = matmul_output_layout(layout, b.layout, FlagOrder::RowMajor);
+        const FlagOrder ord = device().default_order();
+        Layout lc = matmul_output_layout(layout, b.layout, ord);
         Tensor c = uninit(device(), lc, dtype);
-        AbiLayout xc(lc), xa(layout), xb(b.layout);
+        const bool f = ord == FlagOrder::ColMajor;
+        const Layout lcd = f ? lc.reverse_axes() : lc;
+        const Layout lad = f ? b.layout.reverse_axes() : layout;
+        const Layout lbd = f ? layout.reverse_axes() : b.layout;
+        const void* pa = f ? b.ptr() : ptr();
+        const void* pb = f ? ptr() : b.ptr();
+        AbiLayout xc(lcd), xa(lad), xb(lbd);
         unsigned char one[16] = {0}, zero[16] = {0};
         switch (dtype) {
             case RSTSR_F32: { float v = 1; std::memcpy(one, &v, 4); break; }
@@ -305,7 +322,7 @@ class Tensor {
             case RSTSR_C128: { double v[2] = {1, 0}; std::memcpy(one, v, 16); break; }
             default: RSTSR_RAISE(UnImplemented, "matmul dtype");
         }
-        check(rstsr_hip_matmul(device().raw(), dtype, c.ptr(), &xc, ptr(), &xa, b.ptr(), &xb, one, zero, mode));
+        check(rstsr_hip_matmul(device().raw(), dtype, c.ptr(), &xc, pa, &xa, pb, &xb, one, zero, mode));
         return c;
     }
 };

@@ -239,6 +239,42 @@ struct ProdI {
     __device__ static Out fin(S s, T, uint64_t) { return (T)s; }
 };
 
+// all / any of a bool array (auto_impl/reduction.rs:466-519): fold with && / || from true /
+// false. Elements are read as bytes (nonzero is true).
+template <bool ALL>
+struct BoolFold {
+    using Val = uint8_t;
+    using Out = uint8_t;
+    using S = uint32_t;
+    __device__ static S init() { return ALL ? 1u : 0u; }
+    __device__ static void add(S& s, uint8_t x, uint64_t) {
+        if (ALL) s &= (x != 0);
+        else s |= (x != 0);
+    }
+    __device__ static S combine(S a, S b) { return ALL ? (a & b) : (a | b); }
+    __device__ static S shfl(S s, int m) { return (S)__shfl_xor((int)s, m, 64); }
+    __device__ static Out fin(S s, uint8_t, uint64_t) { return (Out)s; }
+};
+
+// count_nonzero (auto_impl/reduction.rs:521-551): elements != T::zero() counted into a usize;
+// a bool array's sum (OpSumBoolAPI, :678-715) is the same count. A complex element is nonzero
+// when either part is; NaN is nonzero, -0.0 is zero.
+template <class T>
+struct CountNZ {
+    using Val = T;
+    using Out = uint64_t;
+    using S = uint64_t;
+    __device__ static bool nz(T x) {
+        if constexpr (std::is_same<T, c128>::value || std::is_same<T, c64>::value) return x.re != 0 || x.im != 0;
+        else return x != 0;
+    }
+    __device__ static S init() { return 0; }
+    __device__ static void add(S& s, T x, uint64_t) { s += nz(x) ? 1 : 0; }
+    __device__ static S combine(S a, S b) { return a + b; }
+    __device__ static S shfl(S s, int m) { return shfl_u64(s, m); }
+    __device__ static Out fin(S s, T, uint64_t) { return s; }
+};
+
 // mean (auto_impl/reduction.rs:167-205): the compensated sum divided by the
 // number of reduced elements.
 template <class T>
@@ -863,8 +899,29 @@ static void reduce_dispatch(rstsr_hip_ctx* ctx, rstsr_redop_t op, rstsr_dtype_t 
         run_reduce<A, T>(ctx, s, (const T*)a, (typename A::Out*)out, fixed);
     };
     switch (op) {
+        case RSTSR_RED_ALL:
+        case RSTSR_RED_ANY:
+            if (t == RSTSR_BOOL) {
+                if (op == RSTSR_RED_ALL) go(BoolFold<true>{}, uint8_t{});
+                else go(BoolFold<false>{}, uint8_t{});
+                return;
+            }
+            break;
+        case RSTSR_RED_COUNT_NONZERO:
+            switch (t) {
+                case RSTSR_F32: go(CountNZ<float>{}, float{}); return;
+                case RSTSR_F64: go(CountNZ<double>{}, double{}); return;
+                case RSTSR_I32: go(CountNZ<int32_t>{}, int32_t{}); return;
+                case RSTSR_I64: go(CountNZ<int64_t>{}, int64_t{}); return;
+                case RSTSR_C64: go(CountNZ<c64>{}, c64{}); return;
+                case RSTSR_C128: go(CountNZ<c128>{}, c128{}); return;
+                case RSTSR_BOOL: go(CountNZ<uint8_t>{}, uint8_t{}); return;
+                default: break;
+            }
+            break;
         case RSTSR_RED_SUM:
             switch (t) {
+                case RSTSR_BOOL: go(CountNZ<uint8_t>{}, uint8_t{}); return;
                 case RSTSR_F32: go(SumF<float>{}, float{}); return;
                 case RSTSR_F64: go(SumF<double>{}, double{}); return;
                 case RSTSR_I32: go(SumI<int32_t>{}, int32_t{}); return;
@@ -943,6 +1000,9 @@ static void reduce_dispatch(rstsr_hip_ctx* ctx, rstsr_redop_t op, rstsr_dtype_t 
 
 static size_t out_size_of(rstsr_redop_t op, rstsr_dtype_t t) {
     if (op == RSTSR_RED_ARGMAX || op == RSTSR_RED_ARGMIN) return 8;
+    // all / any: bool; count_nonzero and the sum of a bool array: usize
+    if (op == RSTSR_RED_ALL || op == RSTSR_RED_ANY) return 1;
+    if (op == RSTSR_RED_COUNT_NONZERO || (op == RSTSR_RED_SUM && t == RSTSR_BOOL)) return 8;
     // var/std/l2_norm of a complex array are real (T::Real)
     if ((op == RSTSR_RED_VAR || op == RSTSR_RED_STD || op == RSTSR_RED_L2NORM) && (t == RSTSR_C64 || t == RSTSR_C128))
         return dtype_size(t) / 2;
@@ -950,7 +1010,7 @@ static size_t out_size_of(rstsr_redop_t op, rstsr_dtype_t t) {
 }
 
 static void check_op(rstsr_redop_t op, bool arg) {
-    if ((int)op < 0 || (int)op > 9) RSTSR_RAISE(InvalidValue, "unknown reduction op");
+    if ((int)op < 0 || (int)op > 12) RSTSR_RAISE(InvalidValue, "unknown reduction op");
     bool is_arg = op == RSTSR_RED_ARGMAX || op == RSTSR_RED_ARGMIN;
     if (is_arg != arg) RSTSR_RAISE(InvalidValue, arg ? "expected argmax/argmin" : "argmax/argmin need the arg entry point");
 }
@@ -961,12 +1021,19 @@ static void check_op(rstsr_redop_t op, bool arg) {
 // and argmax/argmin over nothing are errors there (auto_impl/reduction.rs:95-97).
 static bool empty_value_op(rstsr_redop_t op) {
     return op == RSTSR_RED_SUM || op == RSTSR_RED_PROD || op == RSTSR_RED_MEAN || op == RSTSR_RED_VAR ||
-           op == RSTSR_RED_STD || op == RSTSR_RED_L2NORM;
+           op == RSTSR_RED_STD || op == RSTSR_RED_L2NORM || op == RSTSR_RED_ALL || op == RSTSR_RED_ANY ||
+           op == RSTSR_RED_COUNT_NONZERO;
 }
 
 static void empty_value(rstsr_redop_t op, rstsr_dtype_t t, void* v) {
     const size_t es = out_size_of(op, t);
     std::memset(v, 0, 16);
+    // all of nothing is true; any, count_nonzero and a bool sum of nothing are zero
+    if (op == RSTSR_RED_ALL) {
+        *(uint8_t*)v = 1;
+        return;
+    }
+    if (op == RSTSR_RED_ANY || op == RSTSR_RED_COUNT_NONZERO || (op == RSTSR_RED_SUM && t == RSTSR_BOOL)) return;
     const bool nan = op == RSTSR_RED_MEAN || op == RSTSR_RED_VAR || op == RSTSR_RED_STD;
     auto put = [&](double re, double im) {
         if (t == RSTSR_F32 || ((t == RSTSR_C64) && es == 4)) *(float*)v = (float)re;
@@ -1035,15 +1102,12 @@ static void reduce_axes_impl(rstsr_hip_ctx* ctx, rstsr_redop_t op, rstsr_dtype_t
             hipLaunchKernelGGL((k_fill_value<T>), dim3(ceil_div_u32(M, RB)), dim3(RB), 0, ctx->stream, (T*)out, p, v);
             check_launch("fill");
         };
-        // the output element type: T, or its real part type for var/std/l2_norm of complex
-        const size_t es = out_size_of(op, t);
-        switch (t) {
-            case RSTSR_F32: z(float{}); break;
-            case RSTSR_F64: z(double{}); break;
-            case RSTSR_I32: z(int32_t{}); break;
-            case RSTSR_I64: z(int64_t{}); break;
-            case RSTSR_C64: es == 4 ? z(float{}) : z(c64{}); break;
-            case RSTSR_C128: es == 8 ? z(double{}) : z(c128{}); break;
+        // the output element, stored by its width (the value's bytes are already in ev)
+        switch (out_size_of(op, t)) {
+            case 1: z(uint8_t{}); break;
+            case 4: z(uint32_t{}); break;
+            case 8: z(uint64_t{}); break;
+            case 16: z(c128{}); break;
             default: RSTSR_RAISE(UnImplemented, "dtype");
         }
         return;

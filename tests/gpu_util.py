@@ -2,7 +2,7 @@
 import numpy as np
 
 import rstsr_amd as rt
-from oracle_ffi import View, c_view
+from oracle_ffi import View, c_view, red_out_dtype
 
 
 def pair(dev, base, shape, stride, offset=0):
@@ -36,10 +36,7 @@ def oracle_reduce(oracle, op, va, axes, dtype):
     nd = len(va.shape)
     ax = [a % nd for a in axes]
     shape = [va.shape[i] for i in range(nd) if i not in ax]
-    odt = np.uint64 if op in (3, 4) else dtype
-    if op in (7, 8, 9) and np.dtype(dtype).kind == "c":  # var/std/l2_norm of complex are real
-        odt = np.float32 if np.dtype(dtype) == np.complex64 else np.float64
-    vo = c_view(shape, odt)
+    vo = c_view(shape, red_out_dtype(op, dtype))
     r = oracle.reduce_axes(op, va, list(axes), vo)
     assert r == 0
     return vo.array()

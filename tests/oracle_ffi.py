@@ -33,6 +33,19 @@ UNARY_OPS = (
 UN = {name: i for i, name in enumerate(UNARY_OPS)}
 
 
+def red_out_dtype(op, dtype):
+    """Element type of a reduction's result: u64 (usize) for argmax/argmin, count_nonzero and the
+    sum of a bool array; bool for all/any; the real part type for var/std/l2_norm of complex."""
+    dt = np.dtype(dtype)
+    if op in (3, 4, 12) or (op == 0 and dt == np.bool_):
+        return np.dtype(np.uint64)
+    if op in (10, 11):
+        return np.dtype(np.bool_)
+    if op in (7, 8, 9) and dt.kind == "c":
+        return np.dtype(np.float32) if dt == np.complex64 else np.dtype(np.float64)
+    return dt
+
+
 class OLayout(C.Structure):
     _fields_ = [
         ("ndim", C.c_size_t),
@@ -140,10 +153,7 @@ class Oracle:
                                       C.byref(out.c))
 
     def reduce_all(self, op, a):
-        outdt = np.uint64 if op in (3, 4) else a.base.dtype
-        if op in (7, 8, 9) and np.issubdtype(a.base.dtype, np.complexfloating):
-            outdt = np.float32 if a.base.dtype == np.complex64 else np.float64
-        out = np.zeros(1, dtype=outdt)
+        out = np.zeros(1, dtype=red_out_dtype(op, a.base.dtype))
         r = self.lib.o_reduce_all(op, CODES[a.base.dtype], a.ptr, C.byref(a.c), out.ctypes.data_as(C.c_void_p))
         if r != 0:
             raise ValueError("oracle reduce_all error %d" % r)
