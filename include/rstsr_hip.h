@@ -226,6 +226,14 @@ int rstsr_hip_op_mutc_refa_numb(rstsr_hip_ctx_t* ctx, rstsr_binop_t op, rstsr_dt
 /* c = scalar a (op) b. */
 int rstsr_hip_op_mutc_numa_refb(rstsr_hip_ctx_t* ctx, rstsr_binop_t op, rstsr_dtype_t t, void* c,
                                 const rstsr_layout_t* lc, const void* a, const void* b, const rstsr_layout_t* lb);
+/* c = a (op) b with one operand complex and the other its real type: the reference's
+ * OpAddAPI/OpSubAPI/OpMulAPI/OpDivAPI<TA, TB, TC> with TA != TB (auto_impl/op_ternary_arithmetic.rs:3-14
+ * over num-complex 0.4's `Complex<T> op T` and `T op Complex<T>`): tc is c64 or c128, (ta, tb) is
+ * (tc, its real type) or (the real type, tc). Either of la / lb may be NULL: that operand is one
+ * host value (op_mutc_refa_numb / op_mutc_numa_refb). Layouts already broadcast to lc's shape. */
+int rstsr_hip_op_mutc_mixed(rstsr_hip_ctx_t* ctx, rstsr_binop_t op, rstsr_dtype_t tc, void* c,
+                            const rstsr_layout_t* lc, rstsr_dtype_t ta, const void* a, const rstsr_layout_t* la,
+                            rstsr_dtype_t tb, const void* b, const rstsr_layout_t* lb);
 /* c (op)= b  (OpAddAssignAPI etc.). */
 int rstsr_hip_op_muta_refb(rstsr_hip_ctx_t* ctx, rstsr_binop_t op, rstsr_dtype_t t, void* c,
                            const rstsr_layout_t* lc, const void* b, const rstsr_layout_t* lb);

@@ -238,6 +238,66 @@ int o_binary(int op, int t, void* c, const olayout* lc, const void* a, const ola
     return 0;
 }
 
+/* c = a op b with one operand complex and the other its real type (num-complex 0.4's
+ * `Complex<T> op T` and `T op Complex<T>` behind auto_impl/op_ternary_arithmetic.rs:3-14):
+ * Complex op t = (re + t, im), (re - t, im), (re t, im t), (re / t, im / t);
+ * t op Complex = (t + re, im), (t - re, 0 - im), (t re, t im), (t c / n, (-t) d / n), n = c c + d d.
+ * lb == NULL: b is one value. */
+static void mixed1(int op, int tc, int a_cplx, char* c, const char* a, const char* b) {
+    if (tc == C128) {
+        double re, im, t;
+        double* o = (double*)c;
+        if (a_cplx) { re = ((const double*)a)[0]; im = ((const double*)a)[1]; t = *(const double*)b; }
+        else { re = ((const double*)b)[0]; im = ((const double*)b)[1]; t = *(const double*)a; }
+        if (a_cplx) {
+            if (op == OP_ADD) { o[0] = re + t; o[1] = im; }
+            else if (op == OP_SUB) { o[0] = re - t; o[1] = im; }
+            else if (op == OP_MUL) { o[0] = re * t; o[1] = im * t; }
+            else { o[0] = re / t; o[1] = im / t; }
+        } else {
+            if (op == OP_ADD) { o[0] = t + re; o[1] = im; }
+            else if (op == OP_SUB) { o[0] = t - re; o[1] = 0.0 - im; }
+            else if (op == OP_MUL) { o[0] = t * re; o[1] = t * im; }
+            else { double n = re * re + im * im; o[0] = t * re / n; o[1] = -t * im / n; }
+        }
+    } else {
+        float re, im, t;
+        float* o = (float*)c;
+        if (a_cplx) { re = ((const float*)a)[0]; im = ((const float*)a)[1]; t = *(const float*)b; }
+        else { re = ((const float*)b)[0]; im = ((const float*)b)[1]; t = *(const float*)a; }
+        if (a_cplx) {
+            if (op == OP_ADD) { o[0] = re + t; o[1] = im; }
+            else if (op == OP_SUB) { o[0] = re - t; o[1] = im; }
+            else if (op == OP_MUL) { o[0] = re * t; o[1] = im * t; }
+            else { o[0] = re / t; o[1] = im / t; }
+        } else {
+            if (op == OP_ADD) { o[0] = t + re; o[1] = im; }
+            else if (op == OP_SUB) { o[0] = t - re; o[1] = 0.0f - im; }
+            else if (op == OP_MUL) { o[0] = t * re; o[1] = t * im; }
+            else { float n = re * re + im * im; o[0] = t * re / n; o[1] = -t * im / n; }
+        }
+    }
+}
+
+int o_binary_mixed(int op, int tc, void* c, const olayout* lc, int ta, const void* a, const olayout* la, int tb,
+                   const void* b, const olayout* lb) {
+    size_t n = lsize(lc), ec = esize(tc), ea = esize(ta), eb = esize(tb);
+    int a_cplx = ta == tc;
+    if (n == 0) return 0;
+    odo_t oc, oa, ob;
+    odo_init(&oc, lc);
+    odo_init(&oa, la);
+    if (lb) odo_init(&ob, lb);
+    for (size_t i = 0; i < n; ++i) {
+        const char* pb = lb ? (const char*)b + ob.off * (ptrdiff_t)eb : (const char*)b;
+        mixed1(op, tc, a_cplx, (char*)c + oc.off * (ptrdiff_t)ec, (const char*)a + oa.off * (ptrdiff_t)ea, pb);
+        odo_next(&oc, lc);
+        odo_next(&oa, la);
+        if (lb) odo_next(&ob, lb);
+    }
+    return 0;
+}
+
 static void cast_elem(int tc, void* c, int ta, const void* a) {
     double re = 0, im = 0;
     int64_t iv = 0;
@@ -540,6 +600,77 @@ int o_reduce_all(int op, int t, const void* a, const olayout* la, void* out) {
     if (lsize(la) == 0 && (op == R_MAX || op == R_MIN || op == R_ARGMAX || op == R_ARGMIN)) return 2;
     reduce_one(op, t, a, la, out);
     return 0;
+}
+
+/* allclose_all (feature_rayon/auto_impl/reduction.rs:725-750): every element pair isclose
+ * (rstsr-dtype-traits/src/isclose.rs:92-106): diff = a.ext_abs_diff(b) and |b| = b.ext_abs() in
+ * the element type, both then cast to f64; close when diff <= atol + rtol * |b|, or when
+ * equal_nan and both are NaN. ext_abs_diff (ext_num.rs:76-214): floats |a - b| in the float
+ * type; signed integers (a >= b ? a - b : b - a) in the integer type (wrapping, as Rust release
+ * builds); complex |a - b| (hypot of the parts); ext_abs: floats fabs, integers a.abs()
+ * (wrapping: MIN stays MIN), complex hypot. A NaN diff (inf - inf) compares false. la and lb
+ * already broadcast to one shape. Returns 1 (close), 0 (not close), -1 (zero-size: an error in
+ * the reference). */
+static int o_isclose1(int t, const char* pa, const char* pb, double rtol, double atol, int equal_nan) {
+    double diff = 0.0, ab = 0.0;
+    int nan_a = 0, nan_b = 0;
+    switch (t) {
+        case F64: {
+            double x = *(const double*)pa, y = *(const double*)pb;
+            diff = fabs(x - y); ab = fabs(y); nan_a = isnan(x); nan_b = isnan(y);
+            break;
+        }
+        case F32: {
+            float x = *(const float*)pa, y = *(const float*)pb;
+            diff = (double)fabsf(x - y); ab = (double)fabsf(y); nan_a = isnan(x); nan_b = isnan(y);
+            break;
+        }
+        case I32: {
+            int32_t x = *(const int32_t*)pa, y = *(const int32_t*)pb;
+            int32_t d = x >= y ? (int32_t)((uint32_t)x - (uint32_t)y) : (int32_t)((uint32_t)y - (uint32_t)x);
+            int32_t m = y < 0 ? (int32_t)(0u - (uint32_t)y) : y;
+            diff = (double)d; ab = (double)m;
+            break;
+        }
+        case I64: {
+            int64_t x = *(const int64_t*)pa, y = *(const int64_t*)pb;
+            int64_t d = x >= y ? (int64_t)((uint64_t)x - (uint64_t)y) : (int64_t)((uint64_t)y - (uint64_t)x);
+            int64_t m = y < 0 ? (int64_t)(0ull - (uint64_t)y) : y;
+            diff = (double)d; ab = (double)m;
+            break;
+        }
+        case C128: {
+            const double* x = (const double*)pa; const double* y = (const double*)pb;
+            diff = hypot(x[0] - y[0], x[1] - y[1]); ab = hypot(y[0], y[1]);
+            nan_a = isnan(x[0]) || isnan(x[1]); nan_b = isnan(y[0]) || isnan(y[1]);
+            break;
+        }
+        case C64: {
+            const float* x = (const float*)pa; const float* y = (const float*)pb;
+            diff = (double)hypotf(x[0] - y[0], x[1] - y[1]); ab = (double)hypotf(y[0], y[1]);
+            nan_a = isnan(x[0]) || isnan(x[1]); nan_b = isnan(y[0]) || isnan(y[1]);
+            break;
+        }
+    }
+    int comp = diff <= atol + rtol * ab;
+    return comp || (equal_nan && nan_a && nan_b);
+}
+
+int o_allclose(int t, const void* a, const olayout* la, const void* b, const olayout* lb, double rtol, double atol,
+               int equal_nan) {
+    size_t n = lsize(la), es = esize(t);
+    if (n == 0 || lsize(lb) == 0) return -1;
+    odo_t oa, ob;
+    odo_init(&oa, la);
+    odo_init(&ob, lb);
+    int all = 1;
+    for (size_t i = 0; i < n; ++i) {
+        all = all && o_isclose1(t, (const char*)a + oa.off * (ptrdiff_t)es, (const char*)b + ob.off * (ptrdiff_t)es,
+                                rtol, atol, equal_nan);
+        odo_next(&oa, la);
+        odo_next(&ob, lb);
+    }
+    return all;
 }
 
 /* gemm_naive_cpu_serial (cpu_serial/matmul_naive.rs:78-122): c = beta*c, then for k in order

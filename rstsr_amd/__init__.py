@@ -104,6 +104,12 @@ def _axes(axes, ndim):
     return [int(x) for x in axes]
 
 
+def _mixed_pair(a, b):
+    """A complex dtype and its real type, in either order."""
+    a, b = np.dtype(a), np.dtype(b)
+    return (a.kind == "c" and _REAL_OF.get(a) == b) or (b.kind == "c" and _REAL_OF.get(b) == a)
+
+
 class Tensor:
     """A strided view of device storage (rstsr-core/src/tensorbase.rs:5-24).
 
@@ -382,7 +388,18 @@ class Tensor:
     # ---------------- arithmetic ----------------
     def _bin(self, op, other, left=False):
         if isinstance(other, Tensor):
-            return Tensor(self._t.binary(op, other._t) if not left else other._t.binary(op, self._t))
+            x, y = (other, self) if left else (self, other)
+            if np.dtype(x.dtype) != np.dtype(y.dtype) and _mixed_pair(x.dtype, y.dtype):
+                return Tensor(x._t.binary_mixed(op, y._t))  # Complex<T> op T / T op Complex<T>
+            return Tensor(x._t.binary(op, y._t))
+        dt = np.dtype(self.dtype)
+        if dt.kind == "c" and isinstance(other, (float, int, np.floating, np.integer)) and op <= OP_DIV:
+            # a real scalar with a complex tensor: num-complex's Complex<T> op T
+            rt_ = _REAL_OF[dt]
+            return Tensor(self._t.binary_scalar_mixed(op, np.array(other, dtype=rt_).tobytes(), _code(rt_), left))
+        if dt.kind == "f" and isinstance(other, (complex, np.complexfloating)) and op <= OP_DIV:
+            ct = np.dtype(np.complex128 if dt == np.float64 else np.complex64)
+            return Tensor(self._t.binary_scalar_mixed(op, np.array(other, dtype=ct).tobytes(), _code(ct), left))
         s = np.array(other, dtype=self.dtype).tobytes()
         return Tensor(self._t.binary_scalar(op, s, left))
 

@@ -50,6 +50,7 @@ def lib():
     sig = {
         "rstsr_hip_op_mutc_refa_refb": [vp, C.c_int, C.c_int, vp, P, vp, P, vp, P],
         "rstsr_hip_assign": [vp, C.c_int, vp, P, C.c_int, vp, P],
+        "rstsr_hip_op_mutc_mixed": [vp, C.c_int, C.c_int, vp, P, C.c_int, vp, P, C.c_int, vp, P],
         "rstsr_hip_assign_arbitary": [vp, C.c_int, vp, P, C.c_int, vp, P],
         "rstsr_hip_reduce_axes": [vp, C.c_int, C.c_int, vp, P, C.c_size_t, C.POINTER(C.c_ssize_t), vp, P],
         "rstsr_hip_reduce_axes_noalloc": [vp, C.c_int, C.c_int, vp, P, C.c_size_t, C.POINTER(C.c_ssize_t), vp, P],

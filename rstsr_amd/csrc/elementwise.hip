@@ -147,6 +147,27 @@ constexpr bool op_defined() {
     else return integ;
 }
 
+// num-complex 0.4's operators between a complex number and its real type:
+//   Complex op t: (re + t, im), (re - t, im), (re t, im t), (re / t, im / t);
+//   t op Complex: (t + re, im), (t - re, 0 - im), (t re, t im), (t c / n, (-t) d / n), n = c c + d d.
+template <int OP, class C, class R>
+__device__ __forceinline__ C mix_cr(C a, R t) {
+    if constexpr (OP == RSTSR_OP_ADD) return C{a.re + t, a.im};
+    if constexpr (OP == RSTSR_OP_SUB) return C{a.re - t, a.im};
+    if constexpr (OP == RSTSR_OP_MUL) return C{a.re * t, a.im * t};
+    if constexpr (OP == RSTSR_OP_DIV) return C{a.re / t, a.im / t};
+}
+template <int OP, class C, class R>
+__device__ __forceinline__ C mix_rc(R t, C b) {
+    if constexpr (OP == RSTSR_OP_ADD) return C{t + b.re, b.im};
+    if constexpr (OP == RSTSR_OP_SUB) return C{t - b.re, (R)0 - b.im};
+    if constexpr (OP == RSTSR_OP_MUL) return C{t * b.re, t * b.im};
+    if constexpr (OP == RSTSR_OP_DIV) {
+        R n = b.re * b.re + b.im * b.im;
+        return C{t * b.re / n, -t * b.im / n};
+    }
+}
+
 // value conversion for assign with promotion (Rust `as`-free DTypeCastAPI subset)
 template <class TC, class TA>
 __device__ __forceinline__ TC cast_to(TA a) {
@@ -185,6 +206,30 @@ struct BinBody {
     __device__ __forceinline__ void st(int64_t o, T x, T y) const { sto<NT>(c + o, apply_op<OP, T>(x, y)); }
 };
 
+// c = a op b with one operand complex (C) and the other its real type; AS / BS: host scalar.
+template <int OP, class C, class TAv, class TBv, bool AS, bool BS>
+struct MixBody {
+    using TA = TAv;
+    using TB = TBv;
+    C* c;
+    const TAv* a;
+    const TBv* b;
+    TAv sa;
+    TBv sb;
+    static constexpr int OPS = (AS ? 0 : 1) | (BS ? 0 : 2);
+    static constexpr size_t OUT_SIZE = sizeof(C);
+    static constexpr bool NT_JOINT = false;
+    template <bool NT = false>
+    __device__ __forceinline__ TAv la(int64_t o) const { return AS ? sa : ld<NT>(a + o); }
+    template <bool NT = false>
+    __device__ __forceinline__ TBv lb(int64_t o) const { return BS ? sb : ld<NT>(b + o); }
+    template <bool NT = false>
+    __device__ __forceinline__ void st(int64_t o, TAv x, TBv y) const {
+        if constexpr (std::is_same<TAv, C>::value) sto<NT>(c + o, mix_cr<OP, C>(x, y));
+        else sto<NT>(c + o, mix_rc<OP, C>(x, y));
+    }
+};
+
 template <class TCv, class TAv>
 struct CastBody {
     using TA = TAv;
@@ -219,18 +264,26 @@ struct FillBody {
     __device__ __forceinline__ void st(int64_t o, None, None) const { sto<NT>(c + o, v); }
 };
 
+// ExtNum::ext_abs_diff / ext_abs (rstsr-dtype-traits/src/ext_num.rs:72-214) in the element
+// type, then as f64: floats |a - b|; signed integers (a >= b ? a - b : b - a) and a.abs()
+// wrapping like Rust release builds (MIN.abs() stays MIN); complex the norm (hypot).
 template <class T>
 __device__ __forceinline__ double abs_diff_d(T a, T b) {
     if constexpr (std::is_same<T, c128>::value) return hypot(a.re - b.re, a.im - b.im);
     else if constexpr (std::is_same<T, c64>::value) return (double)hypotf(a.re - b.re, a.im - b.im);
-    else if constexpr (std::is_integral<T>::value) return fabs((double)a - (double)b);
-    else return (double)fabs(a - b);
+    else if constexpr (std::is_integral<T>::value) {
+        using U = typename std::make_unsigned<T>::type;
+        return (double)(a >= b ? (T)((U)a - (U)b) : (T)((U)b - (U)a));
+    } else return (double)fabs(a - b);
 }
 template <class T>
 __device__ __forceinline__ double abs_d(T a) {
     if constexpr (std::is_same<T, c128>::value) return hypot(a.re, a.im);
     else if constexpr (std::is_same<T, c64>::value) return (double)hypotf(a.re, a.im);
-    else return fabs((double)a);
+    else if constexpr (std::is_integral<T>::value) {
+        using U = typename std::make_unsigned<T>::type;
+        return (double)(a < 0 ? (T)((U)0 - (U)a) : a);
+    } else return (double)fabs(a);
 }
 template <class T>
 __device__ __forceinline__ bool is_nan_v(T a) {
@@ -432,6 +485,61 @@ using namespace rstsr::hip;
 
 extern "C" {
 
+// Mixed complex / real operands; a or b may be a host scalar (null layout).
+static void mixed_impl(rstsr_hip_ctx* ctx, rstsr_binop_t op, rstsr_dtype_t tc, void* c, const rstsr_layout_t* lc_,
+                       rstsr_dtype_t ta, const void* a, const rstsr_layout_t* la_, rstsr_dtype_t tb, const void* b,
+                       const rstsr_layout_t* lb_) {
+    bind(ctx);
+    if ((int)op < 0 || (int)op > 3) RSTSR_RAISE(UnImplemented, "mixed complex/real operands take add, sub, mul, div");
+    const bool is128 = tc == RSTSR_C128, is64 = tc == RSTSR_C64;
+    const rstsr_dtype_t tr = is128 ? RSTSR_F64 : RSTSR_F32;
+    if (!(is128 || is64) || !((ta == tc && tb == tr) || (ta == tr && tb == tc)))
+        RSTSR_RAISE(UnImplemented, "mixed operands are a complex dtype and its real type, with a complex output");
+    Layout lc = from_abi(lc_);
+    const bool has_a = la_ != nullptr, has_b = lb_ != nullptr;
+    if (!has_a && !has_b) RSTSR_RAISE(InvalidValue, "at least one operand must be an array");
+    Layout la = has_a ? from_abi(la_) : lc, lb = has_b ? from_abi(lb_) : lc;
+    if (la.shape != lc.shape || lb.shape != lc.shape)
+        RSTSR_RAISE(InvalidLayout, "operands of an elementwise op must be broadcast to one shape before the device call");
+    if (!a || !b) RSTSR_RAISE(InvalidValue, "null operand");
+    check_extent(c, lc, dtype_size(tc), "c");
+    if (has_a) check_extent(a, la, dtype_size(ta), "a");
+    if (has_b) check_extent(b, lb, dtype_size(tb), "b");
+    const Stride* s[3] = {&lc.stride, has_a ? &la.stride : nullptr, has_b ? &lb.stride : nullptr};
+    size_t off[3] = {lc.offset, has_a ? la.offset : 0, has_b ? lb.offset : 0};
+    Dims d = collapse3(lc.shape, s, off);
+    EwChoice ch = choose(d, has_a, has_b);
+    auto by_types = [&](auto ctag, auto atag, auto btag) {
+        using C = decltype(ctag);
+        using TA = decltype(atag);
+        using TB = decltype(btag);
+        auto run = [&](auto opc) {
+            constexpr int OP = decltype(opc)::value;
+            TA sa{};
+            TB sb{};
+            if (!has_a) std::memcpy(&sa, a, sizeof(TA));
+            if (!has_b) std::memcpy(&sb, b, sizeof(TB));
+            auto go = [&](auto body) { launch_ew<decltype(body), false, false>(ctx, body, d, ch); };
+            if (has_a && has_b) go(MixBody<OP, C, TA, TB, false, false>{(C*)c, (const TA*)a, (const TB*)b, sa, sb});
+            else if (has_a) go(MixBody<OP, C, TA, TB, false, true>{(C*)c, (const TA*)a, nullptr, sa, sb});
+            else go(MixBody<OP, C, TA, TB, true, false>{(C*)c, nullptr, (const TB*)b, sa, sb});
+        };
+        switch (op) {
+            case RSTSR_OP_ADD: run(std::integral_constant<int, RSTSR_OP_ADD>{}); break;
+            case RSTSR_OP_SUB: run(std::integral_constant<int, RSTSR_OP_SUB>{}); break;
+            case RSTSR_OP_MUL: run(std::integral_constant<int, RSTSR_OP_MUL>{}); break;
+            default: run(std::integral_constant<int, RSTSR_OP_DIV>{}); break;
+        }
+    };
+    if (is128) {
+        if (ta == tc) by_types(c128{}, c128{}, double{});
+        else by_types(c128{}, double{}, c128{});
+    } else {
+        if (ta == tc) by_types(c64{}, c64{}, float{});
+        else by_types(c64{}, float{}, c64{});
+    }
+}
+
 int rstsr_hip_op_mutc_refa_refb(rstsr_hip_ctx_t* ctx, rstsr_binop_t op, rstsr_dtype_t t, void* c,
                                 const rstsr_layout_t* lc, const void* a, const rstsr_layout_t* la, const void* b,
                                 const rstsr_layout_t* lb) {
@@ -455,6 +563,12 @@ int rstsr_hip_op_mutc_numa_refb(rstsr_hip_ctx_t* ctx, rstsr_binop_t op, rstsr_dt
         if (!lb) RSTSR_RAISE(InvalidValue, "null layout");
         binary_impl(ctx, op, t, c, lc, a, nullptr, b, lb);
     });
+}
+
+int rstsr_hip_op_mutc_mixed(rstsr_hip_ctx_t* ctx, rstsr_binop_t op, rstsr_dtype_t tc, void* c,
+                            const rstsr_layout_t* lc, rstsr_dtype_t ta, const void* a, const rstsr_layout_t* la,
+                            rstsr_dtype_t tb, const void* b, const rstsr_layout_t* lb) {
+    return guarded([&] { mixed_impl(ctx, op, tc, c, lc, ta, a, la, tb, b, lb); });
 }
 
 int rstsr_hip_op_muta_refb(rstsr_hip_ctx_t* ctx, rstsr_binop_t op, rstsr_dtype_t t, void* c,

@@ -418,8 +418,37 @@ static void launch_ew(rstsr_hip_ctx* ctx, const Body& body, Dims d, EwChoice ch,
         d.st.push_back({0, 0, 0});
     }
     const uint64_t LIM = 1ull << 31;
-    for (auto n : d.shape)
-        if (n >= LIM) RSTSR_RAISE(UnImplemented, "an axis of 2^31 or more elements after merging is not supported");
+    // Kernels index one launch with 32-bit arithmetic: an axis of 2^31 or more elements, or
+    // inner axes (all but the outermost) holding 2^30 or more, are cut into pieces of 2^30
+    // along that axis, each launched on its own with the operands' offsets advanced.
+    {
+        int cut = -1;
+        for (int i = 0; i < d.nd; ++i)
+            if (d.shape[i] >= LIM && (cut < 0 || d.shape[i] > d.shape[cut])) cut = i;
+        if (cut < 0 && d.nd > 1) {
+            uint64_t inner = 1;
+            for (int i = 0; i + 1 < d.nd; ++i) inner *= d.shape[i];
+            if (inner >= LIM / 2) {
+                for (int i = 0; i + 1 < d.nd; ++i)
+                    if (cut < 0 || d.shape[i] > d.shape[cut]) cut = i;
+            }
+        }
+        if (cut >= 0) {
+            const uint64_t n = d.shape[cut];
+            uint64_t rest = 1;
+            for (int i = 0; i < d.nd; ++i)
+                if (i != cut && i + 1 < d.nd) rest *= d.shape[i];
+            // pieces short enough that the inner product stays under 2^30
+            const uint64_t step = std::max<uint64_t>(1, std::min<uint64_t>(1ull << 30, (LIM / 2) / std::max<uint64_t>(rest, 1)));
+            for (uint64_t s0 = 0; s0 < n; s0 += step) {
+                Dims dd = d;
+                dd.shape[cut] = std::min(step, n - s0);
+                for (int k = 0; k < 3; ++k) dd.off[k] += (int64_t)s0 * d.st[cut][k];
+                launch_ew<Body, TILED, NTV>(ctx, body, dd, ch, false);
+            }
+            return;
+        }
+    }
     uint64_t all = 1;
     for (auto n : d.shape) all *= n;
     bool once[3] = {true, true, true};

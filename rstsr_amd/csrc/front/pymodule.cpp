@@ -93,6 +93,13 @@ PYBIND11_MODULE(_rstsr, m) {
              py::arg("order") = "C")
         .def("astype", [](const Tensor& t, int code) { return t.astype((rstsr_dtype_t)code); })
         .def("binary", [](const Tensor& a, int op, const Tensor& b) { return a.binary((rstsr_binop_t)op, b); })
+        .def("binary_mixed",
+             [](const Tensor& a, int op, const Tensor& b) { return a.binary_mixed((rstsr_binop_t)op, b); })
+        .def("binary_scalar_mixed",
+             [](const Tensor& a, int op, py::bytes v, int ts, bool left) {
+                 std::string s = v;
+                 return a.binary_scalar_mixed((rstsr_binop_t)op, s.data(), (rstsr_dtype_t)ts, left);
+             })
         .def("binary_scalar",
              [](const Tensor& a, int op, py::bytes s, bool left) {
                  std::string v = s;

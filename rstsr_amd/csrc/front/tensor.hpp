@@ -200,6 +200,32 @@ class Tensor {
         check(rstsr_hip_op_mutc_refa_refb(device().raw(), op, dtype, c.ptr(), &xc, ptr(), &xa, b.ptr(), &xb));
         return c;
     }
+    // A complex and a real operand (Complex<T> op T, T op Complex<T>): a complex result laid out
+    // as for same-type operands.
+    Tensor binary_mixed(rstsr_binop_t op, const Tensor& b) const {
+        device().order_after(b.device());
+        FlagOrder ord = device().default_order();
+        auto bl = broadcast_layout(layout, b.layout, ord);
+        Layout lc = get_layout_for_binary_op(bl.first, bl.second, ord);
+        rstsr_dtype_t tc = (dtype == RSTSR_C64 || dtype == RSTSR_C128) ? dtype : b.dtype;
+        Tensor c = uninit(device(), lc, tc);
+        AbiLayout xc(lc), xa(bl.first), xb(bl.second);
+        check(rstsr_hip_op_mutc_mixed(device().raw(), op, tc, c.ptr(), &xc, dtype, ptr(), &xa, b.dtype, b.ptr(), &xb));
+        return c;
+    }
+    // tensor op scalar (or scalar op tensor) with the scalar of dtype ts: complex tensor and real
+    // scalar, or real tensor and complex scalar
+    Tensor binary_scalar_mixed(rstsr_binop_t op, const void* s, rstsr_dtype_t ts, bool scalar_left) const {
+        Layout lc = layout_for_array_copy(layout, IterOrder::K);
+        rstsr_dtype_t tc = (dtype == RSTSR_C64 || dtype == RSTSR_C128) ? dtype : ts;
+        Tensor c = uninit(device(), lc, tc);
+        AbiLayout xc(lc), xa(layout);
+        if (scalar_left)
+            check(rstsr_hip_op_mutc_mixed(device().raw(), op, tc, c.ptr(), &xc, ts, s, nullptr, dtype, ptr(), &xa));
+        else
+            check(rstsr_hip_op_mutc_mixed(device().raw(), op, tc, c.ptr(), &xc, dtype, ptr(), &xa, ts, s, nullptr));
+        return c;
+    }
     Tensor binary_scalar(rstsr_binop_t op, const void* s, bool scalar_left) const {
         Layout lc = layout_for_array_copy(layout, IterOrder::K);
         Tensor c = uninit(device(), lc, dtype);
@@ -289,7 +315,9 @@ class Tensor {
     bool allclose(const Tensor& b, double rtol, double atol, bool equal_nan) const {
         if (dtype != b.dtype) RSTSR_RAISE(InvalidValue, "operands must share one dtype");
         device().order_after(b.device());
-        AbiLayout xa(layout), xb(b.layout);
+        // allclose_all_f (tensor/reduction.rs:324-351) broadcasts in the device's default order
+        auto bl = broadcast_layout(layout, b.layout, device().default_order());
+        AbiLayout xa(bl.first), xb(bl.second);
         int ok = 0;
         check(rstsr_hip_allclose(device().raw(), dtype, ptr(), &xa, b.ptr(), &xb, rtol, atol, equal_nan ? 1 : 0, &ok));
         return ok != 0;

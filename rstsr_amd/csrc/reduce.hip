@@ -720,6 +720,23 @@ static RedSetup reduce_plan(rstsr_hip_ctx* ctx, const Layout& la, const std::vec
         rm.push_back(d);
     }
     if (rm.empty()) rm.push_back({1, 0, 0});
+    // The kernels walk reduced dim 0 with 32-bit positions: an extent of 2^31 or more is
+    // factored into n = f * g (f < 2^31 the inner dim, g the next; logical strides chain, so
+    // the logical index order is unchanged). An extent without such a factor pair stays
+    // unsupported.
+    if (rm[0].n >= (1ull << 31)) {
+        const uint64_t n = rm[0].n;
+        uint64_t f = 0;
+        for (uint64_t g = 2; g * g <= n && g < (1ull << 31); ++g)
+            if (n % g == 0 && n / g < (1ull << 31)) {
+                f = n / g;
+                break;
+            }
+        if (f == 0) RSTSR_RAISE(UnImplemented, "a reduced extent of 2^31 or more without a factor below 2^31");
+        const RD d0 = rm[0];
+        rm[0] = {f, d0.in, d0.lg};
+        rm.insert(rm.begin() + 1, RD{n / f, d0.in * (int64_t)f, d0.lg * (int64_t)f});
+    }
     // kept dims: sort by |in stride| then |out stride|; merge when both chain
     struct KD {
         uint64_t n;
@@ -1061,6 +1078,29 @@ static void reduce_axes_impl(rstsr_hip_ctx* ctx, rstsr_redop_t op, rstsr_dtype_t
         if (op == RSTSR_RED_MAX || op == RSTSR_RED_MIN)
             RSTSR_RAISE(InvalidValue, "zero-size array is not supported for max/min");
         RSTSR_RAISE(InvalidLayout, "empty sequence is not allowed for reduce_arg.");
+    }
+    // Launches address at most 2^31 - 1 outputs: 2^31 or more are reduced in pieces of the
+    // largest kept axis (2^30 positions each), input and output views narrowed alike.
+    if (lo.size() >= (1ull << 31)) {
+        auto ax = normalize_axes_index(axes, la.ndim(), false);
+        std::vector<size_t> kept;
+        for (size_t i = 0; i < la.ndim(); ++i)
+            if (std::find(ax.begin(), ax.end(), (ptrdiff_t)i) == ax.end()) kept.push_back(i);
+        size_t j = 0;  // position in lo (the kept axes in order) of the largest kept axis
+        for (size_t q = 1; q < kept.size(); ++q)
+            if (la.shape[kept[q]] > la.shape[kept[j]]) j = q;
+        const size_t ia = kept[j];
+        const uint64_t n = la.shape[ia], step = 1ull << 30;
+        for (uint64_t s0 = 0; s0 < n; s0 += step) {
+            const uint64_t cnt = std::min(step, n - s0);
+            Layout la2 = la, lo2 = lo;
+            la2.shape[ia] = cnt;
+            lo2.shape[j] = cnt;
+            la2.offset = (size_t)((int64_t)la.offset + (int64_t)s0 * la.stride[ia]);
+            lo2.offset = (size_t)((int64_t)lo.offset + (int64_t)s0 * lo.stride[j]);
+            reduce_axes_impl(ctx, op, t, a, la2, axes, out, lo2, fixed);
+        }
+        return;
     }
     RedSetup s = reduce_plan(ctx, la, axes, lo, t);
     if (s.empty_out) return;

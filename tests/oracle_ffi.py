@@ -118,6 +118,7 @@ class Oracle:
         vp = C.c_void_p
         self.lib.o_binary.argtypes = [C.c_int, C.c_int, vp, L, vp, L, vp, L]
         self.lib.o_assign.argtypes = [C.c_int, vp, L, C.c_int, vp, L]
+        self.lib.o_binary_mixed.argtypes = [C.c_int, C.c_int, vp, L, C.c_int, vp, L, C.c_int, vp, L]
         self.lib.o_reduce_axes.argtypes = [C.c_int, C.c_int, vp, L, C.c_size_t, C.POINTER(C.c_ssize_t), vp, L]
         self.lib.o_reduce_all.argtypes = [C.c_int, C.c_int, vp, L, vp]
         self.lib.o_gemm_naive.argtypes = [C.c_int, C.c_size_t, C.c_size_t, C.c_size_t, vp, vp, C.c_ssize_t,
@@ -151,6 +152,24 @@ class Oracle:
         ax = (C.c_ssize_t * max(len(axes), 1))(*axes)
         return self.lib.o_reduce_axes(op, CODES[a.base.dtype], a.ptr, C.byref(a.c), len(axes), ax, out.ptr,
                                       C.byref(out.c))
+
+    def binary_mixed(self, op, c, a, b):
+        """o_binary_mixed: c (complex View) = a op b, one of a / b complex and the other its real type;
+        b may be a numpy scalar (one host value)."""
+        if isinstance(b, View):
+            return self.lib.o_binary_mixed(op, CODES[c.base.dtype], c.ptr, C.byref(c.c), CODES[a.base.dtype], a.ptr,
+                                           C.byref(a.c), CODES[b.base.dtype], b.ptr, C.byref(b.c))
+        sb = np.array(b)
+        return self.lib.o_binary_mixed(op, CODES[c.base.dtype], c.ptr, C.byref(c.c), CODES[a.base.dtype], a.ptr,
+                                       C.byref(a.c), CODES[sb.dtype], sb.ctypes.data_as(C.c_void_p), None)
+
+    def allclose(self, a, b, rtol=1e-5, atol=1e-8, equal_nan=False):
+        """o_allclose on two Views already broadcast to one shape: True/False, None for zero size."""
+        L = self.lib
+        L.o_allclose.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_double,
+                                 C.c_int]
+        r = L.o_allclose(CODES[a.base.dtype], a.ptr, C.byref(a.c), b.ptr, C.byref(b.c), rtol, atol, int(equal_nan))
+        return None if r < 0 else bool(r)
 
     def reduce_all(self, op, a):
         out = np.zeros(1, dtype=red_out_dtype(op, a.base.dtype))

@@ -156,3 +156,58 @@ def test_planted_mistakes_fail(oracle):
     assert not np.array_equal(sr, (a.view(np.uint32) >> 1).view(np.int32))
     sl = o_bin(oracle, SHL, np.array([1], dtype=np.int32), np.array([33], dtype=np.int32))
     assert sl[0] == 2  # 33 mod 32 = 1, not 0 (an unmasked 1 << 33 overflows to 0)
+
+
+# ---------------- mixed complex / real operands (num-complex Complex<T> op T, T op Complex<T>) ----------------
+
+ADD, SUB, MUL = 0, 1, 2
+
+
+def o_mixed(oracle, op, a, b, tc):
+    a, b = np.broadcast_arrays(a, b)
+    vc = c_view(a.shape, tc)
+    oracle.binary_mixed(op, vc, View.of(a), View.of(b))
+    return vc.array()
+
+
+@pytest.mark.parametrize("tc,tr", [(np.complex128, np.float64), (np.complex64, np.float32)])
+def test_mixed_add_sub_mul_match_python(oracle, tc, tr):
+    """Python's complex-with-float operators treat the float as (t, 0) and the results are exact
+    componentwise for finite values: (re + t, im), (t - re, 0 - im), (re t, im t) ... here in
+    float64, cast to the pair's precision for complex64 inputs (one rounding each)."""
+    rng = np.random.default_rng(53)
+    z = (rng.normal(size=200) + 1j * rng.normal(size=200)).astype(tc)
+    t = rng.normal(size=200).astype(tr)
+    for op, f in ((ADD, lambda x, y: x + y), (SUB, lambda x, y: x - y), (MUL, lambda x, y: x * y)):
+        got_cr = o_mixed(oracle, op, z, t, tc)
+        got_rc = o_mixed(oracle, op, t, z, tc)
+        for zi, ti, g1, g2 in zip(z, t, got_cr, got_rc):
+            r1 = f(complex(zi), float(ti))
+            r2 = f(float(ti), complex(zi))
+            assert g1 == np.array(r1).astype(tc) and g2 == np.array(r2).astype(tc)
+
+
+def test_mixed_div_exact_cases(oracle):
+    """t / z on exact cases: 25 / (3 + 4i) = 3 - 4i, 10 / (1 + 3i) = 1 - 3i; z / t = (re/t, im/t)."""
+    z = np.array([3 + 4j, 1 + 3j, -6 + 8j])
+    t = np.array([25.0, 10.0, 50.0])
+    got = o_mixed(oracle, DIV, t, z, np.complex128)
+    assert got.tolist() == [3 - 4j, 1 - 3j, -3 - 4j]
+    got2 = o_mixed(oracle, DIV, z, np.array([2.0, 0.5, -4.0]), np.complex128)
+    assert got2.tolist() == [1.5 + 2j, 2 + 6j, 1.5 - 2j]
+
+
+def test_mixed_div_close_to_python(oracle):
+    rng = np.random.default_rng(59)
+    z = rng.normal(size=300) + 1j * rng.normal(size=300)
+    t = rng.normal(size=300)
+    got = o_mixed(oracle, DIV, t, z, np.complex128)
+    ref = np.array([float(x) / complex(y) for x, y in zip(t, z)])
+    assert np.all(np.abs(got - ref) <= 8 * 2.0 ** -53 * np.abs(ref))
+
+
+def test_mixed_sub_negative_zero(oracle):
+    """t - z keeps (0 - im): for im = +0 the imaginary part is +0, not -0 (a plain negation would
+    give -0), so a planted -im fails here."""
+    got = o_mixed(oracle, SUB, np.array([1.0]), np.array([2.0 + 0.0j]), np.complex128)
+    assert np.signbit(got[0].imag) == False  # noqa: E712
