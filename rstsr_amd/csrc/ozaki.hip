@@ -181,22 +181,41 @@ struct ModSet {
     int n;
     double m[OZ_MAXMOD];
     double inv[OZ_MAXMOD];
+    uint32_t mi[OZ_MAXMOD];     // m
+    uint32_t magic[OZ_MAXMOD];  // ceil(2^32 / m)
+    uint32_t clo[OZ_MAXMOD];    // bytes 256^k mod m, k = 0..3
+    uint32_t chi[OZ_MAXMOD];    // bytes 256^k mod m, k = 4..6 (byte 3 zero)
 };
 
-// symmetric residue of an integer-valued double x (|x| < 2^60) modulo m <= 256
-__device__ __forceinline__ int sym_residue(double x, double m, double inv) {
-    double q = floor(x * inv);
-    double r = fma(-q, m, x);  // exact: the true value is a small integer
-    if (r < 0.0) r += m;
-    if (r >= m) r -= m;
-    int ri = (int)r;
-    return ri > 127 ? ri - (int)m : ri;
+// The split integers x (|x| <= 2^55: ||x||_2 <= 2^g, g = 55) as 7 magnitude bytes in two words.
+struct Digits {
+    uint32_t lo, hi;
+    bool neg;
+};
+__device__ __forceinline__ Digits to_digits(double x) {
+    const double ax = fabs(x);
+    const double h = floor(ax * 0x1p-32);  // < 2^24
+    const double l = fma(-h, 0x1p32, ax);  // exact, in [0, 2^32)
+    return {(uint32_t)l, (uint32_t)h, x < 0.0};
 }
 
-__device__ __forceinline__ uint32_t pack_residues(const double* x, double m, double inv) {
+// Symmetric residue (in [-128, 127]) of the integer with digits d modulo m = ms.mi[l]:
+// |x| = sum_k b_k 256^k, so |x| mod m = (sum_k b_k (256^k mod m)) mod m, the sum (< 7 * 255^2
+// < 2^19) taken with two byte dot products; its quotient by m is exact from the high word of
+// s * ceil(2^32 / m) (the approximation's error s (ceil(2^32/m) m - 2^32) / 2^32 < 2^-5 stays
+// below the 1/m a non-multiple keeps from the next integer). The sign follows x.
+__device__ __forceinline__ int sym_residue_d(const Digits& d, const ModSet& ms, int l) {
+    const uint32_t s = __builtin_amdgcn_udot4(d.hi, ms.chi[l], __builtin_amdgcn_udot4(d.lo, ms.clo[l], 0u, false), false);
+    const uint32_t m = ms.mi[l];
+    int r = (int)(s - __umulhi(s, ms.magic[l]) * m);
+    if (d.neg && r != 0) r = (int)m - r;
+    return r > 127 ? r - (int)m : r;
+}
+
+__device__ __forceinline__ uint32_t pack_residues_d(const Digits* d, const ModSet& ms, int l) {
     uint32_t w = 0;
 #pragma unroll
-    for (int i = 0; i < 4; ++i) w |= (uint32_t)(uint8_t)(int8_t)sym_residue(x[i], m, inv) << (8 * i);
+    for (int i = 0; i < 4; ++i) w |= (uint32_t)(uint8_t)(int8_t)sym_residue_d(d[i], ms, l) << (8 * i);
     return w;
 }
 
@@ -223,17 +242,16 @@ __device__ __forceinline__ void split_tile(const double* __restrict__ s, int64_t
     const int vv = tid >> 2, tq = (tid & 3) * 16;
     const int64_t gv = v0 + vv;
     if (gv >= nvp) return;
-    double x[16];
+    Digits dg[16];
 #pragma unroll
-    for (int i = 0; i < 16; ++i) x[i] = tile[vv][tq + i];
+    for (int i = 0; i < 16; ++i) dg[i] = to_digits(tile[vv][tq + i]);
     int8_t* dst = planes + gv * lenp + t0 + tq;
     for (int l = 0; l < ms.n; ++l) {
-        const double m = ms.m[l], inv = ms.inv[l];
         uint4 w;
-        w.x = pack_residues(x, m, inv);
-        w.y = pack_residues(x + 4, m, inv);
-        w.z = pack_residues(x + 8, m, inv);
-        w.w = pack_residues(x + 12, m, inv);
+        w.x = pack_residues_d(dg, ms, l);
+        w.y = pack_residues_d(dg + 4, ms, l);
+        w.z = pack_residues_d(dg + 8, ms, l);
+        w.w = pack_residues_d(dg + 12, ms, l);
         *(uint4*)(dst + l * plane) = w;
     }
 }
@@ -252,6 +270,65 @@ __global__ void __launch_bounds__(256) k_split(const double* __restrict__ s, int
     split_tile(s + (which ? ob : oa), nv, len, sv, st, ev + (int64_t)b * nvp, nvp, lenp,
                planes + (int64_t)b * nvp * lenp, ms, read_along_t != 0, plane, (int64_t)blockIdx.y * 64,
                (int64_t)blockIdx.x * 64, tile);
+}
+
+// Norm and split fused, for vectors whose elements are contiguous (st = 1): one wave per
+// vector v < nvp of batch b = blockIdx.y. Pass 1 computes the scale exponent (ev[b*nvp + v])
+// exactly as k_norm_rows does; pass 2 re-reads the vector (from cache) and writes its 16-byte
+// residue runs to every plane like split_tile. Padding rows v >= nv and positions t >= len
+// are zero.
+__global__ void __launch_bounds__(256) k_normsplit_rows(const double* __restrict__ x, int64_t nv, int64_t len,
+                                                        int64_t sv, BatchDesc bd, int which, int64_t nvp,
+                                                        int64_t lenp, int g, int* __restrict__ ev,
+                                                        int8_t* __restrict__ planes, int64_t plane, ModSet ms,
+                                                        int* __restrict__ flag) {
+    const int lane = threadIdx.x & 63;
+    const int64_t v = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const uint32_t b = blockIdx.y;
+    if (v >= nvp) return;  // uniform across the wave
+    int8_t* pv = planes + ((int64_t)b * nvp + v) * lenp;
+    if (v >= nv) {
+        for (int64_t t0 = (int64_t)lane * 16; t0 < lenp; t0 += 1024)
+            for (int l = 0; l < ms.n; ++l) *(uint4*)(pv + l * plane + t0) = uint4{0, 0, 0, 0};
+        return;
+    }
+    int64_t oa, ob, oc;
+    batch_offsets(bd, b, oa, ob, oc);
+    const double* xv = x + (which ? ob : oa) + v * sv;
+    NormAcc a{0.0, 0, 0.0};
+    bool bad = false;
+    // the elements of k_norm_rows's lane walk in the same order, four loads in flight
+    int64_t t = lane;
+    for (; t + 192 < len; t += 256) {
+        const double x0 = xv[t], x1 = xv[t + 64], x2 = xv[t + 128], x3 = xv[t + 192];
+        norm_add(a, x0, bad);
+        norm_add(a, x1, bad);
+        norm_add(a, x2, bad);
+        norm_add(a, x3, bad);
+    }
+    for (; t < len; t += 64) norm_add(a, xv[t], bad);
+    a = wave_norm(a);
+    if (bad) atomicOr(flag, 1);
+    const int e = scale_exponent(a, g);
+    if (lane == 0) ev[(int64_t)b * nvp + v] = e;
+    for (int64_t t0 = (int64_t)lane * 16; t0 < lenp; t0 += 1024) {
+        Digits dg[16];
+        if (t0 + 16 <= len) {
+#pragma unroll
+            for (int i = 0; i < 16; ++i) dg[i] = to_digits(rint(ldexp(xv[t0 + i], e)));
+        } else {
+#pragma unroll
+            for (int i = 0; i < 16; ++i) dg[i] = to_digits(t0 + i < len ? rint(ldexp(xv[t0 + i], e)) : 0.0);
+        }
+        for (int l = 0; l < ms.n; ++l) {
+            uint4 w;
+            w.x = pack_residues_d(dg, ms, l);
+            w.y = pack_residues_d(dg + 4, ms, l);
+            w.z = pack_residues_d(dg + 8, ms, l);
+            w.w = pack_residues_d(dg + 12, ms, l);
+            *(uint4*)(pv + l * plane + t0) = w;
+        }
+    }
 }
 
 /* ---------------- INT8 MFMA GEMM per modulus ---------------- */
@@ -508,22 +585,24 @@ __device__ __forceinline__ double crt_value(const uint8_t* __restrict__ p, int64
 // R planes: [l][batch][Mp][N] (plane stride pl); eA [nbA][Mp], fB [nbB][Np].
 // C = alpha 2^-(eA_i + fB_j) CRT(R) + beta C: columns along x (V consecutive
 // per thread, read as one V-byte word per plane), rows along y, batches along z.
-template <int V>
+template <int V>  // columns per thread: 1, 4 (one 4-byte word per plane) or 16 (one 16-byte word)
 __global__ void __launch_bounds__(256) k_crt(const uint8_t* __restrict__ R, int64_t M, int64_t N, int64_t Mp,
                                              int64_t Np, int64_t pl, int a_batched, int b_batched, BatchDesc bd,
                                              CrtConst cc, const int* __restrict__ eA, const int* __restrict__ fB,
                                              double* C, int64_t rsc, int64_t csc, double alpha, double beta,
-                                             int beta_zero, const int* __restrict__ skip, int lower) {
-    static_assert(V == 1 || V == 4, "word size");
+                                             int beta_zero, const int* __restrict__ skip, int lower, int tpr) {
+    static_assert(V == 1 || V == 4 || V == 16, "word size");
     if (*skip) return;
-    const int64_t j0 = ((int64_t)blockIdx.x * 256 + threadIdx.x) * V;
+    // tpr threads (a power of two <= 256) per row, 256 / tpr rows of a block at a time
+    const int rpb = 256 / tpr;
+    const int64_t j0 = ((int64_t)blockIdx.x * tpr + (threadIdx.x & (tpr - 1))) * V;
     if (j0 >= N) return;
     const uint32_t b = blockIdx.z;
     int64_t oa, ob, oc;
     batch_offsets(bd, b, oa, ob, oc);
     const int* eAb = eA + (a_batched ? (int64_t)b * Mp : 0);
     const int* fBb = fB + (b_batched ? (int64_t)b * Np : 0);
-    for (int64_t i = blockIdx.y; i < M; i += gridDim.y) {
+    for (int64_t i = (int64_t)blockIdx.y * rpb + threadIdx.x / tpr; i < M; i += (int64_t)gridDim.y * rpb) {
         const uint8_t* rp = R + ((int64_t)b * Mp + i) * N + j0;
         double* cp = C + oc + i * rsc + j0 * csc;
         const int ei = eAb[i];
@@ -532,15 +611,27 @@ __global__ void __launch_bounds__(256) k_crt(const uint8_t* __restrict__ R, int6
             const double v = ldexp(crt_value(rp, pl, cc), -(ei + fBb[j0]));
             *cp = beta_zero ? alpha * v : alpha * v + beta * (*cp);
         } else {
-            uint32_t w[OZ_MAXMOD];
+            constexpr int NW = V / 4;  // 4-byte words per plane
+            uint32_t w[OZ_MAXMOD][NW];
 #pragma unroll
-            for (int l = 0; l < OZ_MAXMOD; ++l) w[l] = l < cc.n ? *(const uint32_t*)(rp + l * pl) : 0u;
+            for (int l = 0; l < OZ_MAXMOD; ++l) {
+                if constexpr (NW == 1) {
+                    w[l][0] = l < cc.n ? *(const uint32_t*)(rp + l * pl) : 0u;
+                } else {
+                    const uint4 q = l < cc.n ? *(const uint4*)(rp + l * pl) : uint4{0, 0, 0, 0};
+                    w[l][0] = q.x;
+                    w[l][1] = q.y;
+                    w[l][2] = q.z;
+                    w[l][3] = q.w;
+                }
+            }
 #pragma unroll
-            for (int k = 0; k < 4; ++k) {
+            for (int k = 0; k < V; ++k) {
                 uint32_t r[OZ_MAXMOD];
 #pragma unroll
-                for (int l = 0; l < OZ_MAXMOD; ++l) r[l] = (w[l] >> (8 * k)) & 255u;
+                for (int l = 0; l < OZ_MAXMOD; ++l) r[l] = (w[l][k / 4] >> (8 * (k % 4))) & 255u;
                 if (lower && j0 + k > i) break;
+                if (j0 + k >= N) break;
                 const double v = ldexp(crt_from(r, cc), -(ei + fBb[j0 + k]));
                 double* ck = cp + k * csc;
                 *ck = beta_zero ? alpha * v : alpha * v + beta * (*ck);
@@ -780,6 +871,17 @@ static OzSetup ozaki_setup() {
     for (int l = 0; l < su.n; ++l) {
         su.ms.m[l] = (double)su.cc.mod[l];
         su.ms.inv[l] = 1.0 / (double)su.cc.mod[l];
+        const uint32_t m = su.cc.mod[l];
+        su.ms.mi[l] = m;
+        su.ms.magic[l] = (uint32_t)(((1ull << 32) + m - 1) / m);
+        uint32_t pw = 1 % m, lo = 0, hi = 0;  // 256^k mod m
+        for (int k = 0; k < 7; ++k) {
+            if (k < 4) lo |= pw << (8 * k);
+            else hi |= pw << (8 * (k - 4));
+            pw = (pw * 256) % m;
+        }
+        su.ms.clo[l] = lo;
+        su.ms.chi[l] = hi;
     }
     return su;
 }
@@ -908,18 +1010,25 @@ void ozaki_gemm_f64(rstsr_hip_ctx* ctx, const GemmProblem& P, double alpha, doub
         }
         check_launch("ozaki norms");
     };
-    run_norm(qa, eA);
-    hipLaunchKernelGGL(k_split, dim3((uint32_t)(Kp / 64), (uint32_t)(Mp / 64), (uint32_t)nbA), dim3(256), 0,
-                       ctx->stream, (const double*)P.a, M, K, P.rsa, P.csa, P.bd, 0, eA, Mp, Kp, A8, ms,
-                       std::llabs(P.csa) == 1 ? 1 : 0, flag, (int64_t)planeA);
-    check_launch("ozaki split A");
-    if (!sym) {
-        run_norm(qb, fB);  // reuses the partials buffer after the A norms (stream order)
-        hipLaunchKernelGGL(k_split, dim3((uint32_t)(Kp / 64), (uint32_t)(Np / 64), (uint32_t)nbB), dim3(256), 0,
-                           ctx->stream, (const double*)P.b, N, K, P.csb, P.rsb, P.bd, 1, fB, Np, Kp, B8, ms,
-                           std::llabs(P.rsb) == 1 ? 1 : 0, flag, (int64_t)planeB);
-        check_launch("ozaki split B");
-    }
+    // vectors with contiguous elements: norm and split in one pass (k_normsplit_rows)
+    auto norm_split = [&](const NormPlan& q, const double* x, int64_t nv, int64_t nvp, int64_t sv, int64_t st,
+                          int which, int* ev, int8_t* planes, size_t plane, const char* what) {
+        if (st == 1) {
+            hipLaunchKernelGGL(k_normsplit_rows, dim3(ceil_div_u32(nvp, 4), (uint32_t)q.nbx), dim3(256), 0,
+                               ctx->stream, x, nv, K, sv, P.bd, which, nvp, Kp, g, ev, planes, (int64_t)plane, ms,
+                               flag);
+            check_launch(what);
+            return;
+        }
+        run_norm(q, ev);
+        hipLaunchKernelGGL(k_split, dim3((uint32_t)(Kp / 64), (uint32_t)(nvp / 64), (uint32_t)q.nbx), dim3(256), 0,
+                           ctx->stream, x, nv, K, sv, st, P.bd, which, ev, nvp, Kp, planes, ms,
+                           std::llabs(st) == 1 ? 1 : 0, flag, (int64_t)plane);
+        check_launch(what);
+    };
+    norm_split(qa, (const double*)P.a, M, Mp, P.rsa, P.csa, 0, eA, A8, planeA, "ozaki split A");
+    if (!sym)  // the B norms reuse the partials buffer after the A norms (stream order)
+        norm_split(qb, (const double*)P.b, N, Np, P.csb, P.rsb, 1, fB, B8, planeB, "ozaki split B");
     // Non-finite inputs set *flag: the split kernels exit at once and the plain
     // FP64 kernel (which runs only when *flag != 0) computes C with IEEE special
     // values. No host round trip, so the sequence can be graph-captured.
@@ -948,12 +1057,19 @@ void ozaki_gemm_f64(rstsr_hip_ctx* ctx, const GemmProblem& P, double alpha, doub
     // four columns per thread when every row and plane of R starts on a 4-byte boundary
     auto crt = [&](auto v) {
         constexpr int V = decltype(v)::value;
-        // four rows per block (rows i, i + gridDim.y, ...): fewer, longer blocks
-        hipLaunchKernelGGL((k_crt<V>), dim3(ceil_div_u32(N, 256 * V), (uint32_t)std::min<int64_t>((M + 3) / 4, 65535),
-                                            (uint32_t)nb),
-                           dim3(256), 0, ctx->stream, R, M, N, Mp, Np, (int64_t)planeR, a_b ? 1 : 0, b_b ? 1 : 0, P.bd,
-                           cc, eA, fB, (double*)P.c, P.rsc, P.csc, alpha, beta, beta_zero ? 1 : 0, flag, sym ? 1 : 0);
+        // a row's column groups over tpr threads (no idle lanes for short rows); each block
+        // walks about four row groups (rows i, i + gridDim.y * rpb, ...): fewer, longer blocks
+        const int64_t ncg = (N + V - 1) / V;
+        int tpr = 1;
+        while (tpr < 256 && tpr < ncg) tpr *= 2;
+        const int64_t rpb = 256 / tpr;
+        const int64_t gy = std::min<int64_t>(std::max<int64_t>(1, (M + 4 * rpb - 1) / (4 * rpb)), 65535);
+        hipLaunchKernelGGL((k_crt<V>), dim3(ceil_div_u32(ncg, tpr), (uint32_t)gy, (uint32_t)nb), dim3(256), 0,
+                           ctx->stream, R, M, N, Mp, Np, (int64_t)planeR, a_b ? 1 : 0, b_b ? 1 : 0, P.bd, cc, eA, fB,
+                           (double*)P.c, P.rsc, P.csc, alpha, beta, beta_zero ? 1 : 0, flag, sym ? 1 : 0, tpr);
     };
+    // R rows are N bytes long: 16-byte words need N % 16 == 0 (planes start 256-byte aligned)
+    // (16 columns per thread measured slower: each lane's 16 doubles land 128 bytes apart)
     if (N % 4 == 0) crt(std::integral_constant<int, 4>{});
     else crt(std::integral_constant<int, 1>{});
     check_launch("ozaki crt");
