@@ -22,7 +22,7 @@
 //     f64 and scaled by 2^-(e_i + f_j); then alpha/beta.
 // The only rounding errors are the integer roundings of step 1 (relative
 // 2^-g per element against the vector norm) and the final conversion, so the
-// error is set by g: with 14 moduli M ~ 2^111.5 and g = 55.
+// error is set by g: with 14 moduli M ~ 2^110.2 and g = 54.
 //
 // Planes live in the context workspace: 14 x (M + N) x K int8 + 14 x M x N
 // uint8 (2.7 GB at 8192^3).
@@ -38,6 +38,7 @@ namespace rstsr {
 namespace hip {
 
 typedef int v4i __attribute__((ext_vector_type(4)));
+typedef int v16i __attribute__((ext_vector_type(16)));
 typedef unsigned __int128 u128;
 typedef __int128 i128;
 
@@ -50,8 +51,8 @@ static const uint32_t OZ_MODULI[16] = {256, 255, 253, 251, 247, 241, 239, 233, 2
 struct CrtConst {
     int n;
     uint32_t mod[OZ_MAXMOD];
-    uint32_t w32[OZ_MAXMOD][4];
-    uint32_t M32[4];
+    double wl[OZ_MAXMOD][3];  // w_l = wl[0] 2^76 + wl[1] 2^38 + wl[2], limbs < 2^38
+    double Ml[3];             // M in the same limbs
     u128 M;
     double Md, invM;
 };
@@ -187,7 +188,7 @@ struct ModSet {
     uint32_t chi[OZ_MAXMOD];    // bytes 256^k mod m, k = 4..6 (byte 3 zero)
 };
 
-// The split integers x (|x| <= 2^55: ||x||_2 <= 2^g, g = 55) as 7 magnitude bytes in two words.
+// The split integers x (|x| <= ||x||_2 <= 2^g, g <= 55) as 7 magnitude bytes in two words.
 struct Digits {
     uint32_t lo, hi;
     bool neg;
@@ -336,26 +337,37 @@ __global__ void __launch_bounds__(256) k_normsplit_rows(const double* __restrict
 // INT8 tile configurations: a BM x BN block tile from WVM x WVN waves, K slices
 // of BK bytes double-buffered in LDS (rows padded by PAD bytes against bank
 // conflicts of the 16-byte fragment reads).
-template <int BM_, int BN_, int BK_, int WVM_, int WVN_, int OCC_, int PAD_>
+// FR: the MFMA's output block, 16 (v_mfma_i32_16x16x64_i8) or 32 (v_mfma_i32_32x32x32_i8).
+// D: K slices whose global loads are in flight at once (register sets).
+template <int BM_, int BN_, int BK_, int WVM_, int WVN_, int OCC_, int PAD_, int FR_ = 16, int D_ = 1>
 struct I8Cfg {
-    static constexpr int BM = BM_, BN = BN_, BK = BK_, WVM = WVM_, WVN = WVN_, OCC = OCC_, PAD = PAD_;
+    static constexpr int BM = BM_, BN = BN_, BK = BK_, WVM = WVM_, WVN = WVN_, OCC = OCC_, PAD = PAD_, FR = FR_,
+                         D = D_;
     static constexpr int NT = 64 * WVM * WVN;
 };
-// large products: 64 x 128 per wave
-using I8Wide = I8Cfg<128, 256, 64, 2, 2, 1, 32>;
-// large products, eight waves of 64 x 64, two blocks per CU
+// products more than 128 wide: eight waves of 64 x 64, two blocks per CU
 using I8Wide8 = I8Cfg<128, 256, 64, 2, 4, 2, 32>;
-// the square tile with 32-byte row padding
-using I8Sq32 = I8Cfg<128, 128, 128, 2, 2, 2, 32>;
 // small and grouped products: 64 x 64 per wave, two blocks per CU
 using I8Sq = I8Cfg<128, 128, 128, 2, 2, 2, 16>;
-// planes: rows padded to I8_MPAD (every BM divides it), K to I8_KPAD (every BK divides it)
+// large products: 256 x 256 block tiles from eight waves of 64 x 128 (a third fewer LDS
+// fragment bytes per MFMA than 64 x 64 waves), 128-byte K slices in XOR-swizzled rows, one
+// block per CU
+using I8Big = I8Cfg<256, 256, 128, 4, 2, 1, 0>;
+// planes: rows padded to a multiple of the tile's BM (I8_MPAD for all but the 256-row tiles),
+// K to I8_KPAD (every BK divides it)
 constexpr int I8_MPAD = 128, I8_KPAD = 128;
 // one launch sums at most I8_KCHUNK products per output: |sum| <= 2^16 * 2^14 < 2^31
 constexpr int64_t I8_KCHUNK = 65536;
-static_assert(I8Wide::BM == I8_MPAD && I8Wide8::BM == I8_MPAD && I8Sq::BM == I8_MPAD && I8Sq32::BM == I8_MPAD,
-              "tiles_m = Mp / I8_MPAD");
-static_assert(I8Wide::BN == 256 && I8Wide8::BN == 256 && I8Sq::BN == 128 && I8Sq32::BN == 128, "column padding");
+static_assert(I8Sq::BM == I8_MPAD && I8Sq::BN == 128, "grouped problems pad to the square tile");
+
+// x mod m in [0, m) for |x| <= 2^30 and 193 <= m <= 256, magic = ceil(2^32 / m): u = x + m 2^23
+// lies in [0, 2^32); umulhi(u, magic) is floor(u / m) or one more (u magic / 2^32 exceeds u / m by
+// u (magic m - 2^32) / (m 2^32) < 1), so u - q m lies in [-m, m) and one correction ends it.
+__device__ __forceinline__ uint32_t mod_u32(int x, uint32_t m, uint32_t magic) {
+    const uint32_t u = (uint32_t)x + (m << 23);
+    const int r = (int)(u - __umulhi(u, magic) * m);
+    return (uint32_t)(r < 0 ? r + (int)m : r);
+}
 
 // x mod m in [0, m) for |x| < 2^31, m <= 256, inv = 1/m: x * inv is within
 // 2^-21 of x / m, so its floor is the quotient or (x a multiple of m rounding
@@ -378,18 +390,32 @@ struct I8Smem {
 // bytes: A8 [>= (tm+1) BM][ldk] and B8 [>= (tn+1) BN][ldk] (K contiguous, zero
 // padded); R [M][ldr] gets the uint8 residues of (A8 B8^T) mod `mod` for rows
 // < M and columns < N, added to the residues already there when `accum`. Each
-// wave owns (BM / WVM) x (BN / WVN) outputs from v_mfma_i32_16x16x64_i8 tiles;
-// global loads for slice k+1 are in flight while slice k is multiplied. The
-// accumulators see nothing but MFMAs until the epilogue, so they stay in AGPRs.
-template <class CF, bool ROW4G>
+// wave owns (BM / WVM) x (BN / WVN) outputs from FR x FR MFMA blocks
+// (v_mfma_i32_16x16x64_i8 or v_mfma_i32_32x32x32_i8). K runs in slices of BK
+// bytes through two LDS buffers; global loads run CF::D slices ahead in
+// registers (D register sets), so a load has D slices of MFMA work to arrive.
+// The accumulators see nothing but MFMAs until the epilogue, so they stay in
+// AGPRs.
+// TOUT: each product block is formed transposed (B fragment as the MFMA's row
+// operand), so a lane ends with runs of four consecutive columns of one row of
+// R: one 4-byte store per run (N % 4 == 0), the residue by mod_u32. The 32 x 32
+// blocks always use it.
+template <class CF, bool ROW4G, bool TOUT = false>
 __device__ __forceinline__ void i8_tile(const int8_t* __restrict__ A, const int8_t* __restrict__ B,
                                         uint8_t* __restrict__ R, int64_t M, int64_t N, int64_t ldk, int64_t Kc,
-                                        int64_t ldr, int mod, bool accum, uint32_t tm, uint32_t tn, I8Smem<CF>& sm) {
-    constexpr int BM = CF::BM, BN = CF::BN, BK = CF::BK, NT = CF::NT;
+                                        int64_t ldr, int mod, bool accum, uint32_t tm, uint32_t tn, I8Smem<CF>& sm,
+                                        uint32_t magic = 0) {
+    constexpr int BM = CF::BM, BN = CF::BN, BK = CF::BK, NT = CF::NT, FR = CF::FR, D = CF::D;
+    constexpr bool T = TOUT || FR == 32;
+    constexpr int FK = FR == 16 ? 64 : 32;  // bytes of K per MFMA
+    constexpr int NREG = FR == 16 ? 4 : 16;  // output registers per lane
+    using Acc = std::conditional_t<FR == 16, v4i, v16i>;
     constexpr int WTM = BM / CF::WVM, WTN = BN / CF::WVN;
-    constexpr int FM = WTM / 16, FN = WTN / 16;
+    constexpr int FM = WTM / FR, FN = WTN / FR;
     constexpr int CPR = BK / 16, RPP = NT / CPR, LA = BM / RPP, LB = BN / RPP;
-    static_assert(BM % RPP == 0 && BN % RPP == 0 && BK % 64 == 0 && I8_KPAD % BK == 0, "tile shape");
+    static_assert(BM % RPP == 0 && BN % RPP == 0 && BK % FK == 0 && I8_KPAD % BK == 0 && FM > 0 && FN > 0 &&
+                      (FR == 16 || FR == 32) && D >= 1,
+                  "tile shape");
     auto& As = sm.As;
     auto& Bs = sm.Bs;
     const int8_t* Ab = A + (size_t)tm * BM * ldk;
@@ -398,75 +424,138 @@ __device__ __forceinline__ void i8_tile(const int8_t* __restrict__ A, const int8
     const int wm = wave / CF::WVN, wn = wave % CF::WVN;
     const int lr = tid / CPR, lc = (tid % CPR) * 16;  // loader: CPR threads x 16 B per row, RPP rows per pass
 
-    v4i ra[LA], rb[LB];
-    auto load = [&](int64_t k0) {
+    v4i ra[D][LA], rb[D][LB];
+    auto load = [&](int set, int64_t k0) {
 #pragma unroll
-        for (int i = 0; i < LA; ++i) ra[i] = *(const v4i*)(Ab + (size_t)(lr + RPP * i) * ldk + k0 + lc);
+        for (int i = 0; i < LA; ++i) ra[set][i] = *(const v4i*)(Ab + (size_t)(lr + RPP * i) * ldk + k0 + lc);
 #pragma unroll
-        for (int i = 0; i < LB; ++i) rb[i] = *(const v4i*)(Bb + (size_t)(lr + RPP * i) * ldk + k0 + lc);
+        for (int i = 0; i < LB; ++i) rb[set][i] = *(const v4i*)(Bb + (size_t)(lr + RPP * i) * ldk + k0 + lc);
     };
-    auto store = [&](int buf) {
+    // PAD == 0: rows unpadded, 16-byte chunk c of row r stored at chunk c ^ swz(r), which puts
+    // any 8 consecutive rows' chunk c (a fragment read by 8 lanes) and any 2 rows' 4 chunks (a
+    // loader store by 8 lanes) in 8 distinct 16-byte bank groups of 128 bytes
+    static_assert(CF::PAD != 0 || BK == 64 || BK == 128, "swizzle for 64 or 128-byte rows");
+    auto swz = [](int r) { return CF::PAD != 0 ? 0 : (BK == 64 ? (r >> 1) & 3 : r & 7); };
+    const int lcs = ((lc >> 4) ^ swz(lr)) << 4;  // RPP is a multiple of 8: one swizzle for all passes
+    auto store = [&](int set, int buf) {
 #pragma unroll
-        for (int i = 0; i < LA; ++i) *(v4i*)&As[buf][lr + RPP * i][lc] = ra[i];
+        for (int i = 0; i < LA; ++i) *(v4i*)&As[buf][lr + RPP * i][lcs] = ra[set][i];
 #pragma unroll
-        for (int i = 0; i < LB; ++i) *(v4i*)&Bs[buf][lr + RPP * i][lc] = rb[i];
+        for (int i = 0; i < LB; ++i) *(v4i*)&Bs[buf][lr + RPP * i][lcs] = rb[set][i];
     };
-    v4i acc[FM][FN];
-#pragma unroll
-    for (int i = 0; i < FM; ++i)
-#pragma unroll
-        for (int j = 0; j < FN; ++j) acc[i][j] = v4i{0, 0, 0, 0};
-
-    load(0);
-    store(0);
-    __syncthreads();
-    const int fr = lane & 15, fk = (lane >> 4) * 16;
-    int buf = 0;
-    for (int64_t k0 = 0; k0 < Kc; k0 += BK) {
-        const bool more = k0 + BK < Kc;
-        if (more) load(k0 + BK);
-#pragma unroll
-        for (int kk = 0; kk < BK; kk += 64) {
-            v4i af[FM], bf[FN];
-#pragma unroll
-            for (int i = 0; i < FM; ++i) af[i] = *(const v4i*)&As[buf][wm * WTM + i * 16 + fr][kk + fk];
-#pragma unroll
-            for (int j = 0; j < FN; ++j) bf[j] = *(const v4i*)&Bs[buf][wn * WTN + j * 16 + fr][kk + fk];
-#pragma unroll
-            for (int i = 0; i < FM; ++i)
-#pragma unroll
-                for (int j = 0; j < FN; ++j)
-                    acc[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(af[i], bf[j], acc[i][j], 0, 0, 0);
-        }
-        if (more) store(buf ^ 1);
-        __syncthreads();
-        buf ^= 1;
-    }
-    const double inv = 1.0 / (double)mod;
+    Acc acc[FM][FN];
 #pragma unroll
     for (int i = 0; i < FM; ++i)
 #pragma unroll
         for (int j = 0; j < FN; ++j)
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int lrow = ROW4G ? (4 * (lane >> 4) + r) : (4 * r + (lane >> 4));
-                int64_t gm = (int64_t)tm * BM + wm * WTM + i * 16 + lrow;
-                int64_t gn = (int64_t)tn * BN + wn * WTN + j * 16 + (lane & 15);
-                if (gm < M && gn < N) {
-                    int v = mod_small(acc[i][j][r], mod, inv);
-                    if (accum) {
-                        v += R[gm * ldr + gn];
-                        if (v >= mod) v -= mod;
-                    }
-                    R[gm * ldr + gn] = (uint8_t)v;
+            for (int r = 0; r < NREG; ++r) acc[i][j][r] = 0;
+
+    const int fr = lane % FR, fk = (lane / FR) * 16;
+    static_assert(RPP % 8 == 0 && FR % 8 == 0 && WTM % 8 == 0 && WTN % 8 == 0, "swizzle period");
+    const int sr = swz(fr);  // fragment rows start at multiples of 8
+    auto compute = [&](int buf) {
+#pragma unroll
+        for (int kk = 0; kk < BK; kk += FK) {
+            v4i af[FM], bf[FN];
+            const int kc = (((kk + fk) >> 4) ^ sr) << 4;
+#pragma unroll
+            for (int i = 0; i < FM; ++i) af[i] = *(const v4i*)&As[buf][wm * WTM + i * FR + fr][kc];
+#pragma unroll
+            for (int j = 0; j < FN; ++j) bf[j] = *(const v4i*)&Bs[buf][wn * WTN + j * FR + fr][kc];
+#pragma unroll
+            for (int i = 0; i < FM; ++i)
+#pragma unroll
+                for (int j = 0; j < FN; ++j) {
+                    if constexpr (FR == 32)
+                        acc[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(bf[j], af[i], acc[i][j], 0, 0, 0);
+                    else if constexpr (T)
+                        acc[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(bf[j], af[i], acc[i][j], 0, 0, 0);
+                    else
+                        acc[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(af[i], bf[j], acc[i][j], 0, 0, 0);
                 }
+        }
+    };
+
+    // slices 0 .. D-1 in flight, slice 0 in LDS; slice t uses register set t % D and LDS buffer t % 2
+#pragma unroll
+    for (int s = 0; s < D; ++s)
+        if ((int64_t)s * BK < Kc) load(s, (int64_t)s * BK);
+    store(0, 0);
+    __syncthreads();
+    int buf = 0;
+    for (int64_t kb = 0; kb < Kc; kb += (int64_t)D * BK) {
+#pragma unroll
+        for (int s = 0; s < D; ++s) {
+            const int64_t k0 = kb + (int64_t)s * BK;
+            if (k0 < Kc) {
+                if (k0 + (int64_t)D * BK < Kc) load(s, k0 + (int64_t)D * BK);
+                compute(buf);
+                if (k0 + BK < Kc) store((s + 1) % D, buf ^ 1);
+                __syncthreads();
+                buf ^= 1;
             }
+        }
+    }
+    if constexpr (T) {
+        static_assert(ROW4G, "transposed output assumes rows 4 (lane / 16) + r of the 16 x 16 MFMA result");
+        // block (i, j), register quad q: lane holds R[row lane % FR][4 consecutive columns from cq]
+        const bool vec = !accum && (ldr % 4) == 0;
+#pragma unroll
+        for (int i = 0; i < FM; ++i)
+#pragma unroll
+            for (int j = 0; j < FN; ++j)
+#pragma unroll
+                for (int q = 0; q < NREG / 4; ++q) {
+                    const int cq = FR == 16 ? 4 * (lane >> 4) : 8 * q + 4 * (lane >> 5);
+                    const int64_t gm = (int64_t)tm * BM + wm * WTM + i * FR + (lane % FR);
+                    const int64_t gn = (int64_t)tn * BN + wn * WTN + j * FR + cq;
+                    if (gm >= M) continue;
+                    uint32_t w = 0;
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) w |= mod_u32(acc[i][j][4 * q + r], (uint32_t)mod, magic) << (8 * r);
+                    uint8_t* rp = R + gm * ldr + gn;
+                    if (vec && gn + 4 <= N) {
+                        *(uint32_t*)rp = w;
+                    } else {
+                        for (int r = 0; r < 4 && gn + r < N; ++r) {
+                            uint32_t v = (w >> (8 * r)) & 255u;
+                            if (accum) {
+                                v += rp[r];
+                                if (v >= (uint32_t)mod) v -= mod;
+                            }
+                            rp[r] = (uint8_t)v;
+                        }
+                    }
+                }
+    } else {
+        const double inv = 1.0 / (double)mod;
+#pragma unroll
+        for (int i = 0; i < FM; ++i)
+#pragma unroll
+            for (int j = 0; j < FN; ++j)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int lrow = ROW4G ? (4 * (lane >> 4) + r) : (4 * r + (lane >> 4));
+                    int64_t gm = (int64_t)tm * BM + wm * WTM + i * 16 + lrow;
+                    int64_t gn = (int64_t)tn * BN + wn * WTN + j * 16 + (lane & 15);
+                    if (gm < M && gn < N) {
+                        int v = mod_small(acc[i][j][r], mod, inv);
+                        if (accum) {
+                            v += R[gm * ldr + gn];
+                            if (v >= mod) v -= mod;
+                        }
+                        R[gm * ldr + gn] = (uint8_t)v;
+                    }
+                }
+    }
 }
 
 // Batches on blockIdx.y: plane offsets sA / sB (0 for a broadcast operand) and sR.
 // Moduli of one launch (blockIdx.z selects one)
 struct ModList {
     int m[OZ_MAXMOD];
+    uint32_t magic[OZ_MAXMOD];  // ceil(2^32 / m)
 };
 
 // Tiles touching the lower triangle of a square product with BN = R BM: tile row tm
@@ -507,7 +596,7 @@ __device__ __forceinline__ void lower_tile(uint32_t t, uint32_t R, uint32_t tile
     tn = (uint32_t)(t - lower_tile_prefix(r, R));
 }
 
-template <class CF, bool ROW4G>
+template <class CF, bool ROW4G, bool TOUT>
 __global__ void __launch_bounds__(CF::NT, CF::OCC) k_i8gemm_mod(const int8_t* __restrict__ A,
                                                                 const int8_t* __restrict__ B, uint8_t* __restrict__ R,
                                                                 int64_t M, int64_t N, int64_t ldk, int64_t Kc,
@@ -528,8 +617,9 @@ __global__ void __launch_bounds__(CF::NT, CF::OCC) k_i8gemm_mod(const int8_t* __
     // each XCD one contiguous range of them (an equal share of the work)
     if (lower) lower_tile(xcd_chunk(blockIdx.x, gridDim.x), CF::BN / CF::BM, tiles_m, tm, tn);
     else tile_coords<4>(blockIdx.x, tiles_m, tiles_n, tm, tn);
-    i8_tile<CF, ROW4G>(A + (int64_t)blockIdx.y * sA, B + (int64_t)blockIdx.y * sB, R + (int64_t)blockIdx.y * sR, M, N,
-                       ldk, Kc, ldr, mod, accum != 0, tm, tn, sm);
+    i8_tile<CF, ROW4G, TOUT>(A + (int64_t)blockIdx.y * sA, B + (int64_t)blockIdx.y * sB,
+                             R + (int64_t)blockIdx.y * sR, M, N, ldk, Kc, ldr, mod, accum != 0, tm, tn, sm,
+                             mods.magic[l]);
 }
 
 // zero n ints (a kernel rather than a memset node, so the sequence can be graph-captured)
@@ -540,38 +630,35 @@ __global__ void k_zero_i32(int* p, int64_t n) {
 
 /* ---------------- CRT reconstruction ---------------- */
 
-__device__ __forceinline__ double u128_to_double(u128 x) {
-    uint64_t hi = (uint64_t)(x >> 64), lo = (uint64_t)x;
-    return ldexp((double)hi, 64) + (double)lo;
-}
-
-// The integer in (-M/2, M/2] with residues r[l] mod m_l, as a double.
-// X = sum_l r_l w_l is summed per 32-bit limb of w (each limb sum < 14 * 2^8 *
-// 2^32 < 2^44), then X < 14 * 2^8 * M < 2^124 is reduced mod M with a
-// floating quotient estimate q < 2^12 and an exact correction.
+// The integer Z in (-M/2, M/2) with residues r[l] mod m_l, as a double (the
+// product's integer lies well inside: |Z| <= 2^(2g) (1+d)^2 < 0.23 M for g = 54).
+// X = sum_l r_l w_l is summed per 38-bit limb of w in f64 (each limb sum
+// < 14 * 2^8 * 2^38 < 2^50: exact). X < 14 * 2^8 * M, so X / M is off an integer
+// Q by |Z| / M < 0.23, and the f64 estimate of X / M is within 2^-38 of it:
+// rint gives Q exactly, and Z = X - Q M limb by limb (each exact, |.| < 2^51).
+// Carries then bring the two low limbs into [0, 2^38), and the value is
+// assembled with two roundings (a result below 2^53 is exact).
 __device__ __forceinline__ double crt_from(const uint32_t (&r)[OZ_MAXMOD], const CrtConst& cc) {
-    uint64_t S0 = 0, S1 = 0, S2 = 0, S3 = 0;
+    double Sa = 0.0, Sb = 0.0, Sc = 0.0;
 #pragma unroll
     for (int l = 0; l < OZ_MAXMOD; ++l) {
         if (l >= cc.n) break;
-        const uint64_t rl = r[l];
-        S0 += rl * cc.w32[l][0];
-        S1 += rl * cc.w32[l][1];
-        S2 += rl * cc.w32[l][2];
-        S3 += rl * cc.w32[l][3];
+        const double rl = (double)r[l];
+        Sa = fma(rl, cc.wl[l][0], Sa);
+        Sb = fma(rl, cc.wl[l][1], Sb);
+        Sc = fma(rl, cc.wl[l][2], Sc);
     }
-    const u128 X = (u128)S0 + ((u128)S1 << 32) + ((u128)S2 << 64) + ((u128)S3 << 96);
-    double qd = floor(u128_to_double(X) * cc.invM);
-    if (qd < 0.0) qd = 0.0;
-    const uint64_t q = (uint64_t)qd;
-    const u128 qM = (u128)(q * cc.M32[0]) + ((u128)(q * cc.M32[1]) << 32) + ((u128)(q * cc.M32[2]) << 64) +
-                    ((u128)(q * cc.M32[3]) << 96);
-    i128 Y = (i128)(X - qM);  // X - qM in (-M, 2M)
-    if (qM > X) Y = -(i128)(qM - X);
-    if (Y < 0) Y += (i128)cc.M;
-    if (Y >= (i128)cc.M) Y -= (i128)cc.M;
-    if (Y >= (i128)cc.M - Y) Y -= (i128)cc.M;  // symmetric range (-M/2, M/2]
-    return Y < 0 ? -u128_to_double((u128)(-Y)) : u128_to_double((u128)Y);
+    constexpr double T38 = 274877906944.0, I38 = 1.0 / 274877906944.0, T76 = T38 * T38;
+    const double Xd = fma(Sa, T76, fma(Sb, T38, Sc));
+    const double Q = rint(Xd * cc.invM);
+    double Za = fma(-Q, cc.Ml[0], Sa), Zb = fma(-Q, cc.Ml[1], Sb), Zc = fma(-Q, cc.Ml[2], Sc);
+    const double c1 = floor(Zc * I38);
+    Zc = fma(-c1, T38, Zc);
+    Zb += c1;
+    const double c2 = floor(Zb * I38);
+    Zb = fma(-c2, T38, Zb);
+    Za += c2;
+    return fma(fma(Za, T38, Zb), T38, Zc);
 }
 
 // The same from residue bytes p[l * pl].
@@ -826,7 +913,13 @@ static CrtConst make_crt(int n) {
         M *= OZ_MODULI[l];
     }
     cc.M = M;
-    for (int k = 0; k < 4; ++k) cc.M32[k] = (uint32_t)(M >> (32 * k));
+    const u128 L38 = ((u128)1 << 38) - 1;
+    auto limbs = [&](u128 v, double* out) {
+        out[0] = (double)(uint64_t)(v >> 76);
+        out[1] = (double)(uint64_t)((v >> 38) & L38);
+        out[2] = (double)(uint64_t)(v & L38);
+    };
+    limbs(M, cc.Ml);
     cc.Md = (double)(uint64_t)(M >> 64) * 18446744073709551616.0 + (double)(uint64_t)M;
     cc.invM = 1.0 / cc.Md;
     for (int l = 0; l < n; ++l) {
@@ -834,7 +927,7 @@ static CrtConst make_crt(int n) {
         uint64_t r = (uint64_t)(Ml % OZ_MODULI[l]);
         uint64_t inv = inv_mod(r, OZ_MODULI[l]);
         const u128 w = mul_mod(Ml, inv, M);
-        for (int k = 0; k < 4; ++k) cc.w32[l][k] = (uint32_t)(w >> (32 * k));
+        limbs(w, cc.wl[l]);
     }
     return cc;
 }
@@ -886,23 +979,31 @@ static OzSetup ozaki_setup() {
     return su;
 }
 
-// Tile shape of the residue products. Measured at 8192^3, 64 x 1024^3 and
-// 60 x 400 x 24000 x 1600: 128 x 256 from eight 64 x 64 waves at two blocks
-// per CU is fastest (a second resident block hides the other's modular
-// epilogue and barrier waits); 128 x 128 from four waves is next, and the same
-// block from four 64 x 128 waves at one block per CU last. Outputs at most 128
-// wide keep the 128 x 128 tile (no padding of N to 256). RSTSR_HIP_OZAKI_TILE
-// (square, square32, wide, wide8) overrides the choice.
-enum class I8Tile { Sq, Sq32, Wide, Wide8 };
+// Tile shape of the residue products. Measured (14 moduli): at 8192^3 the 256 x 256
+// tile takes 6.7 ms against 8.0 ms for 128 x 256 from eight 64 x 64 waves; 128 x 256 from
+// four 64 x 128 waves, 256 x 256 from four 128 x 128 waves (which spill), 32 x 32 x 32
+// MFMAs and global loads two slices ahead were no faster. Products at most 256 long on a side
+// keep the smaller tiles (less padding); outputs at most 128 wide the 128 x 128 tile.
+// RSTSR_HIP_OZAKI_TILE (square, wide8, big) overrides the choice.
+enum class I8Tile { Sq, Wide8, Big };
 
-static I8Tile pick_i8_tile(int64_t N) {
+static I8Tile pick_i8_tile(int64_t M, int64_t N) {
     const char* s = std::getenv("RSTSR_HIP_OZAKI_TILE");
-    if (!s || !*s) return N > 128 ? I8Tile::Wide8 : I8Tile::Sq;
+    if (!s || !*s) return N <= 128 ? I8Tile::Sq : (M > 256 && N > 256 ? I8Tile::Big : I8Tile::Wide8);
     if (std::strcmp(s, "square") == 0) return I8Tile::Sq;
-    if (std::strcmp(s, "square32") == 0) return I8Tile::Sq32;
-    if (std::strcmp(s, "wide") == 0) return I8Tile::Wide;
     if (std::strcmp(s, "wide8") == 0) return I8Tile::Wide8;
-    RSTSR_RAISE(InvalidValue, "RSTSR_HIP_OZAKI_TILE must be square, square32, wide or wide8");
+    if (std::strcmp(s, "big") == 0) return I8Tile::Big;
+    RSTSR_RAISE(InvalidValue, "RSTSR_HIP_OZAKI_TILE must be square, wide8 or big");
+}
+
+// f(cfg) with a value of the configuration type of tile t
+template <class F>
+static void with_i8_tile(I8Tile t, F&& f) {
+    switch (t) {
+        case I8Tile::Wide8: f(I8Wide8{}); break;
+        case I8Tile::Big: f(I8Big{}); break;
+        default: f(I8Sq{}); break;
+    }
 }
 
 void ozaki_gemm_f64(rstsr_hip_ctx* ctx, const GemmProblem& P, double alpha, double beta, bool beta_zero) {
@@ -913,8 +1014,12 @@ void ozaki_gemm_f64(rstsr_hip_ctx* ctx, const GemmProblem& P, double alpha, doub
     const ModSet& ms = su.ms;
     const int64_t M = P.M, N = P.N, K = P.K;
     if (K > (int64_t)1 << 40) RSTSR_RAISE(UnImplemented, "K too large for the split GEMM");
-    const I8Tile tile = pick_i8_tile(N);
-    const int64_t bn = (tile == I8Tile::Wide || tile == I8Tile::Wide8) ? 256 : 128;
+    const I8Tile tile = pick_i8_tile(M, N);
+    int64_t bm = 0, bn = 0;
+    with_i8_tile(tile, [&](auto c) {
+        bm = decltype(c)::BM;
+        bn = decltype(c)::BN;
+    });
     // Symmetric product (the SYRK condition of device_faer/matmul.rs:38-48: beta = 0, A is B's
     // transpose in the same buffer): B's columns are A's rows, so B's scales and residue planes are
     // A's; only the INT8 tiles touching the lower triangle run, the CRT fills i >= j, and the mirror
@@ -922,7 +1027,7 @@ void ozaki_gemm_f64(rstsr_hip_ctx* ctx, const GemmProblem& P, double alpha, doub
     bool sym = beta_zero && P.a == P.b && M == N && P.rsa == P.csb && P.csa == P.rsb;
     for (int d = 0; d < P.bd.nbd && sym; ++d) sym = P.bd.sa[d] == P.bd.sb[d];
     const int64_t Np = (N + bn - 1) / bn * bn;
-    const int64_t Mp = sym ? Np : (M + I8_MPAD - 1) / I8_MPAD * I8_MPAD;  // one padded extent when shared
+    const int64_t Mp = sym ? Np : (M + bm - 1) / bm * bm;  // one padded extent when shared
     const int64_t Kp = (K + I8_KPAD - 1) / I8_KPAD * I8_KPAD;
     const uint64_t nb = P.nbatch;
     if (nb >= 65536) RSTSR_RAISE(UnImplemented, "too many batches for the split GEMM");
@@ -991,8 +1096,8 @@ void ozaki_gemm_f64(rstsr_hip_ctx* ctx, const GemmProblem& P, double alpha, doub
     int8_t* B8 = (int8_t*)(ws + o_B8);
     uint8_t* R = (uint8_t*)(ws + o_R);
     const uint32_t tiles_n = (uint32_t)(Np / bn);
-    const uint32_t ntiles = sym ? (uint32_t)lower_tile_prefix((uint64_t)(Mp / I8_MPAD), (uint64_t)(bn / I8_MPAD))
-                                : (uint32_t)((Mp / I8_MPAD) * tiles_n);
+    const uint32_t ntiles = sym ? (uint32_t)lower_tile_prefix((uint64_t)(Mp / bm), (uint64_t)(bn / bm))
+                                : (uint32_t)((Mp / bm) * tiles_n);
 
     hipLaunchKernelGGL(k_zero_i32, dim3(ceil_div_u32(n_zero, 256)), dim3(256), 0, ctx->stream, flag, (int64_t)n_zero);
     check_launch("ozaki zero");
@@ -1034,24 +1139,25 @@ void ozaki_gemm_f64(rstsr_hip_ctx* ctx, const GemmProblem& P, double alpha, doub
     // values. No host round trip, so the sequence can be graph-captured.
     plain_dgemm(ctx, P, alpha, beta, beta_zero, flag);
     ModList mods{};
-    for (int l = 0; l < n; ++l) mods.m[l] = (int)cc.mod[l];
+    for (int l = 0; l < n; ++l) {
+        mods.m[l] = (int)cc.mod[l];
+        mods.magic[l] = ms.magic[l];
+    }
     // K beyond I8_KCHUNK: further launches add their residues into R
     for (int64_t k0 = 0; k0 < Kp; k0 += I8_KCHUNK) {
         const int64_t Kc = std::min<int64_t>(I8_KCHUNK, Kp - k0);
         auto go = [&](auto cfg) {
             using CF = decltype(cfg);
-            hipLaunchKernelGGL((k_i8gemm_mod<CF, I8_ROW4G>), dim3(ntiles, (uint32_t)nb, (uint32_t)n), dim3(CF::NT), 0,
+            // the 256 x 256 tile stores its residues four columns at a time
+            constexpr bool TOUT = CF::BM == 256;
+            hipLaunchKernelGGL((k_i8gemm_mod<CF, I8_ROW4G, TOUT>), dim3(ntiles, (uint32_t)nb, (uint32_t)n),
+                               dim3(CF::NT), 0,
                                ctx->stream, A8 + k0, B8 + k0, R, Mp, N, Kp, Kc, N, mods, k0 > 0 ? 1 : 0,
-                               (uint32_t)(Mp / I8_MPAD), tiles_n, a_b ? Mp * Kp : (int64_t)0,
+                               (uint32_t)(Mp / bm), tiles_n, a_b ? Mp * Kp : (int64_t)0,
                                b_b ? Np * Kp : (int64_t)0, Mp * N, (int64_t)planeA, (int64_t)planeB,
                                (int64_t)planeR, flag, sym ? 1 : 0);
         };
-        switch (tile) {
-            case I8Tile::Wide: go(I8Wide{}); break;
-            case I8Tile::Wide8: go(I8Wide8{}); break;
-            case I8Tile::Sq32: go(I8Sq32{}); break;
-            default: go(I8Sq{}); break;
-        }
+        with_i8_tile(tile, go);
         check_launch("ozaki int8 gemm");
     }
     // four columns per thread when every row and plane of R starts on a 4-byte boundary

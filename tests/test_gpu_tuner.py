@@ -152,7 +152,7 @@ def test_ozaki_int8_tiles_in_parity(tmp_path):
     ref = tmp_path / "default.npz"
     out = _child(base, ref, "ozaki")
     assert "GEMM_OK" in out, out
-    for tile in ("square", "square32", "wide", "wide8"):
+    for tile in ("square", "wide8", "big"):
         forced = tmp_path / f"{tile}.npz"
         out = _child(dict(base, RSTSR_HIP_OZAKI_TILE=tile), forced, "ozaki")
         assert "GEMM_OK" in out, (tile, out)
