@@ -501,6 +501,19 @@ def gemm_leg(args, dev, rt):
         res[mode] = _gemm_rates(flop, ms, mode, MODULI * flop)
         res[mode]["normwise_err_full_max_over_problems"] = err
         log(f"[bench] grouped512 {mode}: {ms:.2f} ms {res[mode]['TFLOP/s']:.2f} TFLOP/s err {err:.2e}")
+    if "grouped" in legs and "ozaki" in modes:
+        # the split mode routes each problem (none of these pays for the split); the grouped
+        # split kernels forced on every problem, for the record
+        os.environ["RSTSR_HIP_OZAKI_GROUPED"] = "split"
+        try:
+            ms = _timed(ev, lambda: rt.gemm_grouped(probs, mode="ozaki"), 3)
+            err = max(normwise(dev, tc, r) for (_, _, tc), r in zip(probs, refs))
+        finally:
+            del os.environ["RSTSR_HIP_OZAKI_GROUPED"]
+        res["ozaki_all_split"] = _gemm_rates(flop, ms, "ozaki", MODULI * flop)
+        res["ozaki_all_split"]["normwise_err_full_max_over_problems"] = err
+        log(f"[bench] grouped512 ozaki, every problem split: {ms:.2f} ms "
+            f"{res['ozaki_all_split']['TFLOP/s']:.2f} TFLOP/s err {err:.2e}")
     if "grouped" in legs and not args.no_cpu:
         sel = list(range(0, 512, 16))
 

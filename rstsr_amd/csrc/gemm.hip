@@ -210,6 +210,14 @@ __device__ __forceinline__ void mfma_tile(const TileArgs<T>& g, uint32_t tm, uin
             }
     };
 
+    // 16 x 16 fragments of this wave that hold any row < M and column < N (the wave index is
+    // uniform, read through a scalar register so the tests below are scalar branches): edge
+    // tiles skip the MFMAs of fragments wholly in the padding
+    const int wv = __builtin_amdgcn_readfirstlane(wave);
+    const int64_t rows_left = g.M - (m0 + (wv / CF::WVN) * 16 * FM), cols_left = g.N - (n0 + (wv % CF::WVN) * 16 * FN);
+    const int fmv = rows_left <= 0 ? 0 : (int)std::min<int64_t>(FM, (rows_left + 15) / 16);
+    const int fnv = cols_left <= 0 ? 0 : (int)std::min<int64_t>(FN, (cols_left + 15) / 16);
+
     load_slice(0);
     store_slice(0);
     __syncthreads();
@@ -235,6 +243,7 @@ __device__ __forceinline__ void mfma_tile(const TileArgs<T>& g, uint32_t tm, uin
             for (int i = 0; i < FM; ++i)
 #pragma unroll
                 for (int j = 0; j < FN; ++j) {
+                    if (i >= fmv || j >= fnv) continue;
                     if constexpr (!CX) {
                         acc[0][i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[0][i], fb[0][j], acc[0][i][j], 0, 0, 0);
                     } else {
@@ -725,7 +734,11 @@ void plain_grouped_f64(rstsr_hip_ctx* ctx, const std::vector<GemmProblem>& probs
     for (auto& P : probs)
         ta.push_back(TileArgs<double>{P.M, P.N, P.K, (const double*)P.a, P.rsa, P.csa, (const double*)P.b, P.rsb, P.csb,
                                       (double*)P.c, P.rsc, P.csc});
-    if (!ta.empty()) launch_grouped_cfg<double, D0>(ctx, ta, alpha, beta, beta_zero, run_if);
+    if (ta.empty()) return;
+    // unconditional: the tuned configuration; conditional (a fallback inside a captured
+    // sequence): the fixed one, since the autotuner cannot time a launch that may not run
+    if (run_if) launch_grouped_cfg<double, D0>(ctx, ta, alpha, beta, beta_zero, run_if);
+    else grouped_t<double>(ctx, ta, alpha, beta, beta_zero);
 }
 
 // Zero-fill C (k == 0 with beta == 0: crates-device/rstsr-openblas/src/matmul_impl.rs:78-81).

@@ -90,8 +90,8 @@ void mirror_lower_f64(rstsr_hip_ctx* ctx, const GemmProblem& P);
 // Outputs must not overlap.
 void ozaki_grouped_f64(rstsr_hip_ctx* ctx, const std::vector<GemmProblem>& probs, double alpha, double beta,
                        bool beta_zero);
-// Plain grouped FP64 GEMM in one persistent launch with a fixed tile shape
-// (gemm.hip); run_if as for plain_dgemm.
+// Plain grouped FP64 GEMM in one persistent launch (gemm.hip): the autotuned tile
+// shape, or with run_if (as for plain_dgemm) a fixed one.
 void plain_grouped_f64(rstsr_hip_ctx* ctx, const std::vector<GemmProblem>& probs, double alpha, double beta,
                        bool beta_zero, const int* run_if);
 

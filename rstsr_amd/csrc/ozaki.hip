@@ -45,6 +45,7 @@ typedef __int128 i128;
 constexpr int OZ_MAXMOD = 14;  // X = sum r_l w_l < 14 * 256 * M < 2^128 needs M < 2^116
 // largest pairwise coprime moduli <= 256
 static const uint32_t OZ_MODULI[16] = {256, 255, 253, 251, 247, 241, 239, 233, 229, 227, 223, 217, 211, 199, 197, 193};
+constexpr uint32_t OZ_MODULI_C[OZ_MAXMOD] = {256, 255, 253, 251, 247, 241, 239, 233, 229, 227, 223, 217, 211, 199};
 
 // CRT constants: w_l = (M/m_l) ((M/m_l)^-1 mod m_l) and M = prod m_l, both
 // also in 32-bit limbs (least significant first) for the device sums.
@@ -56,6 +57,38 @@ struct CrtConst {
     u128 M;
     double Md, invM;
 };
+
+// The CRT limbs of the default 14 moduli as compile-time constants: kernels that
+// use them keep no constant table in registers (the 42 limbs would not fit the
+// scalar registers next to the rest).
+struct CrtLimbs {
+    double wl[OZ_MAXMOD][3];
+    double Ml[3];
+    double invM;
+};
+constexpr CrtLimbs make_crt_limbs14() {
+    CrtLimbs k{};
+    u128 M = 1;
+    for (int l = 0; l < OZ_MAXMOD; ++l) M *= OZ_MODULI_C[l];
+    const u128 L38 = ((u128)1 << 38) - 1;
+    for (int l = 0; l < OZ_MAXMOD; ++l) {
+        const uint32_t m = OZ_MODULI_C[l];
+        const u128 Mm = M / m;
+        const uint32_t rm = (uint32_t)(Mm % m);
+        uint32_t inv = 1;
+        while ((rm * inv) % m != 1) ++inv;
+        const u128 w = Mm * inv % M;  // Mm * inv < 2^8 M < 2^128
+        k.wl[l][0] = (double)(uint64_t)(w >> 76);
+        k.wl[l][1] = (double)(uint64_t)((w >> 38) & L38);
+        k.wl[l][2] = (double)(uint64_t)(w & L38);
+    }
+    k.Ml[0] = (double)(uint64_t)(M >> 76);
+    k.Ml[1] = (double)(uint64_t)((M >> 38) & L38);
+    k.Ml[2] = (double)(uint64_t)(M & L38);
+    k.invM = 1.0 / ((double)(uint64_t)(M >> 64) * 18446744073709551616.0 + (double)(uint64_t)M);
+    return k;
+}
+constexpr CrtLimbs CRT14 = make_crt_limbs14();
 
 /* ---------------- vector norms -> scale exponents ---------------- */
 
@@ -181,43 +214,91 @@ __global__ void __launch_bounds__(256) k_norm_fin(const NormAcc* __restrict__ pa
 struct ModSet {
     int n;
     double m[OZ_MAXMOD];
-    double inv[OZ_MAXMOD];
     uint32_t mi[OZ_MAXMOD];     // m
-    uint32_t magic[OZ_MAXMOD];  // ceil(2^32 / m)
+    uint32_t magic[OZ_MAXMOD];  // ceil(2^32 / m), for the INT8 epilogue
     uint32_t clo[OZ_MAXMOD];    // bytes 256^k mod m, k = 0..3
-    uint32_t chi[OZ_MAXMOD];    // bytes 256^k mod m, k = 4..6 (byte 3 zero)
+    uint32_t chi[OZ_MAXMOD];    // bytes 256^k mod m, k = 4..7
+    uint32_t off[OZ_MAXMOD];    // m - (2^56 mod m) + 128
+    uint32_t mg20[OZ_MAXMOD];   // ceil(2^20 / m)
 };
 
-// The split integers x (|x| <= ||x||_2 <= 2^g, g <= 55) as 7 magnitude bytes in two words.
+// Per-modulus split constants (the ModSet fields above) of one modulus.
+struct ResConst {
+    uint32_t clo, chi, off, mg20, m;
+};
+constexpr ResConst make_res_const(uint32_t m) {
+    ResConst c{0, 0, 0, 0, m};
+    uint32_t pw = 1 % m;  // 256^k mod m
+    for (int k = 0; k < 8; ++k) {
+        if (k < 4) c.clo |= pw << (8 * k);
+        else c.chi |= pw << (8 * (k - 4));
+        pw = (pw * 256) % m;
+    }
+    // pw = 256^8 mod m here; 2^56 mod m = 256^7 mod m
+    uint32_t p56 = 1 % m;
+    for (int k = 0; k < 7; ++k) p56 = (p56 * 256) % m;
+    c.off = m - p56 + 128;
+    c.mg20 = (uint32_t)(((1u << 20) + m - 1) / m);
+    return c;
+}
+struct ResTable {
+    ResConst c[OZ_MAXMOD];
+};
+constexpr ResTable make_res_table14() {
+    ResTable t{};
+    for (int l = 0; l < OZ_MAXMOD; ++l) t.c[l] = make_res_const(OZ_MODULI_C[l]);
+    return t;
+}
+// the default 14 moduli's constants at compile time
+constexpr ResTable RES14 = make_res_table14();
+
+// A split integer x (|x| <= ||x||_2 <= 2^g, g <= 55) as u = x + 2^56 in [2^55, 3 2^55]:
+// eight bytes in two words, u = hi 2^32 + lo.
 struct Digits {
     uint32_t lo, hi;
-    bool neg;
 };
 __device__ __forceinline__ Digits to_digits(double x) {
-    const double ax = fabs(x);
-    const double h = floor(ax * 0x1p-32);  // < 2^24
-    const double l = fma(-h, 0x1p32, ax);  // exact, in [0, 2^32)
-    return {(uint32_t)l, (uint32_t)h, x < 0.0};
+    const double h = floor(x * 0x1p-32);  // in [-2^23, 2^23]
+    const double l = fma(-h, 0x1p32, x);  // exact, in [0, 2^32)
+    return {(uint32_t)l, (uint32_t)((int)h + (1 << 24))};
 }
 
-// Symmetric residue (in [-128, 127]) of the integer with digits d modulo m = ms.mi[l]:
-// |x| = sum_k b_k 256^k, so |x| mod m = (sum_k b_k (256^k mod m)) mod m, the sum (< 7 * 255^2
-// < 2^19) taken with two byte dot products; its quotient by m is exact from the high word of
-// s * ceil(2^32 / m) (the approximation's error s (ceil(2^32/m) m - 2^32) / 2^32 < 2^-5 stays
-// below the 1/m a non-multiple keeps from the next integer). The sign follows x.
-__device__ __forceinline__ int sym_residue_d(const Digits& d, const ModSet& ms, int l) {
-    const uint32_t s = __builtin_amdgcn_udot4(d.hi, ms.chi[l], __builtin_amdgcn_udot4(d.lo, ms.clo[l], 0u, false), false);
-    const uint32_t m = ms.mi[l];
-    int r = (int)(s - __umulhi(s, ms.magic[l]) * m);
-    if (d.neg && r != 0) r = (int)m - r;
-    return r > 127 ? r - (int)m : r;
+// Residue byte of x modulo m: t in [0, m) with t = x + 128 (mod m), so that t - 128 in
+// [-128, m - 128) is an int8 representative of x mod m, stored as the byte t ^ 0x80.
+// s = sum_k b_k (256^k mod m) + m - (2^56 mod m) + 128 = u - 2^56 + 128 = x + 128 (mod m), with
+// s < 8 * 255^2 + 384 < 2^19 (two byte dot products, the offset as their accumulator). The
+// quotient from 24-bit products: s * ceil(2^20 / m) < 2^32, and its floor / 2^20 exceeds
+// floor(s / m) by at most one (the error s (ceil(2^20/m) m - 2^20) / (m 2^20) < s / 2^20 < 1/2),
+// so t = s - q m lies in [-m, m) and one correction ends it.
+__device__ __forceinline__ uint32_t res_byte(const Digits& d, const ResConst& c) {
+    const uint32_t s = __builtin_amdgcn_udot4(d.hi, c.chi, __builtin_amdgcn_udot4(d.lo, c.clo, c.off, false), false);
+    const uint32_t q = __umul24(s, c.mg20) >> 20;
+    const int t = (int)s - (int)__umul24(q, c.m);
+    return (uint32_t)(t < 0 ? t + (int)c.m : t);
 }
 
-__device__ __forceinline__ uint32_t pack_residues_d(const Digits* d, const ModSet& ms, int l) {
+template <bool K14>
+__device__ __forceinline__ ResConst res_const(const ModSet& ms, int l) {
+    if constexpr (K14) return RES14.c[l];
+    else return ResConst{ms.clo[l], ms.chi[l], ms.off[l], ms.mg20[l], ms.mi[l]};
+}
+
+__device__ __forceinline__ uint32_t pack_residues(const Digits* d, const ResConst& c) {
     uint32_t w = 0;
 #pragma unroll
-    for (int i = 0; i < 4; ++i) w |= (uint32_t)(uint8_t)(int8_t)sym_residue_d(d[i], ms, l) << (8 * i);
-    return w;
+    for (int i = 0; i < 4; ++i) w |= res_byte(d[i], c) << (8 * i);
+    return w ^ 0x80808080u;
+}
+
+// f(l) for every modulus: unrolled with compile-time constants for the default 14
+template <bool K14, class F>
+__device__ __forceinline__ void for_moduli(const ModSet& ms, F&& f) {
+    if constexpr (K14) {
+#pragma unroll
+        for (int l = 0; l < OZ_MAXMOD; ++l) f(l);
+    } else {
+        for (int l = 0; l < ms.n; ++l) f(l);
+    }
 }
 
 // Tile (v0, t0) of the split: source S(v, t) = s[v*sv + t*st] for v < nv,
@@ -225,6 +306,7 @@ __device__ __forceinline__ uint32_t pack_residues_d(const Digits* d, const ModSe
 // multiple of 64). The 64 x 64 tile is staged through LDS (`tile`) so that
 // reads run along the source's unit-stride axis and each thread writes 16
 // consecutive plane bytes per modulus.
+template <bool K14>
 __device__ __forceinline__ void split_tile(const double* __restrict__ s, int64_t nv, int64_t len, int64_t sv,
                                            int64_t st, const int* __restrict__ ev, int64_t nvp, int64_t lenp,
                                            int8_t* __restrict__ planes, const ModSet& ms, bool read_along_t,
@@ -247,18 +329,17 @@ __device__ __forceinline__ void split_tile(const double* __restrict__ s, int64_t
 #pragma unroll
     for (int i = 0; i < 16; ++i) dg[i] = to_digits(tile[vv][tq + i]);
     int8_t* dst = planes + gv * lenp + t0 + tq;
-    for (int l = 0; l < ms.n; ++l) {
-        uint4 w;
-        w.x = pack_residues_d(dg, ms, l);
-        w.y = pack_residues_d(dg + 4, ms, l);
-        w.z = pack_residues_d(dg + 8, ms, l);
-        w.w = pack_residues_d(dg + 12, ms, l);
+    for_moduli<K14>(ms, [&](int l) {
+        const ResConst c = res_const<K14>(ms, l);
+        const uint4 w{pack_residues(dg, c), pack_residues(dg + 4, c), pack_residues(dg + 8, c),
+                      pack_residues(dg + 12, c)};
         *(uint4*)(dst + l * plane) = w;
-    }
+    });
 }
 
 // Batch b = blockIdx.z of one operand (`which`: 0 = A, 1 = B offsets of bd):
 // exponents ev + b*nvp, planes + b*nvp*lenp.
+template <bool K14>
 __global__ void __launch_bounds__(256) k_split(const double* __restrict__ s, int64_t nv, int64_t len, int64_t sv,
                                                int64_t st, BatchDesc bd, int which, const int* __restrict__ ev,
                                                int64_t nvp, int64_t lenp, int8_t* __restrict__ planes, ModSet ms,
@@ -268,7 +349,7 @@ __global__ void __launch_bounds__(256) k_split(const double* __restrict__ s, int
     const uint32_t b = blockIdx.z;
     int64_t oa, ob, oc;
     batch_offsets(bd, b, oa, ob, oc);
-    split_tile(s + (which ? ob : oa), nv, len, sv, st, ev + (int64_t)b * nvp, nvp, lenp,
+    split_tile<K14>(s + (which ? ob : oa), nv, len, sv, st, ev + (int64_t)b * nvp, nvp, lenp,
                planes + (int64_t)b * nvp * lenp, ms, read_along_t != 0, plane, (int64_t)blockIdx.y * 64,
                (int64_t)blockIdx.x * 64, tile);
 }
@@ -278,6 +359,7 @@ __global__ void __launch_bounds__(256) k_split(const double* __restrict__ s, int
 // exactly as k_norm_rows does; pass 2 re-reads the vector (from cache) and writes its 16-byte
 // residue runs to every plane like split_tile. Padding rows v >= nv and positions t >= len
 // are zero.
+template <bool K14>
 __global__ void __launch_bounds__(256) k_normsplit_rows(const double* __restrict__ x, int64_t nv, int64_t len,
                                                         int64_t sv, BatchDesc bd, int which, int64_t nvp,
                                                         int64_t lenp, int g, int* __restrict__ ev,
@@ -321,14 +403,12 @@ __global__ void __launch_bounds__(256) k_normsplit_rows(const double* __restrict
 #pragma unroll
             for (int i = 0; i < 16; ++i) dg[i] = to_digits(t0 + i < len ? rint(ldexp(xv[t0 + i], e)) : 0.0);
         }
-        for (int l = 0; l < ms.n; ++l) {
-            uint4 w;
-            w.x = pack_residues_d(dg, ms, l);
-            w.y = pack_residues_d(dg + 4, ms, l);
-            w.z = pack_residues_d(dg + 8, ms, l);
-            w.w = pack_residues_d(dg + 12, ms, l);
+        for_moduli<K14>(ms, [&](int l) {
+            const ResConst c = res_const<K14>(ms, l);
+            const uint4 w{pack_residues(dg, c), pack_residues(dg + 4, c), pack_residues(dg + 8, c),
+                          pack_residues(dg + 12, c)};
             *(uint4*)(pv + l * plane + t0) = w;
-        }
+        });
     }
 }
 
@@ -637,21 +717,14 @@ __global__ void k_zero_i32(int* p, int64_t n) {
 // Q by |Z| / M < 0.23, and the f64 estimate of X / M is within 2^-38 of it:
 // rint gives Q exactly, and Z = X - Q M limb by limb (each exact, |.| < 2^51).
 // Carries then bring the two low limbs into [0, 2^38), and the value is
-// assembled with two roundings (a result below 2^53 is exact).
-__device__ __forceinline__ double crt_from(const uint32_t (&r)[OZ_MAXMOD], const CrtConst& cc) {
-    double Sa = 0.0, Sb = 0.0, Sc = 0.0;
-#pragma unroll
-    for (int l = 0; l < OZ_MAXMOD; ++l) {
-        if (l >= cc.n) break;
-        const double rl = (double)r[l];
-        Sa = fma(rl, cc.wl[l][0], Sa);
-        Sb = fma(rl, cc.wl[l][1], Sb);
-        Sc = fma(rl, cc.wl[l][2], Sc);
-    }
+// assembled with two roundings (a result below 2^53 is exact). crt_finish takes the
+// limb sums; crt_from forms them (K14: the default 14 moduli's constants from CRT14,
+// otherwise those of cc).
+__device__ __forceinline__ double crt_finish(double Sa, double Sb, double Sc, double invM, const double (&Ml)[3]) {
     constexpr double T38 = 274877906944.0, I38 = 1.0 / 274877906944.0, T76 = T38 * T38;
     const double Xd = fma(Sa, T76, fma(Sb, T38, Sc));
-    const double Q = rint(Xd * cc.invM);
-    double Za = fma(-Q, cc.Ml[0], Sa), Zb = fma(-Q, cc.Ml[1], Sb), Zc = fma(-Q, cc.Ml[2], Sc);
+    const double Q = rint(Xd * invM);
+    double Za = fma(-Q, Ml[0], Sa), Zb = fma(-Q, Ml[1], Sb), Zc = fma(-Q, Ml[2], Sc);
     const double c1 = floor(Zc * I38);
     Zc = fma(-c1, T38, Zc);
     Zb += c1;
@@ -661,18 +734,47 @@ __device__ __forceinline__ double crt_from(const uint32_t (&r)[OZ_MAXMOD], const
     return fma(fma(Za, T38, Zb), T38, Zc);
 }
 
-// The same from residue bytes p[l * pl].
-__device__ __forceinline__ double crt_value(const uint8_t* __restrict__ p, int64_t pl, const CrtConst& cc) {
-    uint32_t r[OZ_MAXMOD];
+template <bool K14>
+__device__ __forceinline__ double crt_from(const uint32_t (&r)[OZ_MAXMOD], const CrtConst& cc) {
+    // a full unrolled sum (constants past cc.n are zero): no runtime register indexing
+    double Sa = 0.0, Sb = 0.0, Sc = 0.0;
 #pragma unroll
-    for (int l = 0; l < OZ_MAXMOD; ++l) r[l] = l < cc.n ? p[l * pl] : 0u;
-    return crt_from(r, cc);
+    for (int l = 0; l < OZ_MAXMOD; ++l) {
+        const double rl = (double)r[l];
+        Sa = fma(rl, K14 ? CRT14.wl[l][0] : cc.wl[l][0], Sa);
+        Sb = fma(rl, K14 ? CRT14.wl[l][1] : cc.wl[l][1], Sb);
+        Sc = fma(rl, K14 ? CRT14.wl[l][2] : cc.wl[l][2], Sc);
+    }
+    return K14 ? crt_finish(Sa, Sb, Sc, CRT14.invM, CRT14.Ml) : crt_finish(Sa, Sb, Sc, cc.invM, cc.Ml);
+}
+
+// The same from residue bytes p[l * pl]: with the runtime constants a rolled loop that
+// accumulates as it loads (each limb triple read by scalar loads in its iteration; the
+// unrolled form would hold all 42 limbs in scalar registers, which do not fit).
+template <bool K14>
+__device__ __forceinline__ double crt_value(const uint8_t* __restrict__ p, int64_t pl, const CrtConst& cc) {
+    if constexpr (K14) {
+        uint32_t r[OZ_MAXMOD];
+#pragma unroll
+        for (int l = 0; l < OZ_MAXMOD; ++l) r[l] = p[l * pl];
+        return crt_from<true>(r, cc);
+    } else {
+        double Sa = 0.0, Sb = 0.0, Sc = 0.0;
+#pragma unroll 2
+        for (int l = 0; l < cc.n; ++l) {
+            const double rl = (double)p[l * pl];
+            Sa = fma(rl, cc.wl[l][0], Sa);
+            Sb = fma(rl, cc.wl[l][1], Sb);
+            Sc = fma(rl, cc.wl[l][2], Sc);
+        }
+        return crt_finish(Sa, Sb, Sc, cc.invM, cc.Ml);
+    }
 }
 
 // R planes: [l][batch][Mp][N] (plane stride pl); eA [nbA][Mp], fB [nbB][Np].
 // C = alpha 2^-(eA_i + fB_j) CRT(R) + beta C: columns along x (V consecutive
 // per thread, read as one V-byte word per plane), rows along y, batches along z.
-template <int V>  // columns per thread: 1, 4 (one 4-byte word per plane) or 16 (one 16-byte word)
+template <int V, bool K14>  // columns per thread: 1, 4 (one 4-byte word per plane) or 16 (one 16-byte word)
 __global__ void __launch_bounds__(256) k_crt(const uint8_t* __restrict__ R, int64_t M, int64_t N, int64_t Mp,
                                              int64_t Np, int64_t pl, int a_batched, int b_batched, BatchDesc bd,
                                              CrtConst cc, const int* __restrict__ eA, const int* __restrict__ fB,
@@ -695,7 +797,7 @@ __global__ void __launch_bounds__(256) k_crt(const uint8_t* __restrict__ R, int6
         const int ei = eAb[i];
         if (lower && j0 > i) continue;  // symmetric product: the mirror fills j > i
         if constexpr (V == 1) {
-            const double v = ldexp(crt_value(rp, pl, cc), -(ei + fBb[j0]));
+            const double v = ldexp(crt_value<K14>(rp, pl, cc), -(ei + fBb[j0]));
             *cp = beta_zero ? alpha * v : alpha * v + beta * (*cp);
         } else {
             constexpr int NW = V / 4;  // 4-byte words per plane
@@ -712,16 +814,17 @@ __global__ void __launch_bounds__(256) k_crt(const uint8_t* __restrict__ R, int6
                     w[l][3] = q.w;
                 }
             }
+            // every column computed (fB holds Np >= j0 + V entries), the stores guarded
 #pragma unroll
             for (int k = 0; k < V; ++k) {
                 uint32_t r[OZ_MAXMOD];
 #pragma unroll
                 for (int l = 0; l < OZ_MAXMOD; ++l) r[l] = (w[l][k / 4] >> (8 * (k % 4))) & 255u;
-                if (lower && j0 + k > i) break;
-                if (j0 + k >= N) break;
-                const double v = ldexp(crt_from(r, cc), -(ei + fBb[j0 + k]));
-                double* ck = cp + k * csc;
-                *ck = beta_zero ? alpha * v : alpha * v + beta * (*ck);
+                const double v = ldexp(crt_from<K14>(r, cc), -(ei + fBb[j0 + k]));
+                if (j0 + k < N && !(lower && j0 + k > i)) {
+                    double* ck = cp + k * csc;
+                    *ck = beta_zero ? alpha * v : alpha * v + beta * (*ck);
+                }
             }
         }
     }
@@ -802,6 +905,7 @@ __global__ void __launch_bounds__(256) k_gnorm(const OzProb* __restrict__ probs,
 }
 
 // Residue planes for a group: one block per 64 x 64 tile of an A8 or B8 plane.
+template <bool K14>
 __global__ void __launch_bounds__(256) k_gsplit(const OzProb* __restrict__ probs, int count,
                                                 const uint32_t* __restrict__ pre, const int* __restrict__ eA,
                                                 const int* __restrict__ fB, int8_t* __restrict__ A8,
@@ -817,12 +921,14 @@ __global__ void __launch_bounds__(256) k_gsplit(const OzProb* __restrict__ probs
     const uint32_t ta = (uint32_t)(P.Mp / 64) * kt;
     if (local < ta) {
         const uint32_t vt = local / kt, tt = local - vt * kt;
-        split_tile(P.a, P.M, P.K, P.rsa, P.csa, eA + P.oe, P.Mp, P.Kp, A8 + P.oA, ms, P.csa == 1 || P.csa == -1, planeA,
+        split_tile<K14>(P.a, P.M, P.K, P.rsa, P.csa, eA + P.oe, P.Mp, P.Kp, A8 + P.oA, ms, P.csa == 1 || P.csa == -1,
+                        planeA,
                    (int64_t)vt * 64, (int64_t)tt * 64, tile);
     } else {
         const uint32_t lb = local - ta;
         const uint32_t vt = lb / kt, tt = lb - vt * kt;
-        split_tile(P.b, P.N, P.K, P.csb, P.rsb, fB + P.of, P.Np, P.Kp, B8 + P.oB, ms, P.rsb == 1 || P.rsb == -1, planeB,
+        split_tile<K14>(P.b, P.N, P.K, P.csb, P.rsb, fB + P.of, P.Np, P.Kp, B8 + P.oB, ms, P.rsb == 1 || P.rsb == -1,
+                        planeB,
                    (int64_t)vt * 64, (int64_t)tt * 64, tile);
     }
 }
@@ -849,6 +955,7 @@ __global__ void __launch_bounds__(CF::NT, CF::OCC) k_gi8gemm(const OzProb* __res
 }
 
 // CRT for a group: one thread per output element, ceil(M N / 256) blocks per problem.
+template <bool K14>
 __global__ void __launch_bounds__(256) k_gcrt(const OzProb* __restrict__ probs, int count,
                                               const uint32_t* __restrict__ pre, const uint8_t* __restrict__ R,
                                               int64_t planeR, CrtConst cc, const int* __restrict__ eA,
@@ -862,7 +969,7 @@ __global__ void __launch_bounds__(256) k_gcrt(const OzProb* __restrict__ probs, 
     if ((uint64_t)idx >= (uint64_t)P.M * (uint64_t)P.N) return;
     const uint32_t iq = fdiv(idx, P.fdN);
     const int64_t i = iq, j = (int64_t)(idx - iq * (uint32_t)P.N);
-    double v = crt_value(R + P.oR + i * P.N + j, planeR, cc);
+    double v = crt_value<K14>(R + P.oR + i * P.N + j, planeR, cc);
     v = ldexp(v, -(eA[P.oe + i] + fB[P.of + j]));
     double* cp = P.c + i * P.rsc + j * P.csc;
     double out = alpha * v;
@@ -956,25 +1063,21 @@ static OzSetup ozaki_setup() {
     su.n = ozaki_moduli_count();
     su.cc = make_crt(su.n);
     const double log2M = std::log2(su.cc.Md);
-    // Cauchy-Schwarz: (2^g (1 + d))^2 < M/2 with a 1/2-bit margin; exactness of x = rint(A 2^e) as an
-    // integer-valued double and of the residue arithmetic needs g <= 60
+    // Cauchy-Schwarz: (2^g (1 + d))^2 < M/2 with a 1/2-bit margin; the residue digits (x + 2^56 in
+    // eight bytes) need |x| <= 2^55
     su.g = (int)std::floor((log2M - 1.5) / 2.0);
-    if (su.g > 60) su.g = 60;
+    if (su.g > 55) su.g = 55;
     su.ms.n = su.n;
     for (int l = 0; l < su.n; ++l) {
-        su.ms.m[l] = (double)su.cc.mod[l];
-        su.ms.inv[l] = 1.0 / (double)su.cc.mod[l];
         const uint32_t m = su.cc.mod[l];
+        const ResConst c = make_res_const(m);
+        su.ms.m[l] = (double)m;
         su.ms.mi[l] = m;
         su.ms.magic[l] = (uint32_t)(((1ull << 32) + m - 1) / m);
-        uint32_t pw = 1 % m, lo = 0, hi = 0;  // 256^k mod m
-        for (int k = 0; k < 7; ++k) {
-            if (k < 4) lo |= pw << (8 * k);
-            else hi |= pw << (8 * (k - 4));
-            pw = (pw * 256) % m;
-        }
-        su.ms.clo[l] = lo;
-        su.ms.chi[l] = hi;
+        su.ms.clo[l] = c.clo;
+        su.ms.chi[l] = c.chi;
+        su.ms.off[l] = c.off;
+        su.ms.mg20[l] = c.mg20;
     }
     return su;
 }
@@ -1119,14 +1222,16 @@ void ozaki_gemm_f64(rstsr_hip_ctx* ctx, const GemmProblem& P, double alpha, doub
     auto norm_split = [&](const NormPlan& q, const double* x, int64_t nv, int64_t nvp, int64_t sv, int64_t st,
                           int which, int* ev, int8_t* planes, size_t plane, const char* what) {
         if (st == 1) {
-            hipLaunchKernelGGL(k_normsplit_rows, dim3(ceil_div_u32(nvp, 4), (uint32_t)q.nbx), dim3(256), 0,
+            hipLaunchKernelGGL(n == OZ_MAXMOD ? k_normsplit_rows<true> : k_normsplit_rows<false>,
+                               dim3(ceil_div_u32(nvp, 4), (uint32_t)q.nbx), dim3(256), 0,
                                ctx->stream, x, nv, K, sv, P.bd, which, nvp, Kp, g, ev, planes, (int64_t)plane, ms,
                                flag);
             check_launch(what);
             return;
         }
         run_norm(q, ev);
-        hipLaunchKernelGGL(k_split, dim3((uint32_t)(Kp / 64), (uint32_t)(nvp / 64), (uint32_t)q.nbx), dim3(256), 0,
+        hipLaunchKernelGGL(n == OZ_MAXMOD ? k_split<true> : k_split<false>,
+                           dim3((uint32_t)(Kp / 64), (uint32_t)(nvp / 64), (uint32_t)q.nbx), dim3(256), 0,
                            ctx->stream, x, nv, K, sv, st, P.bd, which, ev, nvp, Kp, planes, ms,
                            std::llabs(st) == 1 ? 1 : 0, flag, (int64_t)plane);
         check_launch(what);
@@ -1170,7 +1275,8 @@ void ozaki_gemm_f64(rstsr_hip_ctx* ctx, const GemmProblem& P, double alpha, doub
         while (tpr < 256 && tpr < ncg) tpr *= 2;
         const int64_t rpb = 256 / tpr;
         const int64_t gy = std::min<int64_t>(std::max<int64_t>(1, (M + 4 * rpb - 1) / (4 * rpb)), 65535);
-        hipLaunchKernelGGL((k_crt<V>), dim3(ceil_div_u32(ncg, tpr), (uint32_t)gy, (uint32_t)nb), dim3(256), 0,
+        auto kern = n == OZ_MAXMOD ? k_crt<V, true> : k_crt<V, false>;
+        hipLaunchKernelGGL(kern, dim3(ceil_div_u32(ncg, tpr), (uint32_t)gy, (uint32_t)nb), dim3(256), 0,
                            ctx->stream, R, M, N, Mp, Np, (int64_t)planeR, a_b ? 1 : 0, b_b ? 1 : 0, P.bd, cc, eA, fB,
                            (double*)P.c, P.rsc, P.csc, alpha, beta, beta_zero ? 1 : 0, flag, sym ? 1 : 0, tpr);
     };
@@ -1182,8 +1288,42 @@ void ozaki_gemm_f64(rstsr_hip_ctx* ctx, const GemmProblem& P, double alpha, doub
     if (sym) mirror_lower_f64(ctx, P);
 }
 
+// Grouped split GEMM routing (RSTSR_HIP_OZAKI_GROUPED: "auto", the default, or "split").
+// The split pays for its memory passes (per input element an f64 read and 14 plane
+// bytes written, per output element 14 plane bytes read and an f64 written) only
+// when the 14 INT8 products outrun one FP64 product; with the rates this build
+// measures on MI355X (FP64 MFMA ~60 TFLOP/s, INT8 2.0-2.3 POP/s on large tiles,
+// ~5 TB/s on the memory passes) that needs K and the output large enough to fill
+// the GPU by themselves. "auto" sends such problems through the single-problem
+// split path and computes the rest in one plain grouped FP64 launch (the reference
+// arithmetic, so no accuracy is lost); for the short products of a typical group
+// (sides 32-1024) the grouped split kernels are slower than FP64 at every size.
+// "split" forces every problem through the grouped split kernels.
+static bool ozaki_grouped_forced() {
+    const char* s = std::getenv("RSTSR_HIP_OZAKI_GROUPED");
+    if (!s || !*s || std::strcmp(s, "auto") == 0) return false;
+    if (std::strcmp(s, "split") == 0) return true;
+    RSTSR_RAISE(InvalidValue, "RSTSR_HIP_OZAKI_GROUPED must be auto or split");
+}
+static bool split_pays(const GemmProblem& P) {
+    const double M = (double)P.M, N = (double)P.N, K = (double)P.K;
+    if (M * N < 4194304.0) return false;  // fewer than 64 256 x 256 tiles per modulus
+    const double t_plain = 2.0 * M * N * K / 60e12;
+    const double t_split = 22.0 * (M * K + N * K + M * N) / 5e12 + 28.0 * M * N * K / 2.0e15;
+    return t_split < t_plain;
+}
+
 void ozaki_grouped_f64(rstsr_hip_ctx* ctx, const std::vector<GemmProblem>& probs, double alpha, double beta,
                        bool beta_zero) {
+    if (!ozaki_grouped_forced()) {
+        std::vector<GemmProblem> plain;
+        for (const GemmProblem& P : probs) {
+            if (split_pays(P)) ozaki_gemm_f64(ctx, P, alpha, beta, beta_zero);
+            else plain.push_back(P);
+        }
+        plain_grouped_f64(ctx, plain, alpha, beta, beta_zero, nullptr);
+        return;
+    }
     // a K longer than one INT8 launch can sum goes through the single-problem path
     std::vector<GemmProblem> keep;
     for (const GemmProblem& P : probs) {
@@ -1293,7 +1433,8 @@ void ozaki_grouped_f64(rstsr_hip_ctx* ctx, const std::vector<GemmProblem>& probs
     hipLaunchKernelGGL(k_gnorm, dim3((uint32_t)tn), dim3(256), 0, ctx->stream, dprobs, count, dpre + 0 * (count + 1),
                        su.g, eA, fB, flag);
     check_launch("grouped ozaki norms");
-    hipLaunchKernelGGL(k_gsplit, dim3((uint32_t)tsp), dim3(256), 0, ctx->stream, dprobs, count, dpre + 1 * (count + 1),
+    hipLaunchKernelGGL(n == OZ_MAXMOD ? k_gsplit<true> : k_gsplit<false>, dim3((uint32_t)tsp), dim3(256), 0,
+                       ctx->stream, dprobs, count, dpre + 1 * (count + 1),
                        eA, fB, A8, B8, (int64_t)sA, (int64_t)sB, su.ms, flag);
     check_launch("grouped ozaki split");
     // non-finite inputs anywhere in the group: the plain grouped kernel computes every product
@@ -1303,7 +1444,10 @@ void ozaki_grouped_f64(rstsr_hip_ctx* ctx, const std::vector<GemmProblem>& probs
                        dpre + 2 * (count + 1), (uint32_t)tg, A8, B8, R, (int64_t)sA, (int64_t)sB, (int64_t)sR, su.ms,
                        flag);
     check_launch("grouped ozaki int8 gemm");
-    hipLaunchKernelGGL(k_gcrt, dim3((uint32_t)tc), dim3(256), 0, ctx->stream, dprobs, count, dpre + 3 * (count + 1), R,
+    // (constants from the kernel arguments: one thread per element has no loop to hoist them out of,
+    // and materializing the 42 limbs per element measured twice as slow)
+    hipLaunchKernelGGL(k_gcrt<false>, dim3((uint32_t)tc), dim3(256), 0, ctx->stream,
+                       dprobs, count, dpre + 3 * (count + 1), R,
                        (int64_t)sR, su.cc, eA, fB, alpha, beta, beta_zero ? 1 : 0, flag);
     check_launch("grouped ozaki crt");
 }

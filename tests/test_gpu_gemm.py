@@ -307,7 +307,15 @@ def _grouped_views(rng, g, m, n, k):
     return A, B
 
 
-def test_grouped_ozaki_layouts(dev, oracle):
+@pytest.fixture(params=["split", "auto"])
+def grouped_route(request, monkeypatch):
+    """Both routes of the grouped split GEMM: every problem through the grouped split kernels,
+    and the default per-problem choice between the split path and the plain grouped kernel."""
+    monkeypatch.setenv("RSTSR_HIP_OZAKI_GROUPED", request.param)
+    return request.param
+
+
+def test_grouped_ozaki_layouts(dev, oracle, grouped_route):
     """Grouped split GEMM (one launch per stage over the whole group): odd sizes including
     1 and K = 0, wide exponents, every operand layout, alpha/beta; normwise 1e-14 per problem."""
     rng = np.random.default_rng(21)
@@ -334,7 +342,7 @@ def test_grouped_ozaki_layouts(dev, oracle):
         assert np.all(np.abs(got - ref) <= bound), (A.shape, B.shape)
 
 
-def test_grouped_ozaki_nonfinite_falls_back(dev, oracle):
+def test_grouped_ozaki_nonfinite_falls_back(dev, oracle, grouped_route):
     """A NaN or Inf anywhere in the group sends the whole group to the plain FP64 kernel: IEEE
     special values propagate as in the plain product and finite problems stay accurate."""
     rng = np.random.default_rng(22)
@@ -354,6 +362,27 @@ def test_grouped_ozaki_nonfinite_falls_back(dev, oracle):
         fin = np.isfinite(ref)
         if fin.any():
             assert np.max(np.abs(got[fin] - ref[fin])) <= 1e-13 * np.max(np.abs(ref[fin]))
+
+
+def test_grouped_ozaki_routes_large_problem(dev, oracle, monkeypatch):
+    """A group with one product large enough for the split path (2048^3) among small ones: the
+    default route sends it through the single-problem split path and the rest through the plain
+    grouped kernel; every product normwise 1e-14 against Dot2, the large one the same bits as
+    its own split call."""
+    monkeypatch.delenv("RSTSR_HIP_OZAKI_GROUPED", raising=False)
+    rng = np.random.default_rng(23)
+    shapes = [(2048, 2048, 2048), (40, 24, 56), (300, 17, 129)]
+    hosts = [(rng.normal(size=(m, k)), rng.normal(size=(k, n))) for m, n, k in shapes]
+    probs = [(rt.asarray(A, device=dev), rt.asarray(B, device=dev),
+              rt.empty((A.shape[0], B.shape[1]), np.float64, device=dev)) for A, B in hosts]
+    rt.gemm_grouped(probs, mode="ozaki")
+    alone = rt.matmul(probs[0][0], probs[0][1], mode="ozaki").to_numpy()
+    assert np.array_equal(probs[0][2].to_numpy(), alone)
+    from dot2_gpu import dot2_matmul, normwise
+    assert normwise(dev, probs[0][2], dot2_matmul(rt, dev, probs[0][0], probs[0][1])) <= 1e-14
+    for (A, B), (_, _, tc) in list(zip(hosts, probs))[1:]:
+        ab = oracle.gemm_dot2(A, B)
+        assert np.max(np.abs(tc.to_numpy() - ab)) <= 1e-14 * np.max(np.abs(ab))
 
 
 @pytest.mark.parametrize("shape", [(64, 96, 80), (300, 130, 257)])
