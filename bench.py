@@ -270,6 +270,12 @@ def main():
     roofline = {"bound": "hbm", "kernel": dom.name, "achieved": round(achieved, 1), "peak": HBM_PEAK,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 4), "traffic": traffic,
                 "algorithmic_bytes_per_launch": dom.nbytes, "avg_launch_ms": dms}
+    if not args.only:
+        ceil = _stream_ceilings()
+        if ceil:
+            roofline["stream_ceilings"] = ceil
+            roofline["frac_of_copy_ceiling"] = round(achieved / ceil["copy_GBps"], 4)
+            log("[bench] HBM stream ceilings: " + json.dumps(ceil))
 
     # ---------------- host <-> device transfers (PCIe-inclusive; not part of the step) ----------------
     transfer = {}
@@ -318,6 +324,26 @@ def main():
         "transfer": transfer,
     }
     print(json.dumps(out), flush=True)
+
+
+def _stream_ceilings(nbytes=1 << 30):
+    """Best GB/s of a contiguous 16-byte-lane copy, read-only and write-only pass over 1 GiB
+    (tests/gpu_check/stream_ceiling.hip): what this GPU's HBM sustains for each traffic mix."""
+    path = os.path.join(ROOT, "tests", "gpu_check", "libstreamcheck.so")
+    if not os.path.exists(path):
+        log("[bench] stream-ceiling probe not built; skipped")
+        return None
+    lib = C.CDLL(path)
+    lib.stream_ceiling.argtypes = [C.c_int, C.c_size_t, C.c_int, C.POINTER(C.c_float)]
+    out = {}
+    for kind, name, mult in ((0, "copy_GBps", 2), (1, "read_GBps", 1), (2, "write_GBps", 1)):
+        ms = C.c_float(0)
+        rc = lib.stream_ceiling(kind, nbytes, 10, C.byref(ms))
+        if rc != 0:
+            raise RuntimeError(f"stream_ceiling({kind}) failed with HIP error {rc}")
+        out[name] = round(mult * nbytes / 1e9 / (ms.value / 1e3), 1)
+    out["bytes_per_buffer"] = nbytes
+    return out
 
 
 def _timed(ev, fn, reps):

@@ -10,6 +10,7 @@ Outputs:
     rstsr_amd/_rstsr<ext>.so             pybind11 module over the C++ front end (links the library)
     oracle/liboracle.so                  CPU restatement used only by tests and benchmarks
     tests/gpu_check/libdot2check.so      GPU Dot2 GEMM checker used only by tests and benchmarks
+    tests/gpu_check/libstreamcheck.so    HBM stream-ceiling probe used only by the benchmark
 """
 import os
 import subprocess
@@ -110,17 +111,19 @@ def build_front(force=True):
 
 CHECKER_SRC = os.path.join(ROOT, "tests", "gpu_check", "dot2_check.hip")
 CHECKER_LIB = os.path.join(ROOT, "tests", "gpu_check", "libdot2check.so")
+STREAM_SRC = os.path.join(ROOT, "tests", "gpu_check", "stream_ceiling.hip")
+STREAM_LIB = os.path.join(ROOT, "tests", "gpu_check", "libstreamcheck.so")
 
 
 def build_checker(force=True):
-    """GPU Dot2 GEMM checker (test infrastructure for the GEMM error measurements)."""
-    if not os.path.exists(CHECKER_SRC):
-        return None
-    if not force and not _newer(CHECKER_LIB, [CHECKER_SRC]):
-        return CHECKER_LIB
-    tmp = CHECKER_LIB + ".tmp"
-    _run([HIPCC] + HIP_FLAGS + ["-ffp-contract=off", "-shared", "-o", tmp, CHECKER_SRC])
-    os.replace(tmp, CHECKER_LIB)
+    """GPU Dot2 GEMM checker and HBM stream-ceiling probe (test infrastructure for the
+    GEMM error and roofline measurements)."""
+    for src, out, extra in ((CHECKER_SRC, CHECKER_LIB, ["-ffp-contract=off"]), (STREAM_SRC, STREAM_LIB, [])):
+        if not os.path.exists(src) or (not force and not _newer(out, [src])):
+            continue
+        tmp = out + ".tmp"
+        _run([HIPCC] + HIP_FLAGS + extra + ["-shared", "-o", tmp, src])
+        os.replace(tmp, out)
     return CHECKER_LIB
 
 
