@@ -55,6 +55,7 @@ def parse():
     p.add_argument("--gemm-only", action="store_true", help="skip the strided suite (profiling the GEMM legs)")
     p.add_argument("--gemm-legs", default="dense,batch,ri,grouped,chain")
     p.add_argument("--no-transfer", action="store_true")
+    p.add_argument("--no-small", action="store_true", help="skip the small-configuration leg (profiling)")
     return p.parse_args()
 
 
@@ -285,7 +286,7 @@ def main():
 
     # ---------------- the small configuration, whole (GPU and host port) ----------------
     small = {}
-    if not args.only:
+    if not args.only and not args.no_small:
         small = c1_leg(dev, rt, args)
 
     # ---------------- FP64 GEMM leg ----------------
@@ -346,8 +347,9 @@ def _stream_ceilings(nbytes=1 << 30):
     return out
 
 
-def _timed(ev, fn, reps):
-    fn()  # warmup (also settles the autotuner)
+def _timed(ev, fn, reps, warmups=3):
+    for _ in range(warmups):
+        fn()  # warmup (also settles the autotuner)
     ev.record(4)
     for _ in range(reps):
         fn()
@@ -521,7 +523,9 @@ def gemm_leg(args, dev, rt):
     res = {"count": 512, "flop": flop}
     refs = [dot2_matmul(rt, dev, ta, tb) for ta, tb, _ in probs] if "grouped" in legs else []
     for mode in (modes if "grouped" in legs else []):
-        ms = _timed(ev, lambda: rt.gemm_grouped(probs, mode=mode), 3)
+        # six warm-up calls: the error checks of the previous mode leave the GPU idle long
+        # enough for its clock to drop, and it takes about six 3.5 ms calls to come back
+        ms = _timed(ev, lambda: rt.gemm_grouped(probs, mode=mode), 3, warmups=6)
         # every problem, every element: the largest normwise error over the 512 products
         err = max(normwise(dev, tc, r) for (_, _, tc), r in zip(probs, refs))
         res[mode] = _gemm_rates(flop, ms, mode, MODULI * flop)
@@ -532,7 +536,7 @@ def gemm_leg(args, dev, rt):
         # split kernels forced on every problem, for the record
         os.environ["RSTSR_HIP_OZAKI_GROUPED"] = "split"
         try:
-            ms = _timed(ev, lambda: rt.gemm_grouped(probs, mode="ozaki"), 3)
+            ms = _timed(ev, lambda: rt.gemm_grouped(probs, mode="ozaki"), 3, warmups=6)
             err = max(normwise(dev, tc, r) for (_, _, tc), r in zip(probs, refs))
         finally:
             del os.environ["RSTSR_HIP_OZAKI_GROUPED"]
