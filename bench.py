@@ -328,21 +328,25 @@ def main():
 
 
 def _stream_ceilings(nbytes=1 << 30):
-    """Best GB/s of a contiguous 16-byte-lane copy, read-only and write-only pass over 1 GiB
-    (tests/gpu_check/stream_ceiling.hip): what this GPU's HBM sustains for each traffic mix."""
+    """Best GB/s of a contiguous 16-byte-lane copy, read-only and write-only pass over 1 GiB, with
+    ordinary and with nontemporal accesses (tests/gpu_check/stream_ceiling.hip): what this GPU's
+    HBM sustains for each traffic mix."""
     path = os.path.join(ROOT, "tests", "gpu_check", "libstreamcheck.so")
     if not os.path.exists(path):
         log("[bench] stream-ceiling probe not built; skipped")
         return None
     lib = C.CDLL(path)
-    lib.stream_ceiling.argtypes = [C.c_int, C.c_size_t, C.c_int, C.POINTER(C.c_float)]
+    lib.stream_ceiling.argtypes = [C.c_int, C.c_int, C.c_size_t, C.c_int, C.POINTER(C.c_float)]
     out = {}
-    for kind, name, mult in ((0, "copy_GBps", 2), (1, "read_GBps", 1), (2, "write_GBps", 1)):
-        ms = C.c_float(0)
-        rc = lib.stream_ceiling(kind, nbytes, 10, C.byref(ms))
-        if rc != 0:
-            raise RuntimeError(f"stream_ceiling({kind}) failed with HIP error {rc}")
-        out[name] = round(mult * nbytes / 1e9 / (ms.value / 1e3), 1)
+    for kind, name, mult in ((0, "copy", 2), (1, "read", 1), (2, "write", 1)):
+        for nt in (0, 1):
+            ms = C.c_float(0)
+            rc = lib.stream_ceiling(kind, nt, nbytes, 10, C.byref(ms))
+            if rc != 0:
+                raise RuntimeError(f"stream_ceiling({kind}, {nt}) failed with HIP error {rc}")
+            out[f"{name}_{'nt' if nt else 'plain'}_GBps"] = round(mult * nbytes / 1e9 / (ms.value / 1e3), 1)
+        # the ceiling: the better of ordinary and nontemporal accesses
+        out[f"{name}_GBps"] = max(out[f"{name}_plain_GBps"], out[f"{name}_nt_GBps"])
     out["bytes_per_buffer"] = nbytes
     return out
 

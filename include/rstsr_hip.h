@@ -327,9 +327,10 @@ typedef struct {
 int rstsr_hip_gemm_grouped(rstsr_hip_ctx_t* ctx, rstsr_dtype_t t, rstsr_gemm_mode_t mode, size_t count,
                            const rstsr_gemm_desc_t* descs, const void* alpha, const void* beta);
 
-/* ---- BLAS-style drivers (cblas flag values of rstsr-common/src/flags.rs:62-202: order 101 C / 102 F,
+/* ---- BLAS-style drivers (cblas flag values of rstsr-common/src/flags.rs:62-213: order 101 C / 102 F,
  * trans 111 N / 112 T / 113 C (conjugate transpose) / 114 CN (conjugate only), uplo 121 U / 122 L,
- * side 141 L / 142 R). Dtypes f32, f64, c64, c128; mode as for rstsr_hip_gemm. ---- */
+ * diag 131 N (non-unit) / 132 U (unit), side 141 L / 142 R). Dtypes f32, f64, c64, c128; mode as
+ * for rstsr_hip_gemm. ---- */
 /* GEMMDriverAPI::driver_gemm (rstsr-blas-traits/src/blas3/gemm.rs:4-20):
  * C (m x n) = alpha op(A) op(B) + beta C, leading dimensions as in BLAS (at least the stored row
  * length for order C, column length for order F). Conjugated complex operands are copied
@@ -343,6 +344,13 @@ int rstsr_hip_driver_gemm(rstsr_hip_ctx_t* ctx, rstsr_dtype_t t, rstsr_gemm_mode
 int rstsr_hip_driver_syhemm(rstsr_hip_ctx_t* ctx, rstsr_dtype_t t, rstsr_gemm_mode_t mode, int hermi, int order,
                             int side, int uplo, size_t m, size_t n, const void* alpha, const void* a, size_t lda,
                             const void* b, size_t ldb, const void* beta, void* c, size_t ldc);
+/* TRSMDriverAPI::driver_trsm (blas3/trsm.rs:4-19): solves op(A) X = alpha B (side L, A m x m) or
+ * X op(A) = alpha B (side R, A n x n) for triangular A, B (m x n) overwritten by X; only the uplo
+ * triangle of A is read, and with diag U not its diagonal (taken as one). alpha = 0 sets B to
+ * zero without reading it. The trailing updates of the blocked solve run on the GEMM of `mode`. */
+int rstsr_hip_driver_trsm(rstsr_hip_ctx_t* ctx, rstsr_dtype_t t, rstsr_gemm_mode_t mode, int order, int side,
+                          int uplo, int transa, int diag, size_t m, size_t n, const void* alpha, const void* a,
+                          size_t lda, void* b, size_t ldb);
 
 /* ---- hipGraph capture of the context stream (no counterpart in the reference) ----
  * Everything enqueued between begin and end is recorded, not run; entry points that hand

@@ -68,6 +68,8 @@ def lib():
                                   C.c_size_t, vp, vp, C.c_size_t, vp, C.c_size_t, vp, vp, C.c_size_t],
         "rstsr_hip_driver_syhemm": [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_size_t,
                                     C.c_size_t, vp, vp, C.c_size_t, vp, C.c_size_t, vp, vp, C.c_size_t],
+        "rstsr_hip_driver_trsm": [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_size_t,
+                                  C.c_size_t, vp, vp, C.c_size_t, vp, C.c_size_t],
         "rstsr_hip_memcpy_h2d_mode": [vp, vp, vp, C.c_size_t, C.c_int],
         "rstsr_hip_memcpy_d2h_mode": [vp, vp, vp, C.c_size_t, C.c_int],
         "rstsr_hip_memcpy_h2d": [vp, vp, vp, C.c_size_t],

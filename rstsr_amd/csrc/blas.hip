@@ -1,8 +1,9 @@
 // BLAS-style driver entry points over the MFMA GEMM: GEMMDriverAPI::driver_gemm
-// (rstsr-blas-traits/src/blas3/gemm.rs:4-20) and SYHEMMDriverAPI::driver_syhemm
-// (blas3/syhemm.rs:4-19), with cblas flag values (FlagOrder C = 101 / F = 102,
-// FlagTrans N = 111 / T = 112 / C = 113 / CN = 114, FlagUpLo U = 121 / L = 122,
-// FlagSide L = 141 / R = 142; rstsr-common/src/flags.rs:62-202).
+// (rstsr-blas-traits/src/blas3/gemm.rs:4-20), SYHEMMDriverAPI::driver_syhemm
+// (blas3/syhemm.rs:4-19) and TRSMDriverAPI::driver_trsm (blas3/trsm.rs:4-19), with
+// cblas flag values (FlagOrder C = 101 / F = 102, FlagTrans N = 111 / T = 112 /
+// C = 113 / CN = 114, FlagUpLo U = 121 / L = 122, FlagDiag N = 131 / U = 132,
+// FlagSide L = 141 / R = 142; rstsr-common/src/flags.rs:62-213).
 //
 // Storage order and transposition become (row stride, column stride) pairs of
 // op(A), op(B) and C for the strided GEMM, so no operand is copied, except:
@@ -29,7 +30,7 @@ void run_gemm(rstsr_hip_ctx* ctx, rstsr_dtype_t t, rstsr_gemm_mode_t mode, const
 namespace {
 
 enum : int { ORD_C = 101, ORD_F = 102, TR_N = 111, TR_T = 112, TR_C = 113, TR_CN = 114 };
-enum : int { UPLO_U = 121, UPLO_L = 122, SIDE_L = 141, SIDE_R = 142 };
+enum : int { UPLO_U = 121, UPLO_L = 122, DIAG_N = 131, DIAG_U = 132, SIDE_L = 141, SIDE_R = 142 };
 
 template <class T>
 __device__ __forceinline__ T conj_v(T x) {
@@ -66,6 +67,81 @@ __global__ void __launch_bounds__(256) k_sym_expand(T* __restrict__ dst, const T
             }
         }
         dst[e] = v;
+    }
+}
+
+// complex-aware arithmetic of the triangular solve
+template <class T>
+__device__ __forceinline__ T t_mul(T a, T b) {
+    if constexpr (std::is_same<T, c128>::value || std::is_same<T, c64>::value)
+        return T{a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+    else return a * b;
+}
+template <class T>
+__device__ __forceinline__ T t_sub(T a, T b) {
+    if constexpr (std::is_same<T, c128>::value || std::is_same<T, c64>::value) return T{a.re - b.re, a.im - b.im};
+    else return a - b;
+}
+template <class T>
+__device__ __forceinline__ T t_div(T a, T b) {
+    if constexpr (std::is_same<T, c128>::value || std::is_same<T, c64>::value) {
+        // a conj(b) / |b|^2 with |b|^2 scaled by the larger part of b (no overflow for moderate b)
+        const auto s = fmax(fabs(b.re), fabs(b.im));
+        const auto br = b.re / s, bi = b.im / s, d = br * br + bi * bi;
+        return T{(a.re * br + a.im * bi) / d / s, (a.im * br - a.re * bi) / d / s};
+    } else {
+        return a / b;
+    }
+}
+
+constexpr int TRSM_NB = 64;  // rows of a diagonal block solved in LDS
+
+// One diagonal block of the triangular solve, in place: X (kb x r, strides xrs / xcs) :=
+// T^-1 X for the kb x kb triangle T (strides trs / tcs; lower or upper; unit: diagonal taken
+// as one and not read). One thread per column of X (64 per block); T and the block's 64
+// columns staged in LDS, each column by forward (lower) or backward substitution.
+template <class T>
+__global__ void __launch_bounds__(TRSM_NB) k_trsm_diag(const T* __restrict__ Tm, int64_t trs, int64_t tcs, int lower,
+                                                      int unit, int kb, T* __restrict__ X, int64_t xrs, int64_t xcs,
+                                                      int64_t r) {
+    __shared__ T Ts[TRSM_NB][TRSM_NB + 1];
+    __shared__ T Xs[TRSM_NB][TRSM_NB];
+    const int j = threadIdx.x;
+    const int64_t col = (int64_t)blockIdx.x * TRSM_NB + j;
+    for (int e = j; e < kb * kb; e += TRSM_NB) {
+        const int i = e / kb, k = e - i * kb;
+        // only the stored triangle is read, and a unit diagonal not at all
+        const bool in = (lower ? k <= i : k >= i) && !(unit && k == i);
+        Ts[i][k] = in ? Tm[(int64_t)i * trs + (int64_t)k * tcs] : T{};
+    }
+    for (int i = 0; i < kb; ++i) Xs[i][j] = col < r ? X[(int64_t)i * xrs + col * xcs] : T{};
+    __syncthreads();
+    if (col >= r) return;
+    if (lower) {
+        for (int i = 0; i < kb; ++i) {
+            T acc = Xs[i][j];
+            for (int k = 0; k < i; ++k) acc = t_sub(acc, t_mul(Ts[i][k], Xs[k][j]));
+            Xs[i][j] = unit ? acc : t_div(acc, Ts[i][i]);
+        }
+    } else {
+        for (int i = kb - 1; i >= 0; --i) {
+            T acc = Xs[i][j];
+            for (int k = i + 1; k < kb; ++k) acc = t_sub(acc, t_mul(Ts[i][k], Xs[k][j]));
+            Xs[i][j] = unit ? acc : t_div(acc, Ts[i][i]);
+        }
+    }
+    for (int i = 0; i < kb; ++i) X[(int64_t)i * xrs + col * xcs] = Xs[i][j];
+}
+
+// X (rows x cols, strides rs / cs) *= alpha, or := 0 when zero (BLAS: B is not read)
+template <class T>
+__global__ void __launch_bounds__(256) k_scale2d(T* __restrict__ X, uint32_t rows, uint32_t cols, int64_t rs,
+                                                 int64_t cs, T alpha, int zero) {
+    const uint64_t n = (uint64_t)rows * cols;
+    for (uint64_t e = (uint64_t)blockIdx.x * 256 + threadIdx.x; e < n; e += (uint64_t)gridDim.x * 256) {
+        const uint32_t i = (uint32_t)(e / cols), j = (uint32_t)(e - (uint64_t)i * cols);
+        T* p = X + (int64_t)i * rs + (int64_t)j * cs;
+        *p = zero ? T{} : t_mul(alpha, *p);
     }
 }
 
@@ -244,6 +320,110 @@ static void driver_syhemm(rstsr_hip_ctx* ctx, rstsr_dtype_t t, rstsr_gemm_mode_t
     run_gemm(ctx, t, mode, P, alpha, beta);
 }
 
+// driver_trsm: op(A) X = alpha B (side L, A m x m) or X op(A) = alpha B (side R, A n x n), B (m x n)
+// overwritten by X; A triangular (its `uplo` triangle read; diag U: unit diagonal, not read).
+// Every case becomes T Y = alpha Z with T lower or upper: side R solves op(A)^T X^T = alpha B^T, and
+// a transposition swaps T's strides and its triangle; a conjugated A (trans C or CN) is copied
+// conjugated into scratch first. T Y = Z is solved recursively: halves of the rows, the
+// trailing half updated by the GEMM (alpha -1, beta 1) with K the leading half's size, down to
+// 64-row diagonal blocks solved in LDS (k_trsm_diag).
+static void driver_trsm(rstsr_hip_ctx* ctx, rstsr_dtype_t t, rstsr_gemm_mode_t mode, int order, int side, int uplo,
+                        int transa, int diag, size_t m, size_t n, const void* alpha, const void* a, size_t lda, void* b,
+                        size_t ldb) {
+    if (order != ORD_C && order != ORD_F) RSTSR_RAISE(InvalidValue, "order must be 101 (C) or 102 (F)");
+    if (side != SIDE_L && side != SIDE_R) RSTSR_RAISE(InvalidValue, "side must be 141 (L) or 142 (R)");
+    if (uplo != UPLO_U && uplo != UPLO_L) RSTSR_RAISE(InvalidValue, "uplo must be 121 (U) or 122 (L)");
+    if (diag != DIAG_N && diag != DIAG_U) RSTSR_RAISE(InvalidValue, "diag must be 131 (N) or 132 (U)");
+    if (transa < TR_N || transa > TR_CN) RSTSR_RAISE(InvalidValue, "trans must be 111 (N), 112 (T), 113 (C) or 114 (CN)");
+    dispatch_gemm_dtype(t, [](auto) {});  // BLAS types only
+    const size_t es = dtype_size(t);
+    const size_t s = side == SIDE_L ? m : n;  // order of A
+    check_ld(order, s, s, lda, "a");
+    check_ld(order, m, n, ldb, "b");
+    if (m == 0 || n == 0) return;
+    int64_t ars, acs, brs, bcs;
+    stored_strides(order, (int64_t)lda, ars, acs);
+    stored_strides(order, (int64_t)ldb, brs, bcs);
+    check_mat(a, s, s, ars, acs, es, "a");
+    check_mat(b, m, n, brs, bcs, es, "b");
+    // T = op(A) (side L) or op(A)^T (side R); Y = X (rows s = m) or X^T (rows s = n)
+    const bool trans_op = transa == TR_T || transa == TR_C;
+    const bool tr = trans_op != (side == SIDE_R);
+    const bool conj = is_cplx(t) && (transa == TR_C || transa == TR_CN);
+    int64_t trs = tr ? acs : ars, tcs = tr ? ars : acs;
+    const bool lower = (uplo == UPLO_L) != tr;
+    const int64_t yrs = side == SIDE_L ? brs : bcs, ycs = side == SIDE_L ? bcs : brs;
+    const int64_t r = (int64_t)(side == SIDE_L ? n : m);  // columns of Y
+    std::lock_guard<std::mutex> g(ctx->mu);
+    dispatch_gemm_dtype(t, [&](auto tag) {
+        using T = decltype(tag);
+        const T* Tm = (const T*)a;
+        if (conj) {
+            T* d = (T*)aux_buffer(ctx, s * s * es + 256);
+            hipLaunchKernelGGL(k_conj_copy<T>, dim3(grid_for((uint64_t)s * s)), dim3(256), 0, ctx->stream, d, Tm,
+                               (uint32_t)s, (uint32_t)s, trs, tcs);
+            check_launch("conjugate copy");
+            Tm = d;
+            trs = (int64_t)s;
+            tcs = 1;
+        }
+        T* Y = (T*)b;
+        T al;
+        std::memcpy(&al, alpha, sizeof(T));
+        T one{}, minus_one{};
+        if constexpr (std::is_same<T, c128>::value || std::is_same<T, c64>::value) {
+            one.re = 1;
+            minus_one.re = -1;
+        } else {
+            one = 1;
+            minus_one = -1;
+        }
+        const bool is_one = std::memcmp(&al, &one, sizeof(T)) == 0;
+        const T zero{};
+        const bool is_zero = std::memcmp(&al, &zero, sizeof(T)) == 0;
+        if (!is_one) {
+            hipLaunchKernelGGL(k_scale2d<T>, dim3(grid_for((uint64_t)s * r)), dim3(256), 0, ctx->stream, Y,
+                               (uint32_t)s, (uint32_t)r, yrs, ycs, al, is_zero ? 1 : 0);
+            check_launch("trsm scale");
+            if (is_zero) return;
+        }
+        // rows [lo, hi) of the solve
+        auto solve = [&](auto&& self, int64_t lo, int64_t hi) -> void {
+            const int64_t len = hi - lo;
+            if (len <= TRSM_NB) {
+                hipLaunchKernelGGL(k_trsm_diag<T>, dim3((uint32_t)((r + TRSM_NB - 1) / TRSM_NB)), dim3(TRSM_NB), 0,
+                                   ctx->stream, Tm + lo * trs + lo * tcs, trs, tcs, lower ? 1 : 0,
+                                   diag == DIAG_U ? 1 : 0, (int)len, Y + lo * yrs, yrs, ycs, r);
+                check_launch("trsm diagonal block");
+                return;
+            }
+            const int64_t mid = lo + ((len / 2 + TRSM_NB - 1) / TRSM_NB) * TRSM_NB;
+            // first the half the other depends on, then its update of the other half, then that half
+            const int64_t f0 = lower ? lo : mid, f1 = lower ? mid : hi;  // solved first
+            const int64_t s0 = lower ? mid : lo, s1 = lower ? hi : mid;  // updated, solved second
+            self(self, f0, f1);
+            GemmProblem P{};
+            P.M = s1 - s0;
+            P.N = r;
+            P.K = f1 - f0;
+            P.bd.nbd = 0;
+            P.nbatch = 1;
+            P.a = Tm + s0 * trs + f0 * tcs;
+            P.rsa = trs;
+            P.csa = tcs;
+            P.b = Y + f0 * yrs;
+            P.rsb = yrs;
+            P.csb = ycs;
+            P.c = Y + s0 * yrs;
+            P.rsc = yrs;
+            P.csc = ycs;
+            run_gemm(ctx, t, mode, P, &minus_one, &one);
+            self(self, s0, s1);
+        };
+        solve(solve, 0, (int64_t)s);
+    });
+}
+
 }  // namespace hip
 }  // namespace rstsr
 
@@ -267,6 +447,15 @@ int rstsr_hip_driver_syhemm(rstsr_hip_ctx_t* ctx, rstsr_dtype_t t, rstsr_gemm_mo
     return guarded([&] {
         bind(ctx);
         driver_syhemm(ctx, t, mode, hermi, order, side, uplo, m, n, alpha, a, lda, b, ldb, beta, c, ldc);
+    });
+}
+
+int rstsr_hip_driver_trsm(rstsr_hip_ctx_t* ctx, rstsr_dtype_t t, rstsr_gemm_mode_t mode, int order, int side,
+                          int uplo, int transa, int diag, size_t m, size_t n, const void* alpha, const void* a,
+                          size_t lda, void* b, size_t ldb) {
+    return guarded([&] {
+        bind(ctx);
+        driver_trsm(ctx, t, mode, order, side, uplo, transa, diag, m, n, alpha, a, lda, b, ldb);
     });
 }
 

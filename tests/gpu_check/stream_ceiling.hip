@@ -1,7 +1,8 @@
 // Stream ceilings of one GPU's HBM, for the benchmark's roofline record (test infrastructure,
 // not part of the library): a contiguous copy (16-byte lanes, one per thread, a thread per
-// lane of the whole array), a read-only pass and a write-only pass, each timed with events
-// over repeated launches on buffers this file allocates and frees.
+// lane of the whole array), a read-only pass and a write-only pass, with ordinary or
+// nontemporal accesses, each timed with events over repeated launches on buffers this file
+// allocates and frees.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -9,24 +10,34 @@
 
 typedef float f4 __attribute__((ext_vector_type(4)));
 
+template <bool NT>
 __global__ void __launch_bounds__(256) k_copy(const f4* __restrict__ a, f4* __restrict__ c, size_t n) {
     const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
-    if (i < n) c[i] = a[i];
+    if (i >= n) return;
+    if constexpr (NT) __builtin_nontemporal_store(__builtin_nontemporal_load(a + i), c + i);
+    else c[i] = a[i];
 }
+template <bool NT>
 __global__ void __launch_bounds__(256) k_read(const f4* __restrict__ a, f4* __restrict__ out, size_t n) {
     const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
-    f4 v = i < n ? a[i] : f4{0, 0, 0, 0};
+    f4 v = f4{0, 0, 0, 0};
+    if (i < n) v = NT ? __builtin_nontemporal_load(a + i) : a[i];
     // a value no input here produces keeps the loads without a store per lane
     if (v.x == 1.5e38f && v.y == -1.5e38f) out[blockIdx.x] = v;
 }
+template <bool NT>
 __global__ void __launch_bounds__(256) k_write(f4* __restrict__ c, size_t n) {
     const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
-    if (i < n) c[i] = f4{1.0f, 2.0f, 3.0f, 4.0f};
+    if (i >= n) return;
+    const f4 v = f4{1.0f, 2.0f, 3.0f, 4.0f};
+    if constexpr (NT) __builtin_nontemporal_store(v, c + i);
+    else c[i] = v;
 }
 
-// kind 0: copy, 1: read, 2: write, over `bytes` (a multiple of 4096); best of `reps` launches
-// after two warm-ups, in ms, to *best_ms. Returns a HIP error code (0 on success).
-extern "C" int stream_ceiling(int kind, size_t bytes, int reps, float* best_ms) {
+// kind 0: copy, 1: read, 2: write; nt: nontemporal loads / stores. Over `bytes` (a multiple of
+// 4096); best of `reps` launches after two warm-ups, in ms, to *best_ms. Returns a HIP error code
+// (0 on success).
+extern "C" int stream_ceiling(int kind, int nt, size_t bytes, int reps, float* best_ms) {
     const size_t n = bytes / 16;
     f4 *a = nullptr, *c = nullptr;
     hipError_t e = hipMalloc(&a, bytes);
@@ -39,9 +50,9 @@ extern "C" int stream_ceiling(int kind, size_t bytes, int reps, float* best_ms) 
     std::vector<float> t;
     for (int r = 0; r < reps + 2 && e == hipSuccess; ++r) {
         e = hipEventRecord(e0, 0);
-        if (kind == 0) hipLaunchKernelGGL(k_copy, dim3(grid), dim3(256), 0, 0, a, c, n);
-        else if (kind == 1) hipLaunchKernelGGL(k_read, dim3(grid), dim3(256), 0, 0, a, c, n);
-        else hipLaunchKernelGGL(k_write, dim3(grid), dim3(256), 0, 0, c, n);
+        if (kind == 0) hipLaunchKernelGGL(nt ? k_copy<true> : k_copy<false>, dim3(grid), dim3(256), 0, 0, a, c, n);
+        else if (kind == 1) hipLaunchKernelGGL(nt ? k_read<true> : k_read<false>, dim3(grid), dim3(256), 0, 0, a, c, n);
+        else hipLaunchKernelGGL(nt ? k_write<true> : k_write<false>, dim3(grid), dim3(256), 0, 0, c, n);
         if (e == hipSuccess) e = hipGetLastError();
         if (e == hipSuccess) e = hipEventRecord(e1, 0);
         if (e == hipSuccess) e = hipEventSynchronize(e1);

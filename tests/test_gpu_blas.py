@@ -1,4 +1,4 @@
-"""BLAS-style drivers (rstsr_hip_driver_gemm / rstsr_hip_driver_syhemm) against NumPy.
+"""BLAS-style drivers (rstsr_hip_driver_gemm / rstsr_hip_driver_syhemm / rstsr_hip_driver_trsm) against NumPy.
 
 Bars:
   * vs NumPy in extended precision: componentwise |C - C*| <= (K + 2) u (|alpha| |op(A)| |op(B)| + |beta| |C0|),
@@ -200,3 +200,119 @@ def test_driver_errors(dev):
                                      4) != 0
     # integer dtypes are not BLAS types
     assert L.rstsr_hip_driver_gemm(ctx, 2, 0, ORD_C, 111, 111, 2, 2, 2, p1, ptr(x), 2, ptr(x), 2, p1, ptr(x), 2) != 0
+
+
+# ---------------- TRSMDriverAPI::driver_trsm (rstsr-blas-traits/src/blas3/trsm.rs:4-19) ----------------
+
+DIAG = {"N": 131, "U": 132}
+
+
+def _tri(rng, s, dt, uplo, diag):
+    """A well-conditioned triangular matrix (off-diagonal entries ~ 0.5/s, so each row's
+    off-diagonal sum stays well under its diagonal of modulus 1-2) and its BLAS storage image
+    with the unread parts (other triangle, and a unit diagonal) set to NaN, so that reading
+    them would show in the result."""
+    T = rnd(rng, (s, s), dt) * (0.5 / s)
+    d = 1.0 + rng.uniform(0, 1, s)
+    if np.dtype(dt).kind == "c":
+        d = d * np.exp(1j * rng.uniform(0, 2 * np.pi, s))
+    T[np.diag_indices(s)] = d.astype(dt)
+    T = np.tril(T) if uplo == "L" else np.triu(T)
+    S = T.copy()
+    S[np.triu_indices(s, 1) if uplo == "L" else np.tril_indices(s, -1)] = np.nan
+    if diag == "U":
+        S[np.diag_indices(s)] = np.nan
+        T[np.diag_indices(s)] = 1.0
+    return T, S
+
+
+def _trsm_ref(T, B, side, ta, alpha):
+    hi = np.complex128 if np.iscomplexobj(B) else np.float64
+    oa = op(T.astype(hi), ta)
+    rhs = alpha * B.astype(hi)
+    if side == "L":
+        return np.linalg.solve(oa, rhs)
+    return np.linalg.solve(oa.T, rhs.T).T
+
+
+def _run_trsm(dev, dt, order, side, uplo, ta, diag, m, n, alpha, S, B, lda, ldb, mode=0):
+    L = abi.lib()
+    ctx = C.c_void_p(dev._ctx_ptr())
+    da = rt.asarray(store(S, order, lda), device=dev)
+    db = rt.asarray(store(B, order, ldb), device=dev)
+    al = scal(dt, alpha)
+    r = L.rstsr_hip_driver_trsm(ctx, CODE[np.dtype(dt)], mode, order, SIDE[side], UPLO[uplo], TR[ta], DIAG[diag], m,
+                                n, al.ctypes.data_as(C.c_void_p), ptr(da), lda, ptr(db), ldb)
+    return r, db
+
+
+@pytest.mark.parametrize("dt", [np.float64, np.complex128, np.float32, np.complex64])
+@pytest.mark.parametrize("order", [ORD_C, ORD_F])
+def test_driver_trsm_all_flags(dev, dt, order):
+    """Every side x uplo x trans x diag, with padded leading dimensions: X against the extended-
+    precision solve of the same triangular system (normwise, 64 s u), the unread parts of A
+    (NaN) never used, B's padding untouched. s = 70 crosses one 64-row diagonal block."""
+    rng = np.random.default_rng(11)
+    m, n = 70, 45
+    alpha = (0.75 - 0.5j) if np.dtype(dt).kind == "c" else 0.75
+    for side in ("L", "R"):
+        s = m if side == "L" else n
+        for uplo in ("L", "U"):
+            for ta in ("N", "T", "C", "CN"):
+                for diag in ("N", "U"):
+                    T, S = _tri(rng, s, dt, uplo, diag)
+                    B = rnd(rng, (m, n), dt)
+                    lda = s + 3
+                    ldb = (n if order == ORD_C else m) + 2
+                    r, db = _run_trsm(dev, dt, order, side, uplo, ta, diag, m, n, alpha, S, B, lda, ldb)
+                    assert r == 0, abi.lib().rstsr_hip_last_error()
+                    got = load(db.to_numpy(), (m, n), order, ldb)
+                    ref = _trsm_ref(T, B, side, ta, alpha)
+                    err = np.max(np.abs(got - ref)) / np.max(np.abs(ref))
+                    assert err <= 64 * s * U[np.dtype(dt)], (side, uplo, ta, diag, err)
+                    flat = db.to_numpy()
+                    pad = np.isnan(store(np.zeros((m, n), dt), order, ldb))
+                    assert np.all(np.isnan(flat[pad])), (side, uplo, ta, diag)
+
+
+@pytest.mark.parametrize("dt", [np.float64, np.complex128])
+@pytest.mark.parametrize("side", ["L", "R"])
+def test_driver_trsm_large_blocked(dev, dt, side):
+    """A solve of order 1000 (recursive halves down to 64-row blocks, trailing updates on the
+    MFMA GEMM), plain and split GEMM modes: normwise against the extended-precision solve."""
+    rng = np.random.default_rng(12)
+    m, n = (1000, 300) if side == "L" else (300, 1000)
+    s = m if side == "L" else n
+    for uplo, ta in (("L", "N"), ("U", "T")):
+        T, S = _tri(rng, s, dt, uplo, "N")
+        B = rnd(rng, (m, n), dt)
+        ref = _trsm_ref(T, B, side, ta, 1.0)
+        for mode in (0, 1):
+            r, db = _run_trsm(dev, dt, ORD_C, side, uplo, ta, "N", m, n, 1.0, S, B, s, n, mode)
+            assert r == 0, abi.lib().rstsr_hip_last_error()
+            got = load(db.to_numpy(), (m, n), ORD_C, n)
+            err = np.max(np.abs(got - ref)) / np.max(np.abs(ref))
+            assert err <= 1e-12, (uplo, ta, mode, err)
+
+
+def test_driver_trsm_alpha_zero_and_errors(dev):
+    """alpha = 0 sets B to zero without reading it (NaN in B); bad flags and leading dimensions
+    are errors, with B untouched."""
+    rng = np.random.default_rng(13)
+    T, S = _tri(rng, 8, np.float64, "L", "N")
+    B = np.full((8, 5), np.nan)
+    r, db = _run_trsm(dev, np.float64, ORD_C, "L", "L", "N", "N", 8, 5, 0.0, S, B, 8, 5)
+    assert r == 0 and np.array_equal(db.to_numpy(), np.zeros(40))
+    L = abi.lib()
+    ctx = C.c_void_p(dev._ctx_ptr())
+    B = rnd(rng, (8, 5), np.float64)
+    da = rt.asarray(store(S, ORD_C, 8), device=dev)
+    db = rt.asarray(store(B, ORD_C, 5), device=dev)
+    one = scal(np.float64, 1.0)
+    for args in ((ORD_C, 141, 999, 111, 131, 8), (ORD_C, 141, 122, 111, 999, 8), (ORD_C, 999, 122, 111, 131, 8),
+                 (ORD_C, 141, 122, 111, 131, 4)):  # bad uplo, diag, side; lda < 8
+        o, sd, ul, tr, dg, lda = args
+        r = L.rstsr_hip_driver_trsm(ctx, 1, 0, o, sd, ul, tr, dg, 8, 5, one.ctypes.data_as(C.c_void_p), ptr(da), lda,
+                                    ptr(db), 5)
+        assert r != 0, args
+    assert np.array_equal(db.to_numpy(), B.reshape(-1))
