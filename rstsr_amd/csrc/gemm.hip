@@ -125,7 +125,10 @@ __device__ __forceinline__ void mfma_tile(const TileArgs<T>& g, uint32_t tm, uin
 
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
-    const int wm = wave / CF::WVN, wn = wave % CF::WVN;
+    // wave -> (row group, column group): waves w and w + 4 share a SIMD, so the column group is
+    // rotated by w / 4 to give each SIMD waves of different column groups as well as rows (an edge
+    // tile's padding then idles every SIMD alike)
+    const int wm = wave / CF::WVN, wn = (wave % CF::WVN + wave / 4) % CF::WVN;
     const int64_t m0 = (int64_t)tm * BM, n0 = (int64_t)tn * BN;
 
     dbl4 acc[NP][FM][FN];
@@ -214,7 +217,8 @@ __device__ __forceinline__ void mfma_tile(const TileArgs<T>& g, uint32_t tm, uin
     // uniform, read through a scalar register so the tests below are scalar branches): edge
     // tiles skip the MFMAs of fragments wholly in the padding
     const int wv = __builtin_amdgcn_readfirstlane(wave);
-    const int64_t rows_left = g.M - (m0 + (wv / CF::WVN) * 16 * FM), cols_left = g.N - (n0 + (wv % CF::WVN) * 16 * FN);
+    const int64_t rows_left = g.M - (m0 + (wv / CF::WVN) * 16 * FM),
+                  cols_left = g.N - (n0 + ((wv % CF::WVN + wv / 4) % CF::WVN) * 16 * FN);
     const int fmv = rows_left <= 0 ? 0 : (int)std::min<int64_t>(FM, (rows_left + 15) / 16);
     const int fnv = cols_left <= 0 ? 0 : (int)std::min<int64_t>(FN, (cols_left + 15) / 16);
 
