@@ -419,20 +419,25 @@ __global__ void __launch_bounds__(256) k_normsplit_rows(const double* __restrict
 // conflicts of the 16-byte fragment reads).
 // FR: the MFMA's output block, 16 (v_mfma_i32_16x16x64_i8) or 32 (v_mfma_i32_32x32x32_i8).
 // D: K slices whose global loads are in flight at once (register sets).
-template <int BM_, int BN_, int BK_, int WVM_, int WVN_, int OCC_, int PAD_, int FR_ = 16, int D_ = 1>
+// ST: 0 for loads staged through registers; otherwise the slices go straight from global memory
+// into a ring of ST LDS buffers (global_load_lds_dwordx4, unpadded swizzled rows), ST - 1 ahead.
+// SG: the K steps of a slice scheduled as fragment reads interleaved with the MFMAs before them.
+template <int BM_, int BN_, int BK_, int WVM_, int WVN_, int OCC_, int PAD_, int FR_ = 16, int D_ = 1, int ST_ = 0,
+          int SG_ = 0>
 struct I8Cfg {
     static constexpr int BM = BM_, BN = BN_, BK = BK_, WVM = WVM_, WVN = WVN_, OCC = OCC_, PAD = PAD_, FR = FR_,
-                         D = D_;
+                         D = D_, ST = ST_, SG = SG_;
     static constexpr int NT = 64 * WVM * WVN;
+    static constexpr int NBUF = ST > 2 ? ST : 2;
 };
 // products more than 128 wide: eight waves of 64 x 64, two blocks per CU
 using I8Wide8 = I8Cfg<128, 256, 64, 2, 4, 2, 32>;
 // small and grouped products: 64 x 64 per wave, two blocks per CU
 using I8Sq = I8Cfg<128, 128, 128, 2, 2, 2, 16>;
 // large products: 256 x 256 block tiles from eight waves of 64 x 128 (a third fewer LDS
-// fragment bytes per MFMA than 64 x 64 waves), 128-byte K slices in XOR-swizzled rows, one
-// block per CU
-using I8Big = I8Cfg<256, 256, 128, 4, 2, 1, 0>;
+// fragment bytes per MFMA than 64 x 64 waves), 128-byte K slices in XOR-swizzled rows loaded
+// straight into two LDS buffers, fragment reads interleaved with the MFMAs, one block per CU
+using I8Big = I8Cfg<256, 256, 128, 4, 2, 1, 0, 16, 1, 2, 1>;
 // planes: rows padded to a multiple of the tile's BM (I8_MPAD for all but the 256-row tiles),
 // K to I8_KPAD (every BK divides it)
 constexpr int I8_MPAD = 128, I8_KPAD = 128;
@@ -460,10 +465,17 @@ __device__ __forceinline__ int mod_small(int x, int m, double inv) {
     return r;
 }
 
+// this wave's vector memory operations down to at most N outstanding and its LDS operations
+// done, then the workgroup barrier; a compiler fence for memory on both sides
+template <int N>
+__device__ __forceinline__ void vm_wait_barrier() {
+    asm volatile("s_waitcnt vmcnt(%0) lgkmcnt(0)\n\ts_barrier" ::"n"(N) : "memory");
+}
+
 template <class CF>
 struct I8Smem {
-    int8_t As[2][CF::BM][CF::BK + CF::PAD];
-    int8_t Bs[2][CF::BN][CF::BK + CF::PAD];
+    int8_t As[CF::NBUF][CF::BM][CF::BK + CF::PAD];
+    int8_t Bs[CF::NBUF][CF::BN][CF::BK + CF::PAD];
 };
 
 // Output tile (tm, tn) of one residue product over a K range of Kc <= I8_KCHUNK
@@ -472,10 +484,10 @@ struct I8Smem {
 // < M and columns < N, added to the residues already there when `accum`. Each
 // wave owns (BM / WVM) x (BN / WVN) outputs from FR x FR MFMA blocks
 // (v_mfma_i32_16x16x64_i8 or v_mfma_i32_32x32x32_i8). K runs in slices of BK
-// bytes through two LDS buffers; global loads run CF::D slices ahead in
-// registers (D register sets), so a load has D slices of MFMA work to arrive.
-// The accumulators see nothing but MFMAs until the epilogue, so they stay in
-// AGPRs.
+// bytes through LDS buffers: with CF::ST == 0 global loads run CF::D slices ahead
+// in registers (D register sets) and two LDS buffers take them in turn, so a load
+// has D slices of MFMA work to arrive; with CF::ST > 0 the slices go straight
+// into a ring of ST buffers, ST - 1 slices ahead.
 // TOUT: each product block is formed transposed (B fragment as the MFMA's row
 // operand), so a lane ends with runs of four consecutive columns of one row of
 // R: one 4-byte store per run (N % 4 == 0), the residue by mod_u32. The 32 x 32
@@ -539,10 +551,12 @@ __device__ __forceinline__ void i8_tile(const int8_t* __restrict__ A, const int8
         for (int kk = 0; kk < BK; kk += FK) {
             v4i af[FM], bf[FN];
             const int kc = (((kk + fk) >> 4) ^ sr) << 4;
-#pragma unroll
-            for (int i = 0; i < FM; ++i) af[i] = *(const v4i*)&As[buf][wm * WTM + i * FR + fr][kc];
+            // program order a0, b0 .. b(FN-1), a1 .. a(FM-1): the first row of MFMAs needs the first reads
+            af[0] = *(const v4i*)&As[buf][wm * WTM + fr][kc];
 #pragma unroll
             for (int j = 0; j < FN; ++j) bf[j] = *(const v4i*)&Bs[buf][wn * WTN + j * FR + fr][kc];
+#pragma unroll
+            for (int i = 1; i < FM; ++i) af[i] = *(const v4i*)&As[buf][wm * WTM + i * FR + fr][kc];
 #pragma unroll
             for (int i = 0; i < FM; ++i)
 #pragma unroll
@@ -555,25 +569,86 @@ __device__ __forceinline__ void i8_tile(const int8_t* __restrict__ A, const int8
                         acc[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(af[i], bf[j], acc[i][j], 0, 0, 0);
                 }
         }
+        if constexpr (CF::SG) {
+            // instruction order for the scheduler: the first K step's fragment reads, then each
+            // later step's reads spread over the MFMAs of the step before it (two MFMAs per read),
+            // so fragments arrive while the matrix cores run
+            // SG 1: two MFMAs per read; SG 2: one; SG 3: two, and the first step starts its MFMAs
+            // after the reads of a0 and the b fragments, the other a reads between them
+            constexpr int NR = FM + FN, NMF = FM * FN, Q = CF::SG == 2 ? 1 : 2;
+            constexpr int E = CF::SG == 3 ? FM - 1 : 0;  // first-step reads moved between its MFMAs
+            static_assert(Q * (NR + E) <= NMF && BK == 2 * FK, "two K steps; reads fit between the MFMAs");
+            __builtin_amdgcn_sched_group_barrier(0x100, NR - E, 0);
+#pragma unroll
+            for (int r = 0; r < E + NR; ++r) {
+                __builtin_amdgcn_sched_group_barrier(0x008, Q, 0);
+                __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+            }
+            __builtin_amdgcn_sched_group_barrier(0x008, NMF - Q * (NR + E), 0);
+            __builtin_amdgcn_sched_group_barrier(0x008, NMF, 0);
+        }
     };
 
-    // slices 0 .. D-1 in flight, slice 0 in LDS; slice t uses register set t % D and LDS buffer t % 2
+    if constexpr (CF::ST > 0) {
+        // direct loads: one global_load_lds_dwordx4 fills 1 KiB of LDS (lane i at byte 16 i), RW
+        // whole rows of one wave; lane i takes chunk (i % CPR) ^ swz(row) of its row from global
+        // memory, so the LDS rows come out swizzled as the fragment reads expect (RW is a multiple
+        // of the swizzle period: swz(row) = swz(i / CPR)). Slice t goes to buffer t % ST, issued
+        // ST - 1 slices ahead into the buffer every wave finished reading at the last barrier.
+        constexpr int S = CF::ST, RW = 1024 / BK, RP = (NT / 64) * RW, NI = BM / RP + BN / RP;
+        static_assert(CF::PAD == 0 && RW % 8 == 0 && BM % RP == 0 && BN % RP == 0 && (S - 2) * NI < 64 && S >= 2,
+                      "direct-load tile shape");
+        const int wv = __builtin_amdgcn_readfirstlane(wave);
+        const int dr = lane / CPR, dch = ((lane % CPR) ^ swz(lane / CPR)) * 16;
+        auto dma = [&](int st, int64_t k0) {
 #pragma unroll
-    for (int s = 0; s < D; ++s)
-        if ((int64_t)s * BK < Kc) load(s, (int64_t)s * BK);
-    store(0, 0);
-    __syncthreads();
-    int buf = 0;
-    for (int64_t kb = 0; kb < Kc; kb += (int64_t)D * BK) {
+            for (int i = 0; i < BM / RP; ++i) {
+                const int r0 = i * RP + wv * RW;
+                __builtin_amdgcn_global_load_lds(Ab + (size_t)(r0 + dr) * ldk + k0 + dch, &As[st][r0][0], 16, 0, 0);
+            }
 #pragma unroll
-        for (int s = 0; s < D; ++s) {
-            const int64_t k0 = kb + (int64_t)s * BK;
-            if (k0 < Kc) {
-                if (k0 + (int64_t)D * BK < Kc) load(s, k0 + (int64_t)D * BK);
-                compute(buf);
-                if (k0 + BK < Kc) store((s + 1) % D, buf ^ 1);
-                __syncthreads();
-                buf ^= 1;
+            for (int i = 0; i < BN / RP; ++i) {
+                const int r0 = i * RP + wv * RW;
+                __builtin_amdgcn_global_load_lds(Bb + (size_t)(r0 + dr) * ldk + k0 + dch, &Bs[st][r0][0], 16, 0, 0);
+            }
+        };
+        const int64_t ns = Kc / BK;
+#pragma unroll
+        for (int s = 0; s < S - 1; ++s)
+            if (s < ns) dma(s, (int64_t)s * BK);
+        // slice 0 landed (the slices after it may still be in flight)
+        if (ns >= S - 1) vm_wait_barrier<(S - 2) * NI>();
+        else vm_wait_barrier<0>();
+        int st = 0, sn = S - 1;
+        for (int64_t t = 0; t < ns; ++t) {
+            const bool more = t + S - 1 < ns;
+            if (more) dma(sn, (t + S - 1) * BK);
+            compute(st);
+            // slice t + 1 landed, and no wave reads buffer st any more
+            if (more) vm_wait_barrier<(S - 2) * NI>();
+            else vm_wait_barrier<0>();
+            st = st + 1 == S ? 0 : st + 1;
+            sn = sn + 1 == S ? 0 : sn + 1;
+        }
+    } else {
+        // slices 0 .. D-1 in flight, slice 0 in LDS; slice t uses register set t % D and LDS buffer t % 2
+#pragma unroll
+        for (int s = 0; s < D; ++s)
+            if ((int64_t)s * BK < Kc) load(s, (int64_t)s * BK);
+        store(0, 0);
+        __syncthreads();
+        int buf = 0;
+        for (int64_t kb = 0; kb < Kc; kb += (int64_t)D * BK) {
+#pragma unroll
+            for (int s = 0; s < D; ++s) {
+                const int64_t k0 = kb + (int64_t)s * BK;
+                if (k0 < Kc) {
+                    if (k0 + (int64_t)D * BK < Kc) load(s, k0 + (int64_t)D * BK);
+                    compute(buf);
+                    if (k0 + BK < Kc) store((s + 1) % D, buf ^ 1);
+                    __syncthreads();
+                    buf ^= 1;
+                }
             }
         }
     }
@@ -1083,9 +1158,11 @@ static OzSetup ozaki_setup() {
 }
 
 // Tile shape of the residue products. Measured (14 moduli): at 8192^3 the 256 x 256
-// tile takes 6.7 ms against 8.0 ms for 128 x 256 from eight 64 x 64 waves; 128 x 256 from
-// four 64 x 128 waves, 256 x 256 from four 128 x 128 waves (which spill), 32 x 32 x 32
-// MFMAs and global loads two slices ahead were no faster. Products at most 256 long on a side
+// tile takes 6.2-6.4 ms (6.8-7.1 with loads staged through registers and the compiler's
+// read order) against 8.0 ms for 128 x 256 from eight 64 x 64 waves (7.8 with direct loads,
+// two blocks per CU); 128 x 256 from four 64 x 128 waves, 256 x 256 from four 128 x 128 waves
+// (which spill), 32 x 32 x 32 MFMAs, global loads two slices ahead and 64-byte slices in
+// three or four LDS buffers were no faster. Products at most 256 long on a side
 // keep the smaller tiles (less padding); outputs at most 128 wide the 128 x 128 tile.
 // RSTSR_HIP_OZAKI_TILE (square, wide8, big) overrides the choice.
 enum class I8Tile { Sq, Wide8, Big };
