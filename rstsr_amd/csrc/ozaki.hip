@@ -472,6 +472,12 @@ __device__ __forceinline__ void vm_wait_barrier() {
     asm volatile("s_waitcnt vmcnt(%0) lgkmcnt(0)\n\ts_barrier" ::"n"(N) : "memory");
 }
 
+// Moduli of one launch
+struct ModList {
+    int m[OZ_MAXMOD];
+    uint32_t magic[OZ_MAXMOD];  // ceil(2^32 / m)
+};
+
 template <class CF>
 struct I8Smem {
     int8_t As[CF::NBUF][CF::BM][CF::BK + CF::PAD];
@@ -492,11 +498,15 @@ struct I8Smem {
 // operand), so a lane ends with runs of four consecutive columns of one row of
 // R: one 4-byte store per run (N % 4 == 0), the residue by mod_u32. The 32 x 32
 // blocks always use it.
-template <class CF, bool ROW4G, bool TOUT = false>
+// MM (direct loads, TOUT, no accum; R holds every row and column of the tile grid, ldr a
+// multiple of 4): the tile's products for nmod moduli (planes pA, pB, pR apart; moduli
+// ml->m[l0 ..]) in one call, the residues of padding rows and columns written too.
+template <class CF, bool ROW4G, bool TOUT = false, bool MM = false>
 __device__ __forceinline__ void i8_tile(const int8_t* __restrict__ A, const int8_t* __restrict__ B,
                                         uint8_t* __restrict__ R, int64_t M, int64_t N, int64_t ldk, int64_t Kc,
                                         int64_t ldr, int mod, bool accum, uint32_t tm, uint32_t tn, I8Smem<CF>& sm,
-                                        uint32_t magic = 0) {
+                                        uint32_t magic = 0, int nmod = 1, int64_t pA = 0, int64_t pB = 0,
+                                        int64_t pR = 0, const ModList* ml = nullptr, int l0 = 0) {
     constexpr int BM = CF::BM, BN = CF::BN, BK = CF::BK, NT = CF::NT, FR = CF::FR, D = CF::D;
     constexpr bool T = TOUT || FR == 32;
     constexpr int FK = FR == 16 ? 64 : 32;  // bytes of K per MFMA
@@ -547,6 +557,7 @@ __device__ __forceinline__ void i8_tile(const int8_t* __restrict__ A, const int8
     static_assert(RPP % 8 == 0 && FR % 8 == 0 && WTM % 8 == 0 && WTN % 8 == 0, "swizzle period");
     const int sr = swz(fr);  // fragment rows start at multiples of 8
     auto compute = [&](int buf) {
+        if constexpr (CF::SG) __builtin_amdgcn_sched_barrier(0);  // the slice's reads and MFMAs alone
 #pragma unroll
         for (int kk = 0; kk < BK; kk += FK) {
             v4i af[FM], bf[FN];
@@ -586,7 +597,72 @@ __device__ __forceinline__ void i8_tile(const int8_t* __restrict__ A, const int8
             }
             __builtin_amdgcn_sched_group_barrier(0x008, NMF - Q * (NR + E), 0);
             __builtin_amdgcn_sched_group_barrier(0x008, NMF, 0);
+            __builtin_amdgcn_sched_barrier(0);
         }
+    };
+
+    // residues of the accumulators mod modm into Rm (added to those there when accum)
+    auto epilogue = [&](uint8_t* __restrict__ Rm, int modm, uint32_t magm) {
+        if constexpr (T) {
+            static_assert(ROW4G, "transposed output assumes rows 4 (lane / 16) + r of the 16 x 16 MFMA result");
+            // block (i, j), register quad q: lane holds Rm[row lane % FR][4 consecutive columns from cq]
+            const bool vec = !accum && (ldr % 4) == 0;
+    #pragma unroll
+            for (int i = 0; i < FM; ++i)
+    #pragma unroll
+                for (int j = 0; j < FN; ++j)
+    #pragma unroll
+                    for (int q = 0; q < NREG / 4; ++q) {
+                        const int cq = FR == 16 ? 4 * (lane >> 4) : 8 * q + 4 * (lane >> 5);
+                        const int64_t gm = (int64_t)tm * BM + wm * WTM + i * FR + (lane % FR);
+                        const int64_t gn = (int64_t)tn * BN + wn * WTN + j * FR + cq;
+                        if (gm >= M) continue;
+                        uint32_t w = 0;
+    #pragma unroll
+                        for (int r = 0; r < 4; ++r) w |= mod_u32(acc[i][j][4 * q + r], (uint32_t)modm, magm) << (8 * r);
+                        uint8_t* rp = Rm + gm * ldr + gn;
+                        if (vec && gn + 4 <= N) {
+                            *(uint32_t*)rp = w;
+                        } else {
+                            for (int r = 0; r < 4 && gn + r < N; ++r) {
+                                uint32_t v = (w >> (8 * r)) & 255u;
+                                if (accum) {
+                                    v += rp[r];
+                                    if (v >= (uint32_t)modm) v -= modm;
+                                }
+                                rp[r] = (uint8_t)v;
+                            }
+                        }
+                    }
+        } else {
+            const double inv = 1.0 / (double)modm;
+    #pragma unroll
+            for (int i = 0; i < FM; ++i)
+    #pragma unroll
+                for (int j = 0; j < FN; ++j)
+    #pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const int lrow = ROW4G ? (4 * (lane >> 4) + r) : (4 * r + (lane >> 4));
+                        int64_t gm = (int64_t)tm * BM + wm * WTM + i * 16 + lrow;
+                        int64_t gn = (int64_t)tn * BN + wn * WTN + j * 16 + (lane & 15);
+                        if (gm < M && gn < N) {
+                            int v = mod_small(acc[i][j][r], modm, inv);
+                            if (accum) {
+                                v += Rm[gm * ldr + gn];
+                                if (v >= modm) v -= modm;
+                            }
+                            Rm[gm * ldr + gn] = (uint8_t)v;
+                        }
+                    }
+        }
+    };
+    auto zero_acc = [&]() {
+#pragma unroll
+        for (int i = 0; i < FM; ++i)
+#pragma unroll
+            for (int j = 0; j < FN; ++j)
+#pragma unroll
+                for (int r = 0; r < NREG; ++r) acc[i][j][r] = 0;
     };
 
     if constexpr (CF::ST > 0) {
@@ -600,35 +676,76 @@ __device__ __forceinline__ void i8_tile(const int8_t* __restrict__ A, const int8
                       "direct-load tile shape");
         const int wv = __builtin_amdgcn_readfirstlane(wave);
         const int dr = lane / CPR, dch = ((lane % CPR) ^ swz(lane / CPR)) * 16;
-        auto dma = [&](int st, int64_t k0) {
+        // The block's nmod moduli (planes pA, pB, pR apart) run as one sequence of nmod * ns
+        // slices: the next modulus's first slices load during the last products and the
+        // residue epilogue of the one before.
+        int64_t da = 0, db = 0, dk = 0;  // the next slice to load: plane offsets, K offset
+        auto dma_next = [&](int st) {
 #pragma unroll
             for (int i = 0; i < BM / RP; ++i) {
                 const int r0 = i * RP + wv * RW;
-                __builtin_amdgcn_global_load_lds(Ab + (size_t)(r0 + dr) * ldk + k0 + dch, &As[st][r0][0], 16, 0, 0);
+                __builtin_amdgcn_global_load_lds(Ab + da + (size_t)(r0 + dr) * ldk + dk + dch, &As[st][r0][0], 16, 0,
+                                                 0);
             }
 #pragma unroll
             for (int i = 0; i < BN / RP; ++i) {
                 const int r0 = i * RP + wv * RW;
-                __builtin_amdgcn_global_load_lds(Bb + (size_t)(r0 + dr) * ldk + k0 + dch, &Bs[st][r0][0], 16, 0, 0);
+                __builtin_amdgcn_global_load_lds(Bb + db + (size_t)(r0 + dr) * ldk + dk + dch, &Bs[st][r0][0], 16, 0,
+                                                 0);
+            }
+            dk += BK;
+            if (dk == Kc) {
+                dk = 0;
+                da += pA;
+                db += pB;
             }
         };
-        const int64_t ns = Kc / BK;
+        const int64_t nv = Kc / BK * nmod;
 #pragma unroll
         for (int s = 0; s < S - 1; ++s)
-            if (s < ns) dma(s, (int64_t)s * BK);
+            if (s < nv) dma_next(s);
         // slice 0 landed (the slices after it may still be in flight)
-        if (ns >= S - 1) vm_wait_barrier<(S - 2) * NI>();
+        if (nv >= S - 1) vm_wait_barrier<(S - 2) * NI>();
         else vm_wait_barrier<0>();
         int st = 0, sn = S - 1;
-        for (int64_t t = 0; t < ns; ++t) {
-            const bool more = t + S - 1 < ns;
-            if (more) dma(sn, (t + S - 1) * BK);
-            compute(st);
-            // slice t + 1 landed, and no wave reads buffer st any more
-            if (more) vm_wait_barrier<(S - 2) * NI>();
-            else vm_wait_barrier<0>();
-            st = st + 1 == S ? 0 : st + 1;
-            sn = sn + 1 == S ? 0 : sn + 1;
+        int64_t v = 0;
+        auto slices = [&](int64_t cnt) {
+            for (int64_t t = 0; t < cnt; ++t, ++v) {
+                const bool more = v + S - 1 < nv;
+                if (more) dma_next(sn);
+                compute(st);
+                // slice v + 1 landed, and no wave reads buffer st any more
+                if (more) vm_wait_barrier<(S - 2) * NI>();
+                else vm_wait_barrier<0>();
+                st = st + 1 == S ? 0 : st + 1;
+                sn = sn + 1 == S ? 0 : sn + 1;
+            }
+        };
+        if constexpr (MM) {
+            // rows of R at least tiles x BM, ldr at least tiles x BN (a multiple of 4), nothing to
+            // add to: unguarded word stores (guards or a byte-wise path inside this loop would
+            // cost the slices' schedule registers)
+            static_assert(T && FR == 16 && CF::ST > 0, "word stores from the transposed 16 x 16 blocks");
+            for (int g = 0; g < nmod; ++g) {
+                slices(Kc / BK);
+                const int l = l0 + g;
+                const uint32_t modm = (uint32_t)ml->m[l], magm = ml->magic[l];
+                uint8_t* Rm = R + (int64_t)g * pR;
+#pragma unroll
+                for (int i = 0; i < FM; ++i)
+#pragma unroll
+                    for (int j = 0; j < FN; ++j) {
+                        const int64_t gm = (int64_t)tm * BM + wm * WTM + i * FR + (lane % FR);
+                        const int64_t gn = (int64_t)tn * BN + wn * WTN + j * FR + 4 * (lane >> 4);
+                        uint32_t w = 0;
+#pragma unroll
+                        for (int r = 0; r < 4; ++r) w |= mod_u32(acc[i][j][r], modm, magm) << (8 * r);
+                        *(uint32_t*)(Rm + gm * ldr + gn) = w;
+                    }
+                zero_acc();
+            }
+        } else {
+            slices(nv);
         }
     } else {
         // slices 0 .. D-1 in flight, slice 0 in LDS; slice t uses register set t % D and LDS buffer t % 2
@@ -652,66 +769,10 @@ __device__ __forceinline__ void i8_tile(const int8_t* __restrict__ A, const int8
             }
         }
     }
-    if constexpr (T) {
-        static_assert(ROW4G, "transposed output assumes rows 4 (lane / 16) + r of the 16 x 16 MFMA result");
-        // block (i, j), register quad q: lane holds R[row lane % FR][4 consecutive columns from cq]
-        const bool vec = !accum && (ldr % 4) == 0;
-#pragma unroll
-        for (int i = 0; i < FM; ++i)
-#pragma unroll
-            for (int j = 0; j < FN; ++j)
-#pragma unroll
-                for (int q = 0; q < NREG / 4; ++q) {
-                    const int cq = FR == 16 ? 4 * (lane >> 4) : 8 * q + 4 * (lane >> 5);
-                    const int64_t gm = (int64_t)tm * BM + wm * WTM + i * FR + (lane % FR);
-                    const int64_t gn = (int64_t)tn * BN + wn * WTN + j * FR + cq;
-                    if (gm >= M) continue;
-                    uint32_t w = 0;
-#pragma unroll
-                    for (int r = 0; r < 4; ++r) w |= mod_u32(acc[i][j][4 * q + r], (uint32_t)mod, magic) << (8 * r);
-                    uint8_t* rp = R + gm * ldr + gn;
-                    if (vec && gn + 4 <= N) {
-                        *(uint32_t*)rp = w;
-                    } else {
-                        for (int r = 0; r < 4 && gn + r < N; ++r) {
-                            uint32_t v = (w >> (8 * r)) & 255u;
-                            if (accum) {
-                                v += rp[r];
-                                if (v >= (uint32_t)mod) v -= mod;
-                            }
-                            rp[r] = (uint8_t)v;
-                        }
-                    }
-                }
-    } else {
-        const double inv = 1.0 / (double)mod;
-#pragma unroll
-        for (int i = 0; i < FM; ++i)
-#pragma unroll
-            for (int j = 0; j < FN; ++j)
-#pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    const int lrow = ROW4G ? (4 * (lane >> 4) + r) : (4 * r + (lane >> 4));
-                    int64_t gm = (int64_t)tm * BM + wm * WTM + i * 16 + lrow;
-                    int64_t gn = (int64_t)tn * BN + wn * WTN + j * 16 + (lane & 15);
-                    if (gm < M && gn < N) {
-                        int v = mod_small(acc[i][j][r], mod, inv);
-                        if (accum) {
-                            v += R[gm * ldr + gn];
-                            if (v >= mod) v -= mod;
-                        }
-                        R[gm * ldr + gn] = (uint8_t)v;
-                    }
-                }
-    }
+    if constexpr (!MM) epilogue(R, mod, magic);
 }
 
 // Batches on blockIdx.y: plane offsets sA / sB (0 for a broadcast operand) and sR.
-// Moduli of one launch (blockIdx.z selects one)
-struct ModList {
-    int m[OZ_MAXMOD];
-    uint32_t magic[OZ_MAXMOD];  // ceil(2^32 / m)
-};
 
 // Tiles touching the lower triangle of a square product with BN = R BM: tile row tm
 // holds floor(tm / R) + 1 of them. Index t (row-major over those) -> (tm, tn); the
@@ -751,30 +812,32 @@ __device__ __forceinline__ void lower_tile(uint32_t t, uint32_t R, uint32_t tile
     tn = (uint32_t)(t - lower_tile_prefix(r, R));
 }
 
-template <class CF, bool ROW4G, bool TOUT>
+template <class CF, bool ROW4G, bool TOUT, bool MM>
 __global__ void __launch_bounds__(CF::NT, CF::OCC) k_i8gemm_mod(const int8_t* __restrict__ A,
                                                                 const int8_t* __restrict__ B, uint8_t* __restrict__ R,
                                                                 int64_t M, int64_t N, int64_t ldk, int64_t Kc,
                                                                 int64_t ldr, ModList mods, int accum, uint32_t tiles_m,
                                                                 uint32_t tiles_n, int64_t sA, int64_t sB, int64_t sR,
                                                                 int64_t pA, int64_t pB, int64_t pR,
-                                                                const int* __restrict__ skip, int lower) {
+                                                                const int* __restrict__ skip, int lower, int gmod,
+                                                                int nmods) {
     if (*skip) return;
-    // modulus l = blockIdx.z: every modulus's product in one launch (no tail between them)
-    const int l = (int)blockIdx.z;
-    A += l * pA;
-    B += l * pB;
-    R += l * pR;
-    const int mod = mods.m[l];
+    // moduli l0 .. l0 + gmod - 1 (l0 = blockIdx.z gmod, at most nmods in all): every modulus's
+    // product in one launch (no tail between them); gmod > 1 only with MM
+    const int l0 = (int)blockIdx.z * gmod, g = min(gmod, nmods - l0);
+    A += l0 * pA;
+    B += l0 * pB;
+    R += l0 * pR;
+    const int mod = mods.m[l0];
     __shared__ __attribute__((aligned(16))) I8Smem<CF> sm;
     uint32_t tm, tn;
     // symmetric product: the grid enumerates only the tiles touching the lower triangle,
     // each XCD one contiguous range of them (an equal share of the work)
     if (lower) lower_tile(xcd_chunk(blockIdx.x, gridDim.x), CF::BN / CF::BM, tiles_m, tm, tn);
     else tile_coords<4>(blockIdx.x, tiles_m, tiles_n, tm, tn);
-    i8_tile<CF, ROW4G, TOUT>(A + (int64_t)blockIdx.y * sA, B + (int64_t)blockIdx.y * sB,
+    i8_tile<CF, ROW4G, TOUT, MM>(A + (int64_t)blockIdx.y * sA, B + (int64_t)blockIdx.y * sB,
                              R + (int64_t)blockIdx.y * sR, M, N, ldk, Kc, ldr, mod, accum != 0, tm, tn, sm,
-                             mods.magic[l]);
+                             mods.magic[l0], g, pA, pB, pR, &mods, l0);
 }
 
 // zero n ints (a kernel rather than a memset node, so the sequence can be graph-captured)
@@ -851,7 +914,7 @@ __device__ __forceinline__ double crt_value(const uint8_t* __restrict__ p, int64
 // per thread, read as one V-byte word per plane), rows along y, batches along z.
 template <int V, bool K14>  // columns per thread: 1, 4 (one 4-byte word per plane) or 16 (one 16-byte word)
 __global__ void __launch_bounds__(256) k_crt(const uint8_t* __restrict__ R, int64_t M, int64_t N, int64_t Mp,
-                                             int64_t Np, int64_t pl, int a_batched, int b_batched, BatchDesc bd,
+                                             int64_t Np, int64_t ldr, int64_t pl, int a_batched, int b_batched, BatchDesc bd,
                                              CrtConst cc, const int* __restrict__ eA, const int* __restrict__ fB,
                                              double* C, int64_t rsc, int64_t csc, double alpha, double beta,
                                              int beta_zero, const int* __restrict__ skip, int lower, int tpr) {
@@ -867,7 +930,7 @@ __global__ void __launch_bounds__(256) k_crt(const uint8_t* __restrict__ R, int6
     const int* eAb = eA + (a_batched ? (int64_t)b * Mp : 0);
     const int* fBb = fB + (b_batched ? (int64_t)b * Np : 0);
     for (int64_t i = (int64_t)blockIdx.y * rpb + threadIdx.x / tpr; i < M; i += (int64_t)gridDim.y * rpb) {
-        const uint8_t* rp = R + ((int64_t)b * Mp + i) * N + j0;
+        const uint8_t* rp = R + ((int64_t)b * Mp + i) * ldr + j0;
         double* cp = C + oc + i * rsc + j0 * csc;
         const int ei = eAb[i];
         if (lower && j0 > i) continue;  // symmetric product: the mirror fills j > i
@@ -1263,7 +1326,8 @@ void ozaki_gemm_f64(rstsr_hip_ctx* ctx, const GemmProblem& P, double alpha, doub
     const size_t n_zero = off / sizeof(int);
     off = (off + 255) & ~(size_t)255;
     const size_t o_part = take(std::max(part_bytes(qa), part_bytes(qb)));
-    const size_t planeA = (size_t)nbA * Mp * Kp, planeB = (size_t)nbB * Np * Kp, planeR = (size_t)nb * Mp * N;
+    // residue rows padded to Np bytes: every tile's 4-byte residue words land inside their row
+    const size_t planeA = (size_t)nbA * Mp * Kp, planeB = (size_t)nbB * Np * Kp, planeR = (size_t)nb * Mp * Np;
     const size_t o_A8 = take((size_t)n * planeA);
     const size_t o_B8 = sym ? o_A8 : take((size_t)n * planeB);
     const size_t o_R = take((size_t)n * planeR);
@@ -1332,12 +1396,26 @@ void ozaki_gemm_f64(rstsr_hip_ctx* ctx, const GemmProblem& P, double alpha, doub
             using CF = decltype(cfg);
             // the 256 x 256 tile stores its residues four columns at a time
             constexpr bool TOUT = CF::BM == 256;
-            hipLaunchKernelGGL((k_i8gemm_mod<CF, I8_ROW4G, TOUT>), dim3(ntiles, (uint32_t)nb, (uint32_t)n),
-                               dim3(CF::NT), 0,
-                               ctx->stream, A8 + k0, B8 + k0, R, Mp, N, Kp, Kc, N, mods, k0 > 0 ? 1 : 0,
+            // direct-load tiles over one K chunk run gmod moduli per block (a divisor of the
+            // modulus count): as many as keep a block within 96 K slices and the launch at four
+            // blocks per CU or more. Measured: the 65536 x 1024 x 1024 batch (8 slices per
+            // modulus) 1.28 ms at 7 or 14 moduli per block against 1.38-1.44 ms at one; 8192^3
+            // (64 slices) the same at one or two.
+            int gmod = 1;
+            const bool mm = CF::ST > 0 && TOUT && Kp <= I8_KCHUNK;
+            if (mm) {
+                const uint64_t blocks = (uint64_t)ntiles * nb;
+                const int64_t ns = Kp / CF::BK;
+                for (int g = 1; g <= n; ++g)
+                    if (n % g == 0 && g * ns <= 96 && blocks * (uint64_t)(n / g) >= 1024) gmod = g;
+            }
+            auto kern = mm ? k_i8gemm_mod<CF, I8_ROW4G, TOUT, CF::ST != 0 && TOUT>
+                           : k_i8gemm_mod<CF, I8_ROW4G, TOUT, false>;
+            hipLaunchKernelGGL(kern, dim3(ntiles, (uint32_t)nb, (uint32_t)((n + gmod - 1) / gmod)), dim3(CF::NT), 0,
+                               ctx->stream, A8 + k0, B8 + k0, R, Mp, N, Kp, Kc, Np, mods, k0 > 0 ? 1 : 0,
                                (uint32_t)(Mp / bm), tiles_n, a_b ? Mp * Kp : (int64_t)0,
-                               b_b ? Np * Kp : (int64_t)0, Mp * N, (int64_t)planeA, (int64_t)planeB,
-                               (int64_t)planeR, flag, sym ? 1 : 0);
+                               b_b ? Np * Kp : (int64_t)0, Mp * Np, (int64_t)planeA, (int64_t)planeB,
+                               (int64_t)planeR, flag, sym ? 1 : 0, gmod, n);
         };
         with_i8_tile(tile, go);
         check_launch("ozaki int8 gemm");
@@ -1354,13 +1432,12 @@ void ozaki_gemm_f64(rstsr_hip_ctx* ctx, const GemmProblem& P, double alpha, doub
         const int64_t gy = std::min<int64_t>(std::max<int64_t>(1, (M + 4 * rpb - 1) / (4 * rpb)), 65535);
         auto kern = n == OZ_MAXMOD ? k_crt<V, true> : k_crt<V, false>;
         hipLaunchKernelGGL(kern, dim3(ceil_div_u32(ncg, tpr), (uint32_t)gy, (uint32_t)nb), dim3(256), 0,
-                           ctx->stream, R, M, N, Mp, Np, (int64_t)planeR, a_b ? 1 : 0, b_b ? 1 : 0, P.bd, cc, eA, fB,
+                           ctx->stream, R, M, N, Mp, Np, Np, (int64_t)planeR, a_b ? 1 : 0, b_b ? 1 : 0, P.bd, cc, eA, fB,
                            (double*)P.c, P.rsc, P.csc, alpha, beta, beta_zero ? 1 : 0, flag, sym ? 1 : 0, tpr);
     };
-    // R rows are N bytes long: 16-byte words need N % 16 == 0 (planes start 256-byte aligned)
-    // (16 columns per thread measured slower: each lane's 16 doubles land 128 bytes apart)
-    if (N % 4 == 0) crt(std::integral_constant<int, 4>{});
-    else crt(std::integral_constant<int, 1>{});
+    // R rows are Np bytes long (a multiple of 128): four columns per thread, one 4-byte word per
+    // plane (16 columns per thread measured slower: each lane's 16 doubles land 128 bytes apart)
+    crt(std::integral_constant<int, 4>{});
     check_launch("ozaki crt");
     if (sym) mirror_lower_f64(ctx, P);
 }

@@ -205,6 +205,58 @@ def test_ozaki_batched_and_zero_rows(dev, oracle):
     assert np.all(got[1, 5, :] == 0.0) and np.all(got[:, :, 7] == 0.0)
 
 
+@pytest.fixture
+def int8_tile():
+    """Sets RSTSR_HIP_OZAKI_TILE (read at every call) and restores it afterwards."""
+    old = os.environ.get("RSTSR_HIP_OZAKI_TILE")
+
+    def set_tile(t):
+        if t is None:
+            os.environ.pop("RSTSR_HIP_OZAKI_TILE", None)
+        else:
+            os.environ["RSTSR_HIP_OZAKI_TILE"] = t
+
+    yield set_tile
+    set_tile(old)
+
+
+def test_ozaki_big_tile_moduli_per_block(dev, oracle, int8_tile):
+    """The 256 x 256 direct-load tile runs several moduli per block when the launch stays at four
+    blocks per CU (here [32, 1024, 512] % [512, 1024]: 512 tiles, 7 moduli per block): the same
+    bits as the 128 x 128 tile (one modulus per block, register-staged loads; residue sums are
+    exact), and the Dot2 oracle on sampled rows."""
+    rng = np.random.default_rng(21)
+    A = wide_exponent(rng, (32, 1024, 512))
+    B = wide_exponent(rng, (512, 1000))  # ragged: the residue rows are padded to 1024 bytes
+    ta, tb = rt.asarray(A, device=dev), rt.asarray(B, device=dev)
+    int8_tile("big")
+    big = rt.matmul(ta, tb, mode="ozaki").to_numpy()
+    int8_tile("square")
+    sq = rt.matmul(ta, tb, mode="ozaki").to_numpy()
+    assert np.array_equal(big.view(np.uint64), sq.view(np.uint64))
+    for b in (0, 13, 31):
+        ref = oracle.gemm_dot2(A[b, ::97], B)
+        assert np.max(np.abs(big[b, ::97] - ref)) <= 1e-14 * np.max(np.abs(ref))
+
+
+def test_ozaki_big_tile_long_k(dev, oracle, int8_tile):
+    """K beyond one INT8 launch's 65536 (the second launch adds its residues to the first's) on
+    the 256 x 256 tile: the same bits as the 128 x 128 tile, and normwise against Dot2 within
+    1e-14 sqrt(K / 4096) = 4e-14 (the split's per-element truncation, g = 54 bits relative to
+    each row's and column's largest entry, summed over 16 times the products of K = 4096)."""
+    rng = np.random.default_rng(22)
+    A = rng.standard_normal((260, 65700))
+    B = rng.standard_normal((65700, 260))
+    ta, tb = rt.asarray(A, device=dev), rt.asarray(B, device=dev)
+    int8_tile("big")
+    got = rt.matmul(ta, tb, mode="ozaki").to_numpy()
+    int8_tile("square")
+    sq = rt.matmul(ta, tb, mode="ozaki").to_numpy()
+    assert np.array_equal(got.view(np.uint64), sq.view(np.uint64))
+    ref = oracle.gemm_dot2(A, B)
+    assert np.max(np.abs(got - ref)) <= 4e-14 * np.max(np.abs(ref))
+
+
 def test_ozaki_nonfinite_falls_back(dev):
     A = np.ones((20, 30))
     A[3, 4] = np.inf
