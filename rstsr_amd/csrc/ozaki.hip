@@ -263,18 +263,20 @@ __device__ __forceinline__ Digits to_digits(double x) {
     return {(uint32_t)l, (uint32_t)((int)h + (1 << 24))};
 }
 
-// Residue byte of x modulo m: t in [0, m) with t = x + 128 (mod m), so that t - 128 in
+// Residue byte of x modulo m (res_reduce of the sum s that pack_residues forms from the digits
+// of u = x + 2^56): t in [0, m) with t = x + 128 (mod m), so that t - 128 in
 // [-128, m - 128) is an int8 representative of x mod m, stored as the byte t ^ 0x80.
 // s = sum_k b_k (256^k mod m) + m - (2^56 mod m) + 128 = u - 2^56 + 128 = x + 128 (mod m), with
 // s < 8 * 255^2 + 384 < 2^19 (two byte dot products, the offset as their accumulator). The
 // quotient from 24-bit products: s * ceil(2^20 / m) < 2^32, and its floor / 2^20 exceeds
 // floor(s / m) by at most one (the error s (ceil(2^20/m) m - 2^20) / (m 2^20) < s / 2^20 < 1/2),
-// so t = s - q m lies in [-m, m) and one correction ends it.
-__device__ __forceinline__ uint32_t res_byte(const Digits& d, const ResConst& c) {
-    const uint32_t s = __builtin_amdgcn_udot4(d.hi, c.chi, __builtin_amdgcn_udot4(d.lo, c.clo, c.off, false), false);
-    const uint32_t q = __umul24(s, c.mg20) >> 20;
-    const int t = (int)s - (int)__umul24(q, c.m);
-    return (uint32_t)(t < 0 ? t + (int)c.m : t);
+// so t = s - q m lies in [-m, m) and one correction ends it: min(t, t + m) as unsigned words
+// (a negative t wraps above t + m). q < 2^12 is the bit field [20, 32) of the product, and
+// t = s + q (-m) one 24-bit multiply-add.
+__device__ __forceinline__ uint32_t res_reduce(uint32_t s, const ResConst& c) {
+    const int q = (int)__builtin_amdgcn_ubfe(__umul24(s, c.mg20), 20, 12);
+    const int t = __mul24(q, -(int)c.m) + (int)s;
+    return min((uint32_t)t, (uint32_t)(t + (int)c.m));
 }
 
 template <bool K14>
@@ -283,10 +285,17 @@ __device__ __forceinline__ ResConst res_const(const ModSet& ms, int l) {
     else return ResConst{ms.clo[l], ms.chi[l], ms.off[l], ms.mg20[l], ms.mi[l]};
 }
 
+// four residue bytes in one word, each stage over the four before the next (independent
+// dot products back to back instead of dependent ones, which wait on each other)
 __device__ __forceinline__ uint32_t pack_residues(const Digits* d, const ResConst& c) {
+    uint32_t s[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) s[i] = __builtin_amdgcn_udot4(d[i].lo, c.clo, c.off, false);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) s[i] = __builtin_amdgcn_udot4(d[i].hi, c.chi, s[i], false);
     uint32_t w = 0;
 #pragma unroll
-    for (int i = 0; i < 4; ++i) w |= res_byte(d[i], c) << (8 * i);
+    for (int i = 0; i < 4; ++i) w |= res_reduce(s[i], c) << (8 * i);
     return w ^ 0x80808080u;
 }
 
@@ -305,53 +314,97 @@ __device__ __forceinline__ void for_moduli(const ModSet& ms, F&& f) {
 // t < len; planes P[l][v][t] for v < nvp, t < lenp (zero padded; lenp a
 // multiple of 64). The 64 x 64 tile is staged through LDS (`tile`) so that
 // reads run along the source's unit-stride axis and each thread writes 16
-// consecutive plane bytes per modulus.
-template <bool K14>
-__device__ __forceinline__ void split_tile(const double* __restrict__ s, int64_t nv, int64_t len, int64_t sv,
-                                           int64_t st, const int* __restrict__ ev, int64_t nvp, int64_t lenp,
-                                           int8_t* __restrict__ planes, const ModSet& ms, bool read_along_t,
-                                           int64_t plane, int64_t v0, int64_t t0, double (&tile)[64][65]) {
+// consecutive plane bytes per modulus. split_load fetches a thread's 16 source
+// elements (zero outside the source), split_store scales them into the tile and
+// writes the residues; split_tile is the two in turn.
+__device__ __forceinline__ void split_load(const double* __restrict__ s, int64_t nv, int64_t len, int64_t sv,
+                                           int64_t st, bool read_along_t, int64_t v0, int64_t t0, double (&x)[16]) {
     const int tid = threadIdx.x;
 #pragma unroll
     for (int k = 0; k < 16; ++k) {
         const int e = k * 256 + tid, fast = e & 63, slow = e >> 6;
         const int tt = read_along_t ? fast : slow, vv = read_along_t ? slow : fast;
         const int64_t gv = v0 + vv, gt = t0 + tt;
-        double x = 0.0;
-        if (gv < nv && gt < len) x = rint(ldexp(s[gv * sv + gt * st], ev[gv]));
-        tile[vv][tt] = x;
+        x[k] = gv < nv && gt < len ? s[gv * sv + gt * st] : 0.0;
+    }
+}
+
+template <bool K14>
+__device__ __forceinline__ void split_store(const double (&x)[16], int64_t nv, const int* __restrict__ ev, int64_t nvp,
+                                            int64_t lenp, int8_t* __restrict__ planes, const ModSet& ms,
+                                            bool read_along_t, int64_t plane, int64_t v0, int64_t t0,
+                                            double (&tile)[64][65]) {
+    const int tid = threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        const int e = k * 256 + tid, fast = e & 63, slow = e >> 6;
+        const int tt = read_along_t ? fast : slow, vv = read_along_t ? slow : fast;
+        const int64_t gv = v0 + vv;
+        tile[vv][tt] = gv < nv ? rint(ldexp(x[k], ev[gv])) : 0.0;
     }
     __syncthreads();
     const int vv = tid >> 2, tq = (tid & 3) * 16;
     const int64_t gv = v0 + vv;
-    if (gv >= nvp) return;
-    Digits dg[16];
+    if (gv < nvp) {
+        Digits dg[16];
 #pragma unroll
-    for (int i = 0; i < 16; ++i) dg[i] = to_digits(tile[vv][tq + i]);
-    int8_t* dst = planes + gv * lenp + t0 + tq;
-    for_moduli<K14>(ms, [&](int l) {
-        const ResConst c = res_const<K14>(ms, l);
-        const uint4 w{pack_residues(dg, c), pack_residues(dg + 4, c), pack_residues(dg + 8, c),
-                      pack_residues(dg + 12, c)};
-        *(uint4*)(dst + l * plane) = w;
-    });
+        for (int i = 0; i < 16; ++i) dg[i] = to_digits(tile[vv][tq + i]);
+        int8_t* dst = planes + gv * lenp + t0 + tq;
+        for_moduli<K14>(ms, [&](int l) {
+            const ResConst c = res_const<K14>(ms, l);
+            const uint4 w{pack_residues(dg, c), pack_residues(dg + 4, c), pack_residues(dg + 8, c),
+                          pack_residues(dg + 12, c)};
+            *(uint4*)(dst + l * plane) = w;
+        });
+    }
+}
+
+template <bool K14>
+__device__ __forceinline__ void split_tile(const double* __restrict__ s, int64_t nv, int64_t len, int64_t sv,
+                                           int64_t st, const int* __restrict__ ev, int64_t nvp, int64_t lenp,
+                                           int8_t* __restrict__ planes, const ModSet& ms, bool read_along_t,
+                                           int64_t plane, int64_t v0, int64_t t0, double (&tile)[64][65]) {
+    double x[16];
+    split_load(s, nv, len, sv, st, read_along_t, v0, t0, x);
+    split_store<K14>(x, nv, ev, nvp, lenp, planes, ms, read_along_t, plane, v0, t0, tile);
 }
 
 // Batch b = blockIdx.z of one operand (`which`: 0 = A, 1 = B offsets of bd):
-// exponents ev + b*nvp, planes + b*nvp*lenp.
+// exponents ev + b*nvp, planes + b*nvp*lenp. A block splits tpb tiles along t,
+// the next tile's source loads in flight while it computes the residues of the
+// one before: t tile group blockIdx.x and vector tile blockIdx.y, or the other
+// way round when `swap`.
 template <bool K14>
 __global__ void __launch_bounds__(256) k_split(const double* __restrict__ s, int64_t nv, int64_t len, int64_t sv,
                                                int64_t st, BatchDesc bd, int which, const int* __restrict__ ev,
                                                int64_t nvp, int64_t lenp, int8_t* __restrict__ planes, ModSet ms,
-                                               int read_along_t, const int* __restrict__ skip, int64_t plane) {
+                                               int read_along_t, const int* __restrict__ skip, int64_t plane,
+                                               int tpb, int swap) {
     if (*skip) return;  // non-finite input: the plain kernel computes this product
     __shared__ double tile[64][65];
     const uint32_t b = blockIdx.z;
     int64_t oa, ob, oc;
     batch_offsets(bd, b, oa, ob, oc);
-    split_tile<K14>(s + (which ? ob : oa), nv, len, sv, st, ev + (int64_t)b * nvp, nvp, lenp,
-               planes + (int64_t)b * nvp * lenp, ms, read_along_t != 0, plane, (int64_t)blockIdx.y * 64,
-               (int64_t)blockIdx.x * 64, tile);
+    const double* sb = s + (which ? ob : oa);
+    const int* evb = ev + (int64_t)b * nvp;
+    int8_t* pb = planes + (int64_t)b * nvp * lenp;
+    const bool along = read_along_t != 0;
+    const uint32_t bx = swap ? blockIdx.y : blockIdx.x, by = swap ? blockIdx.x : blockIdx.y;
+    const int64_t v0 = (int64_t)by * 64, ta = (int64_t)bx * tpb * 64;
+    const int64_t te = min(lenp, ta + (int64_t)tpb * 64);
+    double x[16];
+    split_load(sb, nv, len, sv, st, along, v0, ta, x);
+    for (int64_t t0 = ta; t0 < te; t0 += 64) {
+        double xn[16];
+        const bool more = t0 + 64 < te;
+        if (more) split_load(sb, nv, len, sv, st, along, v0, t0 + 64, xn);
+        split_store<K14>(x, nv, evb, nvp, lenp, pb, ms, along, plane, v0, t0, tile);
+        __syncthreads();  // the tile is rewritten for the next t0
+        if (more) {
+#pragma unroll
+            for (int k = 0; k < 16; ++k) x[k] = xn[k];
+        }
+    }
 }
 
 // Norm and split fused, for vectors whose elements are contiguous (st = 1): one wave per
@@ -582,20 +635,20 @@ __device__ __forceinline__ void i8_tile(const int8_t* __restrict__ A, const int8
         }
         if constexpr (CF::SG) {
             // instruction order for the scheduler: the first K step's fragment reads, then each
-            // later step's reads spread over the MFMAs of the step before it (two MFMAs per read),
+            // later step's reads spread over the MFMAs of the step before it (Q MFMAs per read),
             // so fragments arrive while the matrix cores run
-            // SG 1: two MFMAs per read; SG 2: one; SG 3: two, and the first step starts its MFMAs
-            // after the reads of a0 and the b fragments, the other a reads between them
-            constexpr int NR = FM + FN, NMF = FM * FN, Q = CF::SG == 2 ? 1 : 2;
-            constexpr int E = CF::SG == 3 ? FM - 1 : 0;  // first-step reads moved between its MFMAs
-            static_assert(Q * (NR + E) <= NMF && BK == 2 * FK, "two K steps; reads fit between the MFMAs");
-            __builtin_amdgcn_sched_group_barrier(0x100, NR - E, 0);
+            constexpr int NR = FM + FN, NMF = FM * FN, NS = BK / FK, Q = NMF / NR >= 2 ? 2 : 1;
+            static_assert(Q * NR <= NMF && NS >= 2, "reads fit between the MFMAs");
+            __builtin_amdgcn_sched_group_barrier(0x100, NR, 0);
 #pragma unroll
-            for (int r = 0; r < E + NR; ++r) {
-                __builtin_amdgcn_sched_group_barrier(0x008, Q, 0);
-                __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+            for (int s = 0; s + 1 < NS; ++s) {
+#pragma unroll
+                for (int r = 0; r < NR; ++r) {
+                    __builtin_amdgcn_sched_group_barrier(0x008, Q, 0);
+                    __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+                }
+                __builtin_amdgcn_sched_group_barrier(0x008, NMF - Q * NR, 0);
             }
-            __builtin_amdgcn_sched_group_barrier(0x008, NMF - Q * (NR + E), 0);
             __builtin_amdgcn_sched_group_barrier(0x008, NMF, 0);
             __builtin_amdgcn_sched_barrier(0);
         }
@@ -725,7 +778,7 @@ __device__ __forceinline__ void i8_tile(const int8_t* __restrict__ A, const int8
             // rows of R at least tiles x BM, ldr at least tiles x BN (a multiple of 4), nothing to
             // add to: unguarded word stores (guards or a byte-wise path inside this loop would
             // cost the slices' schedule registers)
-            static_assert(T && FR == 16 && CF::ST > 0, "word stores from the transposed 16 x 16 blocks");
+            static_assert(T && CF::ST > 0, "word stores from the transposed blocks");
             for (int g = 0; g < nmod; ++g) {
                 slices(Kc / BK);
                 const int l = l0 + g;
@@ -734,14 +787,17 @@ __device__ __forceinline__ void i8_tile(const int8_t* __restrict__ A, const int8
 #pragma unroll
                 for (int i = 0; i < FM; ++i)
 #pragma unroll
-                    for (int j = 0; j < FN; ++j) {
-                        const int64_t gm = (int64_t)tm * BM + wm * WTM + i * FR + (lane % FR);
-                        const int64_t gn = (int64_t)tn * BN + wn * WTN + j * FR + 4 * (lane >> 4);
-                        uint32_t w = 0;
+                    for (int j = 0; j < FN; ++j)
 #pragma unroll
-                        for (int r = 0; r < 4; ++r) w |= mod_u32(acc[i][j][r], modm, magm) << (8 * r);
-                        *(uint32_t*)(Rm + gm * ldr + gn) = w;
-                    }
+                        for (int q = 0; q < NREG / 4; ++q) {
+                            const int cq = FR == 16 ? 4 * (lane >> 4) : 8 * q + 4 * (lane >> 5);
+                            const int64_t gm = (int64_t)tm * BM + wm * WTM + i * FR + (lane % FR);
+                            const int64_t gn = (int64_t)tn * BN + wn * WTN + j * FR + cq;
+                            uint32_t w = 0;
+#pragma unroll
+                            for (int r = 0; r < 4; ++r) w |= mod_u32(acc[i][j][4 * q + r], modm, magm) << (8 * r);
+                            *(uint32_t*)(Rm + gm * ldr + gn) = w;
+                        }
                 zero_acc();
             }
         } else {
@@ -939,13 +995,14 @@ __global__ void __launch_bounds__(256) k_crt(const uint8_t* __restrict__ R, int6
             *cp = beta_zero ? alpha * v : alpha * v + beta * (*cp);
         } else {
             constexpr int NW = V / 4;  // 4-byte words per plane
+            const int nm = K14 ? OZ_MAXMOD : cc.n;  // a constant for the default moduli: no branches
             uint32_t w[OZ_MAXMOD][NW];
 #pragma unroll
             for (int l = 0; l < OZ_MAXMOD; ++l) {
                 if constexpr (NW == 1) {
-                    w[l][0] = l < cc.n ? *(const uint32_t*)(rp + l * pl) : 0u;
+                    w[l][0] = l < nm ? *(const uint32_t*)(rp + l * pl) : 0u;
                 } else {
-                    const uint4 q = l < cc.n ? *(const uint4*)(rp + l * pl) : uint4{0, 0, 0, 0};
+                    const uint4 q = l < nm ? *(const uint4*)(rp + l * pl) : uint4{0, 0, 0, 0};
                     w[l][0] = q.x;
                     w[l][1] = q.y;
                     w[l][2] = q.z;
@@ -1224,8 +1281,8 @@ static OzSetup ozaki_setup() {
 // tile takes 6.2-6.4 ms (6.8-7.1 with loads staged through registers and the compiler's
 // read order) against 8.0 ms for 128 x 256 from eight 64 x 64 waves (7.8 with direct loads,
 // two blocks per CU); 128 x 256 from four 64 x 128 waves, 256 x 256 from four 128 x 128 waves
-// (which spill), 32 x 32 x 32 MFMAs, global loads two slices ahead and 64-byte slices in
-// three or four LDS buffers were no faster. Products at most 256 long on a side
+// (which spill), 32 x 32 x 32 MFMAs (11 % slower on the direct-load tile), global loads two
+// slices ahead and 64-byte slices in three or four LDS buffers were no faster. Products at most 256 long on a side
 // keep the smaller tiles (less padding); outputs at most 128 wide the 128 x 128 tile.
 // RSTSR_HIP_OZAKI_TILE (square, wide8, big) overrides the choice.
 enum class I8Tile { Sq, Wide8, Big };
@@ -1371,10 +1428,19 @@ void ozaki_gemm_f64(rstsr_hip_ctx* ctx, const GemmProblem& P, double alpha, doub
             return;
         }
         run_norm(q, ev);
+        // tpb tiles along K per block while the grid keeps at least eight blocks per CU
+        const uint64_t tiles = (uint64_t)(Kp / 64) * (uint64_t)(nvp / 64) * q.nbx;
+        // Sources strided along t (B's columns of a row-major B): consecutive blocks take
+        // neighbouring vector tiles, so that the rows they read lie side by side (measured on
+        // 8192^3: 400-425 us against 460-540 with the t tiles consecutive and one per block)
+        int tpb = 1;
+        const int swap = std::llabs(st) == 1 ? 0 : 1;
+        while (tpb < 8 && tiles / (uint64_t)(2 * tpb) >= (uint64_t)std::max(ctx->num_cu, 1) * 8) tpb *= 2;
+        const uint32_t gx = (uint32_t)((Kp / 64 + tpb - 1) / tpb), gy = (uint32_t)(nvp / 64);
         hipLaunchKernelGGL(n == OZ_MAXMOD ? k_split<true> : k_split<false>,
-                           dim3((uint32_t)(Kp / 64), (uint32_t)(nvp / 64), (uint32_t)q.nbx), dim3(256), 0,
-                           ctx->stream, x, nv, K, sv, st, P.bd, which, ev, nvp, Kp, planes, ms,
-                           std::llabs(st) == 1 ? 1 : 0, flag, (int64_t)plane);
+                           dim3(swap ? gy : gx, swap ? gx : gy, (uint32_t)q.nbx),
+                           dim3(256), 0, ctx->stream, x, nv, K, sv, st, P.bd, which, ev, nvp, Kp, planes, ms,
+                           std::llabs(st) == 1 ? 1 : 0, flag, (int64_t)plane, tpb, swap);
         check_launch(what);
     };
     norm_split(qa, (const double*)P.a, M, Mp, P.rsa, P.csa, 0, eA, A8, planeA, "ozaki split A");
