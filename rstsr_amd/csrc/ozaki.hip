@@ -831,13 +831,39 @@ __device__ __forceinline__ void i8_tile(const int8_t* __restrict__ A, const int8
 // Batches on blockIdx.y: plane offsets sA / sB (0 for a broadcast operand) and sR.
 
 // Tiles touching the lower triangle of a square product with BN = R BM: tile row tm
-// holds floor(tm / R) + 1 of them. Index t (row-major over those) -> (tm, tn); the
-// floating root estimate is corrected exactly.
+// holds floor(tm / R) + 1 of them. Index t -> (tm, tn): for R = 1 and 2 in groups of
+// tile rows walked down the columns, otherwise row-major; the floating root estimates
+// are corrected exactly.
 __host__ __device__ inline uint64_t lower_tile_prefix(uint64_t r, uint64_t R) {
     const uint64_t q = r / R, m = r - q * R;
     return R * q * (q + 1) / 2 + m * (q + 1);
 }
 __device__ __forceinline__ void lower_tile(uint32_t t, uint32_t R, uint32_t tiles_m, uint32_t& tm, uint32_t& tn) {
+    if (R == 1) {
+        // groups of four tile rows 4g .. 4g+3 (the last group k <= 4 rows), walked down the
+        // columns so that blocks resident together share A and B panels in L2: columns
+        // 0 .. 4g-1 with all k rows, then the triangle (column 4g + c, rows 4g + c .. 4g + k-1).
+        // Group g starts at 8g^2 + 2g (full groups of 16h + 10 tiles before it).
+        uint64_t g = (uint64_t)((sqrt(4.0 + 32.0 * (double)t) - 2.0) * 0.0625);
+        while (g > 0 && 8 * g * g + 2 * g > t) --g;
+        while (8 * (g + 1) * (g + 1) + 2 * (g + 1) <= t) ++g;
+        uint32_t u = (uint32_t)(t - (8 * g * g + 2 * g));
+        const uint32_t r0 = (uint32_t)(4 * g), k = min(4u, tiles_m - r0);
+        if (u < k * r0) {
+            tn = u / k;
+            tm = r0 + u % k;
+        } else {
+            u -= k * r0;
+            uint32_t c = 0;
+            while (u >= k - c) {
+                u -= k - c;
+                ++c;
+            }
+            tn = r0 + c;
+            tm = r0 + c + u;
+        }
+        return;
+    }
     if (R == 2) {
         // groups of four tile rows 4g .. 4g+3 (counts 2g+1, 2g+1, 2g+2, 2g+2), walked down
         // the columns so that blocks resident together share A and B panels in L2:
