@@ -639,26 +639,6 @@ void plain_dgemm(rstsr_hip_ctx* ctx, const GemmProblem& P, double alpha, double 
     launch_gemm_t<double>(ctx, P, alpha, beta, beta_zero, run_if);
 }
 
-void mirror_lower_f64(rstsr_hip_ctx* ctx, const GemmProblem& P) {
-    GemmArgs<double> g{};
-    g.M = P.M;
-    g.N = P.N;
-    g.c = (double*)P.c;
-    g.rsc = P.rsc;
-    g.csc = P.csc;
-    g.bd = P.bd;
-    const uint64_t t64 = ceil_div_u32(P.M, 64);
-    const uint64_t nm = t64 * (t64 + 1) / 2;
-    if (nm == 0) return;
-    if (nm >= (1ull << 31)) RSTSR_RAISE(UnImplemented, "symmetric product too large");
-    const uint64_t YMAX = 65535;
-    for (uint64_t b0 = 0; b0 < P.nbatch; b0 += YMAX) {
-        g.batch0 = (uint32_t)b0;
-        const uint32_t nb = (uint32_t)std::min<uint64_t>(YMAX, P.nbatch - b0);
-        hipLaunchKernelGGL(k_mirror<double>, dim3((uint32_t)nm, nb), dim3(256), 0, ctx->stream, g);
-        check_launch("gemm mirror");
-    }
-}
 
 /* ---------------- grouped GEMM ---------------- */
 

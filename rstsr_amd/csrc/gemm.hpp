@@ -83,9 +83,6 @@ void ozaki_gemm_f64(rstsr_hip_ctx* ctx, const GemmProblem& P, double alpha, doub
 // the kernel exits unless *run_if != 0.
 void plain_dgemm(rstsr_hip_ctx* ctx, const GemmProblem& P, double alpha, double beta, bool beta_zero,
                  const int* run_if = nullptr);
-// Copy C's strictly lower triangle onto its upper triangle (every batch), for
-// symmetric products computed on and below the diagonal only (gemm.hip).
-void mirror_lower_f64(rstsr_hip_ctx* ctx, const GemmProblem& P);
 // Grouped split GEMM of unbatched f64 problems with M, N, K > 0 (ozaki.hip).
 // Outputs must not overlap.
 void ozaki_grouped_f64(rstsr_hip_ctx* ctx, const std::vector<GemmProblem>& probs, double alpha, double beta,

@@ -1015,7 +1015,7 @@ __global__ void __launch_bounds__(256) k_crt(const uint8_t* __restrict__ R, int6
         const uint8_t* rp = R + ((int64_t)b * Mp + i) * ldr + j0;
         double* cp = C + oc + i * rsc + j0 * csc;
         const int ei = eAb[i];
-        if (lower && j0 > i) continue;  // symmetric product: the mirror fills j > i
+        if (lower && j0 > i) continue;  // lower triangle only: j > i left to the caller
         if constexpr (V == 1) {
             const double v = ldexp(crt_value<K14>(rp, pl, cc), -(ei + fBb[j0]));
             *cp = beta_zero ? alpha * v : alpha * v + beta * (*cp);
@@ -1077,6 +1077,97 @@ __device__ __forceinline__ int find_prob(const uint32_t* __restrict__ pre, int c
         else hi = mid - 1;
     }
     return lo;
+}
+
+// Shape of the CRT tiles: TC columns (64 or 128), TPR = TC / 4 threads per row (four
+// columns each), RPP = 256 / TPR rows per pass, TR = 4 RPP rows (four passes): 64 x 64 or
+// 32 x 128. Host and kernel take the shape from here.
+template <int TC_>
+struct CrtTile {
+    static_assert(TC_ == 64 || TC_ == 128, "tile width");
+    static constexpr int TC = TC_, TPR = TC / 4, RPP = 256 / TPR, TR = 4 * RPP;
+};
+
+// CRT over TR x TC tiles of C (CrtTile): thread t computes rows i0 + t / TPR + RPP k
+// (k < 4) at columns j0 + 4 (t % TPR) .. + 3, from one 4-byte word per plane and row
+// (planes of Mp x ldr bytes, batch b at row b Mp; the grid's tiles lie within Mp x Np).
+// Symmetric product (LOWER): only the tiles touching the lower triangle run (index
+// blockIdx.x over them), and each also writes its transpose onto the mirror tile through
+// LDS (both stores run along C's rows; only elements above the diagonal), so no separate
+// mirror pass reads C back.
+template <bool K14, bool LOWER, int TC>
+__global__ void __launch_bounds__(256) k_crt_tile(const uint8_t* __restrict__ R, int64_t M, int64_t N, int64_t Mp,
+                                                  int64_t Np, int64_t ldr, int64_t pl, int a_batched, int b_batched,
+                                                  BatchDesc bd, CrtConst cc, const int* __restrict__ eA,
+                                                  const int* __restrict__ fB, double* C, int64_t rsc, int64_t csc,
+                                                  double alpha, double beta, int beta_zero, const int* __restrict__ skip,
+                                                  uint32_t tiles_m, uint32_t tiles_n) {
+    constexpr int TPR = CrtTile<TC>::TPR, RPP = CrtTile<TC>::RPP, TR = CrtTile<TC>::TR;
+    if (*skip) return;
+    __shared__ double tile[LOWER ? TR : 1][TC + 1];
+    uint32_t ti, tj;
+    if constexpr (LOWER) lower_tile(blockIdx.x, TC / TR, tiles_m, ti, tj);
+    else {
+        ti = blockIdx.x / tiles_n;
+        tj = blockIdx.x - ti * tiles_n;
+    }
+    const uint32_t b = blockIdx.y;
+    int64_t oa, ob, oc;
+    batch_offsets(bd, b, oa, ob, oc);
+    const int* eAb = eA + (a_batched ? (int64_t)b * Mp : 0);
+    const int* fBb = fB + (b_batched ? (int64_t)b * Np : 0);
+    double* Cb = C + oc;
+    const int tid = threadIdx.x, cg = tid % TPR, rg = tid / TPR;
+    const int64_t i0 = (int64_t)ti * TR, j0 = (int64_t)tj * TC, jc = j0 + 4 * cg;
+    const int nm = K14 ? OZ_MAXMOD : cc.n;
+    uint32_t w[4][OZ_MAXMOD];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int64_t i = i0 + rg + RPP * k;  // i < Mp: rows of R exist up to the padded extent
+        const uint8_t* rp = R + ((int64_t)b * Mp + i) * ldr + jc;
+#pragma unroll
+        for (int l = 0; l < OZ_MAXMOD; ++l) w[k][l] = l < nm ? *(const uint32_t*)(rp + l * pl) : 0u;
+    }
+    int fj[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) fj[c] = fBb[jc + c];  // fB holds Np >= jc + 4 entries
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int64_t i = i0 + rg + RPP * k;
+        const int ei = eAb[i];
+        double v[4];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            uint32_t r[OZ_MAXMOD];
+#pragma unroll
+            for (int l = 0; l < OZ_MAXMOD; ++l) r[l] = (w[k][l] >> (8 * c)) & 255u;
+            v[c] = alpha * ldexp(crt_from<K14>(r, cc), -(ei + fj[c]));
+        }
+        if (i < M) {
+            double* cp = Cb + i * rsc + jc * csc;
+#pragma unroll
+            for (int c = 0; c < 4; ++c)
+                if (jc + c < N && !(LOWER && jc + c > i)) {
+                    double* ck = cp + c * csc;
+                    *ck = beta_zero ? v[c] : v[c] + beta * (*ck);
+                }
+        }
+        if constexpr (LOWER) {
+#pragma unroll
+            for (int c = 0; c < 4; ++c) tile[rg + RPP * k][4 * cg + c] = v[c];
+        }
+    }
+    if constexpr (LOWER) {
+        __syncthreads();
+        // mirror: row j0 + r of C (r < TC) gets columns i0 .. i0 + TR - 1 from the tile's column r
+        constexpr int MPR = 256 / TR;  // mirror rows per pass
+        const int c = tid % TR, r0 = tid / TR;
+#pragma unroll
+        for (int k = 0; k < TC / MPR; ++k) {
+            const int r = r0 + MPR * k;
+            if (j0 + r < N && i0 + c < M && i0 + c > j0 + r) Cb[(j0 + r) * rsc + (i0 + c) * csc] = tile[c][r];
+        }
+    }
 }
 
 // Scale exponents for a group: one block per 64 rows of an A or 64 columns of a
@@ -1348,8 +1439,9 @@ void ozaki_gemm_f64(rstsr_hip_ctx* ctx, const GemmProblem& P, double alpha, doub
     });
     // Symmetric product (the SYRK condition of device_faer/matmul.rs:38-48: beta = 0, A is B's
     // transpose in the same buffer): B's columns are A's rows, so B's scales and residue planes are
-    // A's; only the INT8 tiles touching the lower triangle run, the CRT fills i >= j, and the mirror
-    // copies the rest. Residue sums are exact integers, so the result is the full product bit for bit.
+    // A's; only the INT8 tiles touching the lower triangle run, and the CRT (k_crt_tile) fills i >= j
+    // and mirrors each tile onto j > i. Residue sums are exact integers, so the result is the full
+    // product bit for bit.
     bool sym = beta_zero && P.a == P.b && M == N && P.rsa == P.csb && P.csa == P.rsb;
     for (int d = 0; d < P.bd.nbd && sym; ++d) sym = P.bd.sa[d] == P.bd.sb[d];
     const int64_t Np = (N + bn - 1) / bn * bn;
@@ -1527,11 +1619,28 @@ void ozaki_gemm_f64(rstsr_hip_ctx* ctx, const GemmProblem& P, double alpha, doub
                            ctx->stream, R, M, N, Mp, Np, Np, (int64_t)planeR, a_b ? 1 : 0, b_b ? 1 : 0, P.bd, cc, eA, fB,
                            (double*)P.c, P.rsc, P.csc, alpha, beta, beta_zero ? 1 : 0, flag, sym ? 1 : 0, tpr);
     };
-    // R rows are Np bytes long (a multiple of 128): four columns per thread, one 4-byte word per
-    // plane (16 columns per thread measured slower: each lane's 16 doubles land 128 bytes apart)
-    crt(std::integral_constant<int, 4>{});
-    check_launch("ozaki crt");
-    if (sym) mirror_lower_f64(ctx, P);
+    if (!sym) {
+        // R rows are Np bytes long (a multiple of 128): four columns per thread, one 4-byte word
+        // per plane (16 columns per thread measured slower: each lane's 16 doubles land 128 bytes
+        // apart; 32 x 128 tiles as fast, 64 x 64 tiles a third slower)
+        crt(std::integral_constant<int, 4>{});
+        check_launch("ozaki crt");
+    } else {
+        // symmetric product: the lower tiles' CRT writes each tile's mirror too (RI 24000^2:
+        // 1.70-1.86 ms against 1.77-1.95 ms for the row CRT plus 1.07-1.09 ms for k_mirror)
+        using CT = CrtTile<128>;
+        const uint64_t tmt = (uint64_t)((M + CT::TR - 1) / CT::TR), tnt = (uint64_t)((N + CT::TC - 1) / CT::TC);
+        const uint64_t ntl = lower_tile_prefix(tmt, (uint64_t)(CT::TC / CT::TR));
+        if (ntl >= (1ull << 31) || nb > 65535) RSTSR_RAISE(UnImplemented, "split GEMM output too large");
+        // the kernel reads residue rows and exponents of every tile row and column it runs
+        if ((int64_t)(tmt * CT::TR) > Mp || (int64_t)(tnt * CT::TC) > Np)
+            RSTSR_RAISE(RuntimeError, "CRT tiles exceed the padded planes");
+        auto kern = n == OZ_MAXMOD ? k_crt_tile<true, true, CT::TC> : k_crt_tile<false, true, CT::TC>;
+        hipLaunchKernelGGL(kern, dim3((uint32_t)ntl, (uint32_t)nb), dim3(256), 0, ctx->stream, R, M, N, Mp, Np, Np,
+                           (int64_t)planeR, a_b ? 1 : 0, b_b ? 1 : 0, P.bd, cc, eA, fB, (double*)P.c, P.rsc, P.csc,
+                           alpha, beta, beta_zero ? 1 : 0, flag, (uint32_t)tmt, (uint32_t)tnt);
+        check_launch("ozaki crt");
+    }
 }
 
 // Grouped split GEMM routing (RSTSR_HIP_OZAKI_GROUPED: "auto", the default, or "split").

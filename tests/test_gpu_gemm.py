@@ -579,11 +579,14 @@ def test_free_during_capture_is_deferred(dev):
     assert np.array_equal(tmp.to_numpy(), np.arange(64.0).reshape(8, 8) ** 2)
 
 
-@pytest.mark.parametrize("m,k", [(1, 5), (100, 64), (300, 257), (1024, 512)])
+@pytest.mark.parametrize("m,k", [(1, 5), (33, 40), (100, 64), (129, 70), (257, 33), (300, 257), (1024, 512),
+                                 (2000, 100)])
 def test_ozaki_syrk_bit_identical(dev, oracle, m, k):
     """The split GEMM's symmetric path (X^T X in one buffer, beta = 0: shared scales and residue
-    planes, lower-triangle INT8 tiles, CRT on i >= j, mirror) equals the split GEMM with B in a
-    separate buffer bit for bit (residue sums are exact integers), and stays within 1e-14 normwise."""
+    planes, lower-triangle INT8 tiles, a CRT over the lower 32 x 128 tiles that also writes each
+    tile's mirror) equals the split GEMM with B in a separate buffer bit for bit (residue sums are
+    exact integers), and stays within 1e-14 normwise. Sizes on both sides of the INT8 and CRT tile
+    edges."""
     rng = np.random.default_rng(m * 3 + k)
     X = wide_exponent(rng, (k, m))
     tX = rt.asarray(X, device=dev)
