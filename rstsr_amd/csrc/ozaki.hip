@@ -465,6 +465,92 @@ __global__ void __launch_bounds__(256) k_normsplit_rows(const double* __restrict
     }
 }
 
+// The same with the vector held in registers, read once: RPB vectors per block
+// (v = RPB blockIdx.x + t / TPR, TPR = 256 / RPB threads each) of batch b = blockIdx.y,
+// lenp <= 16 TPR CH. Thread t holds the 16-element chunks t % TPR + TPR i (i < CH) of its
+// vector; the norm combines over the vector's waves (through LDS in a fixed order when a
+// block holds one vector), then each chunk is split from registers. The wave-per-vector
+// kernel above reads each vector twice, and rows of 16 KiB and more miss the cache the
+// second time.
+template <bool K14, int RPB, int CH>
+__global__ void __launch_bounds__(256) k_normsplit_rows_reg(const double* __restrict__ x, int64_t nv, int64_t len,
+                                                            int64_t sv, BatchDesc bd, int which, int64_t nvp,
+                                                            int64_t lenp, int g, int* __restrict__ ev,
+                                                            int8_t* __restrict__ planes, int64_t plane, ModSet ms,
+                                                            int* __restrict__ flag) {
+    static_assert(RPB == 1 || RPB == 4, "a vector per block or per wave");
+    constexpr int TPR = 256 / RPB;
+    __shared__ NormAcc red[4];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, tr = tid % TPR;
+    const int64_t v = (int64_t)blockIdx.x * RPB + tid / TPR;  // uniform across each wave
+    const uint32_t b = blockIdx.y;
+    if (v >= nvp) return;
+    int8_t* pv = planes + ((int64_t)b * nvp + v) * lenp;
+    if (v >= nv) {  // padding vector: zero planes (uniform across the vector's threads)
+#pragma unroll
+        for (int i = 0; i < CH; ++i) {
+            const int64_t t0 = ((int64_t)tr + TPR * i) * 16;
+            if (t0 < lenp)
+                for (int l = 0; l < ms.n; ++l) *(uint4*)(pv + l * plane + t0) = uint4{0, 0, 0, 0};
+        }
+        return;
+    }
+    int64_t oa, ob, oc;
+    batch_offsets(bd, b, oa, ob, oc);
+    const double* xv = x + (which ? ob : oa) + v * sv;
+    const bool pairs = ((uintptr_t)xv & 15) == 0;  // 16-byte loads when aligned
+    double xs[CH][16];
+    NormAcc a{0.0, 0, 0.0};
+    bool bad = false;
+#pragma unroll
+    for (int i = 0; i < CH; ++i) {
+        const int64_t t0 = ((int64_t)tr + TPR * i) * 16;
+        if (t0 + 16 <= len) {
+            if (pairs) {
+#pragma unroll
+                for (int e = 0; e < 16; e += 2) {
+                    const double2 p2 = *(const double2*)(xv + t0 + e);
+                    xs[i][e] = p2.x;
+                    xs[i][e + 1] = p2.y;
+                }
+            } else {
+#pragma unroll
+                for (int e = 0; e < 16; ++e) xs[i][e] = xv[t0 + e];
+            }
+        } else {
+#pragma unroll
+            for (int e = 0; e < 16; ++e) xs[i][e] = t0 + e < len ? xv[t0 + e] : 0.0;
+        }
+#pragma unroll
+        for (int e = 0; e < 16; ++e) norm_add(a, xs[i][e], bad);
+    }
+    a = wave_norm(a);
+    if (bad) atomicOr(flag, 1);
+    if constexpr (RPB == 1) {
+        if (lane == 0) red[w] = a;
+        __syncthreads();
+        a = red[0];
+#pragma unroll
+        for (int q = 1; q < 4; ++q) norm_merge(a, red[q]);
+    }
+    const int e = scale_exponent(a, g);
+    if (tr == 0) ev[(int64_t)b * nvp + v] = e;
+#pragma unroll
+    for (int i = 0; i < CH; ++i) {
+        const int64_t t0 = ((int64_t)tr + TPR * i) * 16;
+        if (t0 >= lenp) continue;
+        Digits dg[16];
+#pragma unroll
+        for (int k = 0; k < 16; ++k) dg[k] = to_digits(rint(ldexp(xs[i][k], e)));
+        for_moduli<K14>(ms, [&](int l) {
+            const ResConst c = res_const<K14>(ms, l);
+            const uint4 wd{pack_residues(dg, c), pack_residues(dg + 4, c), pack_residues(dg + 8, c),
+                           pack_residues(dg + 12, c)};
+            *(uint4*)(pv + l * plane + t0) = wd;
+        });
+    }
+}
+
 /* ---------------- INT8 MFMA GEMM per modulus ---------------- */
 
 // INT8 tile configurations: a BM x BN block tile from WVM x WVN waves, K slices
@@ -1534,9 +1620,30 @@ void ozaki_gemm_f64(rstsr_hip_ctx* ctx, const GemmProblem& P, double alpha, doub
         }
         check_launch("ozaki norms");
     };
-    // vectors with contiguous elements: norm and split in one pass (k_normsplit_rows)
+    // vectors with contiguous elements: norm and split in one pass (k_normsplit_rows_reg, or
+    // k_normsplit_rows past 8192 elements)
     auto norm_split = [&](const NormPlan& q, const double* x, int64_t nv, int64_t nvp, int64_t sv, int64_t st,
                           int which, int* ev, int8_t* planes, size_t plane, const char* what) {
+        if (st == 1 && Kp <= 8192) {
+            // vectors held in registers, read once: four per block (one per wave) up to 2048
+            // elements, one per block up to 8192 (measured: 8192^2 A 380-404 -> 310-333 us,
+            // 4096^2 102-120 -> 80-102 us, 65536 x 1024 317-378 -> 284-332 us)
+            auto go = [&](auto rpb, auto ch) {
+                constexpr int RPB = decltype(rpb)::value, CH = decltype(ch)::value;
+                auto kern = n == OZ_MAXMOD ? k_normsplit_rows_reg<true, RPB, CH> : k_normsplit_rows_reg<false, RPB, CH>;
+                hipLaunchKernelGGL(kern, dim3(ceil_div_u32(nvp, RPB), (uint32_t)q.nbx), dim3(256), 0, ctx->stream, x,
+                                   nv, K, sv, P.bd, which, nvp, Kp, g, ev, planes, (int64_t)plane, ms, flag);
+            };
+            using I1 = std::integral_constant<int, 1>;
+            using I2 = std::integral_constant<int, 2>;
+            using I4 = std::integral_constant<int, 4>;
+            if (Kp <= 1024) go(I4{}, I1{});
+            else if (Kp <= 2048) go(I4{}, I2{});
+            else if (Kp <= 4096) go(I1{}, I1{});
+            else go(I1{}, I2{});
+            check_launch(what);
+            return;
+        }
         if (st == 1) {
             hipLaunchKernelGGL(n == OZ_MAXMOD ? k_normsplit_rows<true> : k_normsplit_rows<false>,
                                dim3(ceil_div_u32(nvp, 4), (uint32_t)q.nbx), dim3(256), 0,
