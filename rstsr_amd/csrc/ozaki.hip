@@ -186,8 +186,23 @@ __global__ void __launch_bounds__(256) k_norm_cols(const double* __restrict__ x,
     NormAcc a{0.0, 0, 0.0};
     bool bad = false;
     if (v < nvec) {
+        // four independent accumulators over rows t, t + 4, t + 8, t + 12 (four loads in flight,
+        // four norm chains instead of one), merged in a fixed order
         const int64_t t1 = min(len, (int64_t)(c + 1) * chunk);
-        for (int64_t t = (int64_t)c * chunk + w; t < t1; t += 4) norm_add(a, xb[v * sv + t * st], bad);
+        const double* xp = xb + v * sv;
+        NormAcc a1{0.0, 0, 0.0}, a2{0.0, 0, 0.0}, a3{0.0, 0, 0.0};
+        int64_t t = (int64_t)c * chunk + w;
+        for (; t + 12 < t1; t += 16) {
+            const double x0 = xp[t * st], x1 = xp[(t + 4) * st], x2 = xp[(t + 8) * st], x3 = xp[(t + 12) * st];
+            norm_add(a, x0, bad);
+            norm_add(a1, x1, bad);
+            norm_add(a2, x2, bad);
+            norm_add(a3, x3, bad);
+        }
+        for (; t < t1; t += 4) norm_add(a, xp[t * st], bad);
+        norm_merge(a, a1);
+        norm_merge(a, a2);
+        norm_merge(a, a3);
     }
     red[w][lane] = a;
     __syncthreads();
