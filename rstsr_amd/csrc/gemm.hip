@@ -772,13 +772,20 @@ static bool is_zero_scalar(rstsr_dtype_t t, const void* v) {
 }
 
 // S (rows x cols, dense row-major) = Re X + Im X for a strided complex matrix X
-__global__ void __launch_bounds__(256) k_zsum_parts(double* __restrict__ S, const c128* __restrict__ X, int64_t rows,
+__global__ void __launch_bounds__(256) k_zsum_parts(double* __restrict__ S, double* __restrict__ Xr,
+                                                    double* __restrict__ Xi, const c128* __restrict__ X, int64_t rows,
                                                     int64_t cols, int64_t rs, int64_t cs) {
-    const int64_t n = rows * cols;
-    for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < n; e += (int64_t)gridDim.x * 256) {
-        const int64_t i = e / cols, j = e - i * cols;
-        const c128 x = X[i * rs + j * cs];
-        S[e] = x.re + x.im;
+    // row blockIdx.y (grid-strided), columns along the block: S = re + im, and the real and
+    // imaginary planes as contiguous rows (the split then reads them once, at unit stride)
+    for (int64_t i = blockIdx.y; i < rows; i += gridDim.y) {
+        const c128* xr = X + i * rs;
+        const int64_t o = i * cols;
+        for (int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x; j < cols; j += (int64_t)gridDim.x * 256) {
+            const c128 x = xr[j * cs];
+            S[o + j] = x.re + x.im;
+            Xr[o + j] = x.re;
+            Xi[o + j] = x.im;
+        }
     }
 }
 
@@ -810,25 +817,34 @@ static void ozaki_zgemm(rstsr_hip_ctx* ctx, const GemmProblem& P, c128 alpha, c1
         const int64_t M = P.M, N = P.N, K = P.K;
         const size_t nSA = (size_t)M * K, nSB = (size_t)K * N, nT = (size_t)M * N;
         auto al = [](size_t x) { return (x * sizeof(double) + 255) & ~(size_t)255; };
-        char* z = (char*)zbuf_buffer(ctx, al(nSA) + al(nSB) + al(nT));
+        // the sums Ar + Ai and Br + Bi, T2, and the four parts as contiguous planes
+        char* z = (char*)zbuf_buffer(ctx, 3 * al(nSA) + 3 * al(nSB) + al(nT));
         double* SA = (double*)z;
         double* SB = (double*)(z + al(nSA));
         double* T2 = (double*)(z + al(nSA) + al(nSB));
-        auto grid = [](size_t n) { return dim3((uint32_t)std::min<size_t>((n + 255) / 256, 65536)); };
-        hipLaunchKernelGGL(k_zsum_parts, grid(nSA), dim3(256), 0, ctx->stream, SA, (const c128*)P.a, M, K, P.rsa, P.csa);
-        hipLaunchKernelGGL(k_zsum_parts, grid(nSB), dim3(256), 0, ctx->stream, SB, (const c128*)P.b, K, N, P.rsb, P.csb);
+        double* Ar = (double*)(z + al(nSA) + al(nSB) + al(nT));
+        double* Ai = Ar + al(nSA) / sizeof(double);
+        double* Br = Ai + al(nSA) / sizeof(double);
+        double* Bi = Br + al(nSB) / sizeof(double);
+        auto grid = [](int64_t rows, int64_t cols) {
+            return dim3((uint32_t)std::min<int64_t>((cols + 255) / 256, 64), (uint32_t)std::min<int64_t>(rows, 65535));
+        };
+        hipLaunchKernelGGL(k_zsum_parts, grid(M, K), dim3(256), 0, ctx->stream, SA, Ar, Ai, (const c128*)P.a, M, K,
+                           P.rsa, P.csa);
+        hipLaunchKernelGGL(k_zsum_parts, grid(K, N), dim3(256), 0, ctx->stream, SB, Br, Bi, (const c128*)P.b, K, N,
+                           P.rsb, P.csb);
         check_launch("complex split sums");
         const double a = alpha.re;
         GemmProblem Q = P;
         Q.bd.nbd = 0;
         Q.nbatch = 1;
         // T1 into Re C
-        Q.a = P.a;
-        Q.rsa = 2 * P.rsa;
-        Q.csa = 2 * P.csa;
-        Q.b = P.b;
-        Q.rsb = 2 * P.rsb;
-        Q.csb = 2 * P.csb;
+        Q.a = Ar;
+        Q.rsa = K;
+        Q.csa = 1;
+        Q.b = Br;
+        Q.rsb = N;
+        Q.csb = 1;
         Q.c = P.c;
         Q.rsc = 2 * P.rsc;
         Q.csc = 2 * P.csc;
@@ -843,17 +859,18 @@ static void ozaki_zgemm(rstsr_hip_ctx* ctx, const GemmProblem& P, c128 alpha, c1
         Q.c = (double*)P.c + 1;
         ozaki_gemm_f64(ctx, Q, a, 0.0, true);
         // T2 into the temporary
-        Q.a = (const double*)P.a + 1;
-        Q.rsa = 2 * P.rsa;
-        Q.csa = 2 * P.csa;
-        Q.b = (const double*)P.b + 1;
-        Q.rsb = 2 * P.rsb;
-        Q.csb = 2 * P.csb;
+        Q.a = Ai;
+        Q.rsa = K;
+        Q.csa = 1;
+        Q.b = Bi;
+        Q.rsb = N;
+        Q.csb = 1;
         Q.c = T2;
         Q.rsc = N;
         Q.csc = 1;
         ozaki_gemm_f64(ctx, Q, a, 0.0, true);
-        hipLaunchKernelGGL(k_z3m_combine, grid(nT), dim3(256), 0, ctx->stream, (c128*)P.c, P.rsc, P.csc, T2, M, N);
+        hipLaunchKernelGGL(k_z3m_combine, dim3((uint32_t)std::min<size_t>((nT + 255) / 256, 65536)), dim3(256), 0,
+                           ctx->stream, (c128*)P.c, P.rsc, P.csc, T2, M, N);
         check_launch("complex split combine");
         return;
     }
