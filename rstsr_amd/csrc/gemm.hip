@@ -370,11 +370,17 @@ template <class T, class CF>
 __global__ void __launch_bounds__(CF::NT, CF::OCC) k_gemm_grouped(const GroupProblem<T>* __restrict__ probs, int count,
                                                                   const uint32_t* __restrict__ prefix, uint32_t total,
                                                                   T alpha, T beta, int beta_zero,
-                                                                  const int* __restrict__ run_if) {
+                                                                  const int* __restrict__ run_if,
+                                                                  uint32_t* __restrict__ queue) {
     if (run_if && *run_if == 0) return;
     __shared__ MfmaSmem<T, CF> sm;
+    __shared__ uint32_t next_tile;
     constexpr bool RV = !CplxOf<T>::value;  // vector loads exist for real operands
-    for (uint32_t tile = blockIdx.x; tile < total; tile += gridDim.x) {
+    // tiles handed out in order: block b starts with tile b, then takes the next unclaimed one
+    // from the queue counter (zero at launch), claimed while its current tile computes
+    for (uint32_t tile = blockIdx.x; tile < total;) {
+        uint32_t claimed = 0;
+        if (threadIdx.x == 0) claimed = atomicAdd(queue, 1u) + gridDim.x;
         // problem of this tile: last g with prefix[g] <= tile
         int lo = 0, hi = count - 1;
         while (lo < hi) {
@@ -394,7 +400,10 @@ __global__ void __launch_bounds__(CF::NT, CF::OCC) k_gemm_grouped(const GroupPro
                 break;
             default: mfma_tile<T, CF, false, false, false>(gp.t, tm, tn, alpha, beta, beta_zero, sm); break;
         }
-        __syncthreads();  // LDS reuse by the next tile
+        if (threadIdx.x == 0) next_tile = claimed;
+        __syncthreads();  // LDS reuse by the next tile; next_tile visible
+        tile = next_tile;
+        __syncthreads();  // every thread has read next_tile before it is written again
     }
 }
 
@@ -672,17 +681,20 @@ static void launch_grouped_cfg(rstsr_hip_ctx* ctx, const std::vector<TileArgs<T>
     if (total == 0) return;
     if (total >= (1ull << 31)) RSTSR_RAISE(UnImplemented, "grouped GEMM has too many tiles");
     const size_t poff = (gp.size() * sizeof(GroupProblem<T>) + 255) & ~(size_t)255;
-    const size_t bytes = poff + prefix.size() * sizeof(uint32_t);
+    const size_t qoff = poff + prefix.size() * sizeof(uint32_t);  // the tile queue counter
+    const size_t bytes = qoff + sizeof(uint32_t);
     char* meta = (char*)aux_buffer(ctx, bytes + 256);
     char* host = (char*)meta_stage(ctx, bytes, "grouped GEMM");
     std::memcpy(host, gp.data(), gp.size() * sizeof(GroupProblem<T>));
     std::memcpy(host + poff, prefix.data(), prefix.size() * sizeof(uint32_t));
+    const uint32_t zero = 0;
+    std::memcpy(host + qoff, &zero, sizeof(zero));
     meta_upload(ctx, meta, bytes);
     // persistent grid: as many blocks as can be resident at once
     uint32_t grid = (uint32_t)std::min<uint64_t>(total, (uint64_t)std::max(ctx->num_cu, 1) * CF::OCC);
     hipLaunchKernelGGL((k_gemm_grouped<T, CF>), dim3(grid), dim3(CF::NT), 0, ctx->stream,
                        (const GroupProblem<T>*)meta, (int)gp.size(), (const uint32_t*)(meta + poff), (uint32_t)total,
-                       alpha, beta, beta_zero ? 1 : 0, run_if);
+                       alpha, beta, beta_zero ? 1 : 0, run_if, (uint32_t*)(meta + qoff));
     check_launch("grouped gemm");
 }
 
