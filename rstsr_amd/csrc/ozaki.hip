@@ -830,6 +830,10 @@ __device__ __forceinline__ void i8_tile(const int8_t* __restrict__ A, const int8
                       "direct-load tile shape");
         const int wv = __builtin_amdgcn_readfirstlane(wave);
         const int dr = lane / CPR, dch = ((lane % CPR) ^ swz(lane / CPR)) * 16;
+        // the lane's part of each load address (row dr of the wave's rows, its chunk): one 32-bit
+        // offset (dr < 8, ldk < 2^28: the host keeps longer K off this tile) on top of a
+        // wave-uniform base
+        const uint32_t loff = (uint32_t)(dr * ldk + dch);
         // The block's nmod moduli (planes pA, pB, pR apart) run as one sequence of nmod * ns
         // slices: the next modulus's first slices load during the last products and the
         // residue epilogue of the one before.
@@ -838,14 +842,14 @@ __device__ __forceinline__ void i8_tile(const int8_t* __restrict__ A, const int8
 #pragma unroll
             for (int i = 0; i < BM / RP; ++i) {
                 const int r0 = i * RP + wv * RW;
-                __builtin_amdgcn_global_load_lds(Ab + da + (size_t)(r0 + dr) * ldk + dk + dch, &As[st][r0][0], 16, 0,
-                                                 0);
+                const int8_t* base = Ab + da + (size_t)r0 * ldk + dk;
+                __builtin_amdgcn_global_load_lds(base + loff, &As[st][r0][0], 16, 0, 0);
             }
 #pragma unroll
             for (int i = 0; i < BN / RP; ++i) {
                 const int r0 = i * RP + wv * RW;
-                __builtin_amdgcn_global_load_lds(Bb + db + (size_t)(r0 + dr) * ldk + dk + dch, &Bs[st][r0][0], 16, 0,
-                                                 0);
+                const int8_t* base = Bb + db + (size_t)r0 * ldk + dk;
+                __builtin_amdgcn_global_load_lds(base + loff, &Bs[st][r0][0], 16, 0, 0);
             }
             dk += BK;
             if (dk == Kc) {
@@ -1532,7 +1536,9 @@ void ozaki_gemm_f64(rstsr_hip_ctx* ctx, const GemmProblem& P, double alpha, doub
     const ModSet& ms = su.ms;
     const int64_t M = P.M, N = P.N, K = P.K;
     if (K > (int64_t)1 << 40) RSTSR_RAISE(UnImplemented, "K too large for the split GEMM");
-    const I8Tile tile = pick_i8_tile(M, N);
+    I8Tile tile = pick_i8_tile(M, N);
+    // the direct-load tile forms its lanes' load offsets in 32 bits (row < 8, ldk < 2^28)
+    if (tile == I8Tile::Big && K >= ((int64_t)1 << 28) - I8_KPAD) tile = I8Tile::Wide8;
     int64_t bm = 0, bn = 0;
     with_i8_tile(tile, [&](auto c) {
         bm = decltype(c)::BM;
