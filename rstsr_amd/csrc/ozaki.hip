@@ -1096,7 +1096,7 @@ __device__ __forceinline__ double crt_value(const uint8_t* __restrict__ p, int64
     }
 }
 
-// R planes: [l][batch][Mp][N] (plane stride pl); eA [nbA][Mp], fB [nbB][Np].
+// R planes: [l][batch][Mp][ldr] (plane stride pl, rows ldr >= Np bytes); eA [nbA][Mp], fB [nbB][Np].
 // C = alpha 2^-(eA_i + fB_j) CRT(R) + beta C: columns along x (V consecutive
 // per thread, read as one V-byte word per plane), rows along y, batches along z.
 template <int V, bool K14>  // columns per thread: 1, 4 (one 4-byte word per plane) or 16 (one 16-byte word)
@@ -1104,7 +1104,7 @@ __global__ void __launch_bounds__(256) k_crt(const uint8_t* __restrict__ R, int6
                                              int64_t Np, int64_t ldr, int64_t pl, int a_batched, int b_batched, BatchDesc bd,
                                              CrtConst cc, const int* __restrict__ eA, const int* __restrict__ fB,
                                              double* C, int64_t rsc, int64_t csc, double alpha, double beta,
-                                             int beta_zero, const int* __restrict__ skip, int lower, int tpr) {
+                                             int beta_zero, const int* __restrict__ skip, int tpr) {
     static_assert(V == 1 || V == 4 || V == 16, "word size");
     if (*skip) return;
     // tpr threads (a power of two <= 256) per row, 256 / tpr rows of a block at a time
@@ -1120,7 +1120,6 @@ __global__ void __launch_bounds__(256) k_crt(const uint8_t* __restrict__ R, int6
         const uint8_t* rp = R + ((int64_t)b * Mp + i) * ldr + j0;
         double* cp = C + oc + i * rsc + j0 * csc;
         const int ei = eAb[i];
-        if (lower && j0 > i) continue;  // lower triangle only: j > i left to the caller
         if constexpr (V == 1) {
             const double v = ldexp(crt_value<K14>(rp, pl, cc), -(ei + fBb[j0]));
             *cp = beta_zero ? alpha * v : alpha * v + beta * (*cp);
@@ -1147,7 +1146,7 @@ __global__ void __launch_bounds__(256) k_crt(const uint8_t* __restrict__ R, int6
 #pragma unroll
                 for (int l = 0; l < OZ_MAXMOD; ++l) r[l] = (w[l][k / 4] >> (8 * (k % 4))) & 255u;
                 const double v = ldexp(crt_from<K14>(r, cc), -(ei + fBb[j0 + k]));
-                if (j0 + k < N && !(lower && j0 + k > i)) {
+                if (j0 + k < N) {
                     double* ck = cp + k * csc;
                     *ck = beta_zero ? alpha * v : alpha * v + beta * (*ck);
                 }
@@ -1745,7 +1744,7 @@ void ozaki_gemm_f64(rstsr_hip_ctx* ctx, const GemmProblem& P, double alpha, doub
         auto kern = n == OZ_MAXMOD ? k_crt<V, true> : k_crt<V, false>;
         hipLaunchKernelGGL(kern, dim3(ceil_div_u32(ncg, tpr), (uint32_t)gy, (uint32_t)nb), dim3(256), 0,
                            ctx->stream, R, M, N, Mp, Np, Np, (int64_t)planeR, a_b ? 1 : 0, b_b ? 1 : 0, P.bd, cc, eA, fB,
-                           (double*)P.c, P.rsc, P.csc, alpha, beta, beta_zero ? 1 : 0, flag, sym ? 1 : 0, tpr);
+                           (double*)P.c, P.rsc, P.csc, alpha, beta, beta_zero ? 1 : 0, flag, tpr);
     };
     if (!sym) {
         // R rows are Np bytes long (a multiple of 128): four columns per thread, one 4-byte word
