@@ -1116,6 +1116,53 @@ __global__ void __launch_bounds__(256) k_crt(const uint8_t* __restrict__ R, int6
     batch_offsets(bd, b, oa, ob, oc);
     const int* eAb = eA + (a_batched ? (int64_t)b * Mp : 0);
     const int* fBb = fB + (b_batched ? (int64_t)b * Np : 0);
+    if constexpr (V == 4) {
+        // one 4-byte word per plane and row; the next row's words load while this row's
+        // CRT runs, and aligned unit-stride rows store two doubles at a time
+        const int nm = K14 ? OZ_MAXMOD : cc.n;  // a constant for the default moduli: no branches
+        const int64_t step = (int64_t)gridDim.y * rpb;
+        int64_t i = (int64_t)blockIdx.y * rpb + threadIdx.x / tpr;
+        auto load = [&](int64_t row, uint32_t (&w)[OZ_MAXMOD]) {
+            const uint8_t* rp = R + ((int64_t)b * Mp + row) * ldr + j0;
+#pragma unroll
+            for (int l = 0; l < OZ_MAXMOD; ++l) w[l] = l < nm ? *(const uint32_t*)(rp + l * pl) : 0u;
+        };
+        uint32_t w[OZ_MAXMOD], wn[OZ_MAXMOD];
+        if (i < M) load(i, w);
+        int fj[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) fj[k] = fBb[j0 + k];  // fB holds Np >= j0 + 4 entries
+        for (; i < M; i += step) {
+            const bool more = i + step < M;
+            if (more) load(i + step, wn);
+            const int ei = eAb[i];
+            double v[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                uint32_t r[OZ_MAXMOD];
+#pragma unroll
+                for (int l = 0; l < OZ_MAXMOD; ++l) r[l] = (w[l] >> (8 * k)) & 255u;
+                v[k] = alpha * ldexp(crt_from<K14>(r, cc), -(ei + fj[k]));
+            }
+            double* cp = C + oc + i * rsc + j0 * csc;
+            if (csc == 1 && beta_zero && j0 + 4 <= N && ((uintptr_t)cp & 15) == 0) {
+                *(double2*)cp = double2{v[0], v[1]};
+                *(double2*)(cp + 2) = double2{v[2], v[3]};
+            } else {
+#pragma unroll
+                for (int k = 0; k < 4; ++k)
+                    if (j0 + k < N) {
+                        double* ck = cp + k * csc;
+                        *ck = beta_zero ? v[k] : v[k] + beta * (*ck);
+                    }
+            }
+            if (more) {
+#pragma unroll
+                for (int l = 0; l < OZ_MAXMOD; ++l) w[l] = wn[l];
+            }
+        }
+        return;
+    }
     for (int64_t i = (int64_t)blockIdx.y * rpb + threadIdx.x / tpr; i < M; i += (int64_t)gridDim.y * rpb) {
         const uint8_t* rp = R + ((int64_t)b * Mp + i) * ldr + j0;
         double* cp = C + oc + i * rsc + j0 * csc;
