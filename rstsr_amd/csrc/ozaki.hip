@@ -334,13 +334,22 @@ __device__ __forceinline__ void for_moduli(const ModSet& ms, F&& f) {
 // writes the residues; split_tile is the two in turn.
 __device__ __forceinline__ void split_load(const double* __restrict__ s, int64_t nv, int64_t len, int64_t sv,
                                            int64_t st, bool read_along_t, int64_t v0, int64_t t0, double (&x)[16]) {
-    const int tid = threadIdx.x;
+    // element k of this thread: the fast index tid % 64 along the unit-stride axis, the slow one
+    // 4k + tid / 64; its address is a uniform base plus a per-lane offset plus k steps of a
+    // uniform stride (no per-element 64-bit address kept live)
+    const int tid = threadIdx.x, fast = tid & 63, slow0 = tid >> 6;
+    const int64_t lane = read_along_t ? (int64_t)slow0 * sv + (int64_t)fast * st : (int64_t)fast * sv + (int64_t)slow0 * st;
+    const int64_t kstep = 4 * (read_along_t ? sv : st);
+    const double* base = s + v0 * sv + t0 * st;
+    if (v0 + 64 <= nv && t0 + 64 <= len) {
 #pragma unroll
-    for (int k = 0; k < 16; ++k) {
-        const int e = k * 256 + tid, fast = e & 63, slow = e >> 6;
-        const int tt = read_along_t ? fast : slow, vv = read_along_t ? slow : fast;
-        const int64_t gv = v0 + vv, gt = t0 + tt;
-        x[k] = gv < nv && gt < len ? s[gv * sv + gt * st] : 0.0;
+        for (int k = 0; k < 16; ++k) x[k] = base[k * kstep + lane];
+    } else {
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            const int tt = read_along_t ? fast : 4 * k + slow0, vv = read_along_t ? 4 * k + slow0 : fast;
+            x[k] = v0 + vv < nv && t0 + tt < len ? base[k * kstep + lane] : 0.0;
+        }
     }
 }
 
