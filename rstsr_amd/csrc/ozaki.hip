@@ -230,7 +230,8 @@ struct ModSet {
     int n;
     double m[OZ_MAXMOD];
     uint32_t mi[OZ_MAXMOD];     // m
-    uint32_t magic[OZ_MAXMOD];  // ceil(2^32 / m), for the INT8 epilogue
+    uint32_t c16[OZ_MAXMOD];    // 2^16 mod m, for the INT8 epilogue
+    float inv[OZ_MAXMOD];       // f32(1/m) (1 - 2^-22), for the INT8 epilogue
     uint32_t clo[OZ_MAXMOD];    // bytes 256^k mod m, k = 0..3
     uint32_t chi[OZ_MAXMOD];    // bytes 256^k mod m, k = 4..7
     uint32_t off[OZ_MAXMOD];    // m - (2^56 mod m) + 128
@@ -608,13 +609,18 @@ constexpr int I8_MPAD = 128, I8_KPAD = 128;
 constexpr int64_t I8_KCHUNK = 65536;
 static_assert(I8Sq::BM == I8_MPAD && I8Sq::BN == 128, "grouped problems pad to the square tile");
 
-// x mod m in [0, m) for |x| <= 2^30 and 193 <= m <= 256, magic = ceil(2^32 / m): u = x + m 2^23
-// lies in [0, 2^32); umulhi(u, magic) is floor(u / m) or one more (u magic / 2^32 exceeds u / m by
-// u (magic m - 2^32) / (m 2^32) < 1), so u - q m lies in [-m, m) and one correction ends it.
-__device__ __forceinline__ uint32_t mod_u32(int x, uint32_t m, uint32_t magic) {
-    const uint32_t u = (uint32_t)x + (m << 23);
-    const int r = (int)(u - __umulhi(u, magic) * m);
-    return (uint32_t)(r < 0 ? r + (int)m : r);
+// x mod m in [0, m) for |x| <= 2^30 and 199 <= m <= 256 in full-rate operations (no 32-bit
+// multiplies, which issue at a quarter of the rate): x = xh 2^16 + xl (xh = x >> 16, xl the low
+// 16 bits); s = xh c16 + xl + m 2^14 with c16 = 2^16 mod m is x mod m and lies in [2^14, 2^24)
+// (|xh| <= 2^14, c16 < m), one 24-bit multiply-add; q = trunc(f32(s) inv) with
+// inv = f32(1/m) (1 - 2^-22) is floor(s / m) or one less (checked for every s < 2^24 and every
+// modulus), so t = s - q m (24-bit multiply-add) lies in [0, 2m) and min(t, t - m) as unsigned
+// words is the residue.
+__device__ __forceinline__ uint32_t mod_res(int x, uint32_t m, uint32_t c16, float inv) {
+    const int s = __mul24(x >> 16, (int)c16) + (x & 0xFFFF) + (int)(m << 14);
+    const int q = (int)((float)s * inv);
+    const int t = __mul24(q, -(int)m) + s;
+    return min((uint32_t)t, (uint32_t)(t - (int)m));
 }
 
 // x mod m in [0, m) for |x| < 2^31, m <= 256, inv = 1/m: x * inv is within
@@ -638,7 +644,8 @@ __device__ __forceinline__ void vm_wait_barrier() {
 // Moduli of one launch
 struct ModList {
     int m[OZ_MAXMOD];
-    uint32_t magic[OZ_MAXMOD];  // ceil(2^32 / m)
+    uint32_t c16[OZ_MAXMOD];  // 2^16 mod m (mod_res)
+    float inv[OZ_MAXMOD];     // f32(1/m) (1 - 2^-22) (mod_res)
 };
 
 template <class CF>
@@ -659,7 +666,7 @@ struct I8Smem {
 // into a ring of ST buffers, ST - 1 slices ahead.
 // TOUT: each product block is formed transposed (B fragment as the MFMA's row
 // operand), so a lane ends with runs of four consecutive columns of one row of
-// R: one 4-byte store per run (N % 4 == 0), the residue by mod_u32. The 32 x 32
+// R: one 4-byte store per run (N % 4 == 0), the residue by mod_res. The 32 x 32
 // blocks always use it.
 // MM (direct loads, TOUT, no accum; R holds every row and column of the tile grid, ldr a
 // multiple of 4): the tile's products for nmod moduli (planes pA, pB, pR apart; moduli
@@ -668,7 +675,7 @@ template <class CF, bool ROW4G, bool TOUT = false, bool MM = false>
 __device__ __forceinline__ void i8_tile(const int8_t* __restrict__ A, const int8_t* __restrict__ B,
                                         uint8_t* __restrict__ R, int64_t M, int64_t N, int64_t ldk, int64_t Kc,
                                         int64_t ldr, int mod, bool accum, uint32_t tm, uint32_t tn, I8Smem<CF>& sm,
-                                        uint32_t magic = 0, int nmod = 1, int64_t pA = 0, int64_t pB = 0,
+                                        uint32_t c16 = 0, float inv = 0.f, int nmod = 1, int64_t pA = 0, int64_t pB = 0,
                                         int64_t pR = 0, const ModList* ml = nullptr, int l0 = 0) {
     constexpr int BM = CF::BM, BN = CF::BN, BK = CF::BK, NT = CF::NT, FR = CF::FR, D = CF::D;
     constexpr bool T = TOUT || FR == 32;
@@ -765,7 +772,7 @@ __device__ __forceinline__ void i8_tile(const int8_t* __restrict__ A, const int8
     };
 
     // residues of the accumulators mod modm into Rm (added to those there when accum)
-    auto epilogue = [&](uint8_t* __restrict__ Rm, int modm, uint32_t magm) {
+    auto epilogue = [&](uint8_t* __restrict__ Rm, int modm, uint32_t c16m, float invm) {
         if constexpr (T) {
             static_assert(ROW4G, "transposed output assumes rows 4 (lane / 16) + r of the 16 x 16 MFMA result");
             // block (i, j), register quad q: lane holds Rm[row lane % FR][4 consecutive columns from cq]
@@ -782,7 +789,7 @@ __device__ __forceinline__ void i8_tile(const int8_t* __restrict__ A, const int8
                         if (gm >= M) continue;
                         uint32_t w = 0;
     #pragma unroll
-                        for (int r = 0; r < 4; ++r) w |= mod_u32(acc[i][j][4 * q + r], (uint32_t)modm, magm) << (8 * r);
+                        for (int r = 0; r < 4; ++r) w |= mod_res(acc[i][j][4 * q + r], (uint32_t)modm, c16m, invm) << (8 * r);
                         uint8_t* rp = Rm + gm * ldr + gn;
                         if (vec && gn + 4 <= N) {
                             *(uint32_t*)rp = w;
@@ -896,7 +903,8 @@ __device__ __forceinline__ void i8_tile(const int8_t* __restrict__ A, const int8
             for (int g = 0; g < nmod; ++g) {
                 slices(Kc / BK);
                 const int l = l0 + g;
-                const uint32_t modm = (uint32_t)ml->m[l], magm = ml->magic[l];
+                const uint32_t modm = (uint32_t)ml->m[l], c16m = ml->c16[l];
+                const float invm = ml->inv[l];
                 uint8_t* Rm = R + (int64_t)g * pR;
 #pragma unroll
                 for (int i = 0; i < FM; ++i)
@@ -909,7 +917,7 @@ __device__ __forceinline__ void i8_tile(const int8_t* __restrict__ A, const int8
                             const int64_t gn = (int64_t)tn * BN + wn * WTN + j * FR + cq;
                             uint32_t w = 0;
 #pragma unroll
-                            for (int r = 0; r < 4; ++r) w |= mod_u32(acc[i][j][4 * q + r], modm, magm) << (8 * r);
+                            for (int r = 0; r < 4; ++r) w |= mod_res(acc[i][j][4 * q + r], modm, c16m, invm) << (8 * r);
                             *(uint32_t*)(Rm + gm * ldr + gn) = w;
                         }
                 zero_acc();
@@ -939,7 +947,7 @@ __device__ __forceinline__ void i8_tile(const int8_t* __restrict__ A, const int8
             }
         }
     }
-    if constexpr (!MM) epilogue(R, mod, magic);
+    if constexpr (!MM) epilogue(R, mod, c16, inv);
 }
 
 // Batches on blockIdx.y: plane offsets sA / sB (0 for a broadcast operand) and sR.
@@ -1033,7 +1041,7 @@ __global__ void __launch_bounds__(CF::NT, CF::OCC) k_i8gemm_mod(const int8_t* __
     else tile_coords<4>(blockIdx.x, tiles_m, tiles_n, tm, tn);
     i8_tile<CF, ROW4G, TOUT, MM>(A + (int64_t)blockIdx.y * sA, B + (int64_t)blockIdx.y * sB,
                              R + (int64_t)blockIdx.y * sR, M, N, ldk, Kc, ldr, mod, accum != 0, tm, tn, sm,
-                             mods.magic[l0], g, pA, pB, pR, &mods, l0);
+                             mods.c16[l0], mods.inv[l0], g, pA, pB, pR, &mods, l0);
 }
 
 // zero n ints (a kernel rather than a memset node, so the sequence can be graph-captured)
@@ -1545,7 +1553,8 @@ static OzSetup ozaki_setup() {
         const ResConst c = make_res_const(m);
         su.ms.m[l] = (double)m;
         su.ms.mi[l] = m;
-        su.ms.magic[l] = (uint32_t)(((1ull << 32) + m - 1) / m);
+        su.ms.c16[l] = (1u << 16) % m;
+        su.ms.inv[l] = (float)(1.0 / (double)m) * (1.0f - 0x1p-22f);
         su.ms.clo[l] = c.clo;
         su.ms.chi[l] = c.chi;
         su.ms.off[l] = c.off;
@@ -1754,7 +1763,8 @@ void ozaki_gemm_f64(rstsr_hip_ctx* ctx, const GemmProblem& P, double alpha, doub
     ModList mods{};
     for (int l = 0; l < n; ++l) {
         mods.m[l] = (int)cc.mod[l];
-        mods.magic[l] = ms.magic[l];
+        mods.c16[l] = ms.c16[l];
+        mods.inv[l] = ms.inv[l];
     }
     // K beyond I8_KCHUNK: further launches add their residues into R
     for (int64_t k0 = 0; k0 < Kp; k0 += I8_KCHUNK) {
