@@ -846,6 +846,9 @@ __device__ __forceinline__ void i8_tile(const int8_t* __restrict__ A, const int8
                       "direct-load tile shape");
         const int wv = __builtin_amdgcn_readfirstlane(wave);
         const int dr = lane / CPR, dch = ((lane % CPR) ^ swz(lane / CPR)) * 16;
+        // the buffers are written only by the direct loads, which the compiler does not count as
+        // stores: their address escapes here, so no read of them can be folded away as undefined
+        asm volatile("" ::"v"(&sm) : "memory");
         // the lane's part of each load address (row dr of the wave's rows, its chunk): one 32-bit
         // offset (dr < 8, ldk < 2^28: the host keeps longer K off this tile) on top of a
         // wave-uniform base
