@@ -257,6 +257,53 @@ def test_ozaki_big_tile_long_k(dev, oracle, int8_tile):
     assert np.max(np.abs(got - ref)) <= 4e-14 * np.max(np.abs(ref))
 
 
+@pytest.mark.parametrize("nmod", [12, 7])
+def test_ozaki_fewer_moduli(dev, oracle, int8_tile, nmod):
+    """RSTSR_HIP_OZAKI_MODULI below 14 takes every split kernel's runtime-constant form (residues,
+    INT8 epilogue, row and tiled CRT, several moduli per block over a divisor of nmod): the 256 x 256
+    and 128 x 128 tiles give the same bits, and each element meets the split's bound. With the scale
+    g = floor((log2 M - 1.5) / 2) of the first nmod moduli, every split element is within
+    2^-g ||row||_2 of its operand's element (the scale is a power of two at most twice the norm), so
+    by Cauchy-Schwarz |C - C*|_ij <= 2 sqrt(K) 2^-g ||a_i|| ||b_j|| (+ the final rounding)."""
+    moduli = [256, 255, 253, 251, 247, 241, 239, 233, 229, 227, 223, 217, 211, 199][:nmod]
+    g = min(55, int(np.floor((np.sum(np.log2(moduli)) - 1.5) / 2)))
+    rng = np.random.default_rng(30 + nmod)
+    A = rng.standard_normal((300, 1000))
+    B = rng.standard_normal((1000, 520))
+    A3 = rng.standard_normal((4, 300, 129))
+    B2 = rng.standard_normal((129, 517))
+    X = rng.standard_normal((90, 257))
+    tA, tB, tA3, tB2, tX = (rt.asarray(v, device=dev) for v in (A, B, A3, B2, X))
+    old = os.environ.get("RSTSR_HIP_OZAKI_MODULI")
+    os.environ["RSTSR_HIP_OZAKI_MODULI"] = str(nmod)
+    try:
+        got = {}
+        for tile in ("big", "square"):
+            int8_tile(tile)
+            got[tile] = (rt.matmul(tA, tB, mode="ozaki").to_numpy(), rt.matmul(tA3, tB2, mode="ozaki").to_numpy(),
+                         rt.matmul(tX.T, tX, mode="ozaki").to_numpy())
+    finally:
+        if old is None:
+            os.environ.pop("RSTSR_HIP_OZAKI_MODULI", None)
+        else:
+            os.environ["RSTSR_HIP_OZAKI_MODULI"] = old
+    for a, b in zip(got["big"], got["square"]):
+        assert np.array_equal(a.view(np.uint64), b.view(np.uint64))
+
+    def within(c, a, b):
+        ref = oracle.gemm_dot2(np.ascontiguousarray(a), np.ascontiguousarray(b))
+        bound = 2.0 * np.sqrt(a.shape[1]) * 2.0 ** -g * np.outer(np.linalg.norm(a, axis=1),
+                                                                np.linalg.norm(b, axis=0))
+        assert np.all(np.abs(c - ref) <= bound + 2.0 ** -52 * np.abs(ref))
+
+    c, c3, cs = got["big"]
+    within(c, A, B)
+    for bi in (0, 3):
+        within(c3[bi], A3[bi], B2)
+    assert np.array_equal(cs, cs.T)
+    within(cs, X.T, X)
+
+
 def test_ozaki_nonfinite_falls_back(dev):
     A = np.ones((20, 30))
     A[3, 4] = np.inf
