@@ -80,11 +80,12 @@ struct TileArgs {
 
 // Block tile configuration: WVM x WVN waves, each owning FM x FN 16x16 output
 // tiles, so a block covers BM x BN = (16 WVM FM) x (16 WVN FN).
-template <int WVM_, int WVN_, int FM_, int FN_>
+// OCC: waves per SIMD the register budget is set for (the launch bound's second argument).
+template <int WVM_, int WVN_, int FM_, int FN_, int OCC_ = 2>
 struct MCfg {
     static constexpr int WVM = WVM_, WVN = WVN_, FM = FM_, FN = FN_;
     static constexpr int NT = 64 * WVM * WVN, BM = 16 * WVM * FM, BN = 16 * WVN * FN;
-    static constexpr int OCC = 2;  // blocks per CU the register budget is set for
+    static constexpr int OCC = OCC_;
 };
 
 // LDS planes of one block (re/im planes for complex). Real types double-buffer
@@ -472,6 +473,7 @@ using D0 = MCfg<4, 2, 2, 4>;  // f64 128 x 128, 8 waves of 32 x 64
 using D1 = MCfg<4, 4, 2, 2>;  // f64 128 x 128, 16 waves of 32 x 32
 using D2 = MCfg<2, 2, 4, 4>;  // f64 128 x 128, 4 waves of 64 x 64
 using D3 = MCfg<2, 2, 2, 2>;  // f64 64 x 64, 4 waves of 32 x 32 (small outputs)
+using D4 = MCfg<4, 2, 2, 2, 4>;  // f64 128 x 64, 8 waves of 32 x 32, two blocks per CU (grouped only)
 using Z0 = MCfg<2, 2, 2, 4>;  // c128 64 x 128, 4 waves of 32 x 64
 using Z1 = MCfg<2, 2, 2, 2>;  // c128 64 x 64
 using Z2 = MCfg<2, 4, 2, 2>;  // c128 64 x 128, 8 waves of 32 x 32
@@ -690,8 +692,15 @@ static void launch_grouped_cfg(rstsr_hip_ctx* ctx, const std::vector<TileArgs<T>
     const uint32_t zero = 0;
     std::memcpy(host + qoff, &zero, sizeof(zero));
     meta_upload(ctx, meta, bytes);
-    // persistent grid: as many blocks as can be resident at once
-    uint32_t grid = (uint32_t)std::min<uint64_t>(total, (uint64_t)std::max(ctx->num_cu, 1) * CF::OCC);
+    // persistent grid: as many blocks as can be resident at once. The launch bound sets waves per
+    // SIMD, not blocks per CU, so the count comes from the occupancy query: a block beyond it would
+    // start only once the queue is empty and then run its first tile (one of the longest) alone.
+    static const int per_cu = [] {
+        int v = 0;
+        HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&v, k_gemm_grouped<T, CF>, CF::NT, 0));
+        return std::max(v, 1);
+    }();
+    uint32_t grid = (uint32_t)std::min<uint64_t>(total, (uint64_t)std::max(ctx->num_cu, 1) * per_cu);
     hipLaunchKernelGGL((k_gemm_grouped<T, CF>), dim3(grid), dim3(CF::NT), 0, ctx->stream,
                        (const GroupProblem<T>*)meta, (int)gp.size(), (const uint32_t*)(meta + poff), (uint32_t)total,
                        alpha, beta, beta_zero ? 1 : 0, run_if, (uint32_t*)(meta + qoff));
@@ -714,13 +723,14 @@ static void grouped_t(rstsr_hip_ctx* ctx, const std::vector<TileArgs<T>>& probs,
         } else {
             if (c == 0) launch_grouped_cfg<T, D0>(ctx, probs, alpha, beta, beta_zero);
             else if (c == 1) launch_grouped_cfg<T, D1>(ctx, probs, alpha, beta, beta_zero);
-            else launch_grouped_cfg<T, D3>(ctx, probs, alpha, beta, beta_zero);
+            else if (c == 2) launch_grouped_cfg<T, D3>(ctx, probs, alpha, beta, beta_zero);
+            else launch_grouped_cfg<T, D4>(ctx, probs, alpha, beta, beta_zero);
         }
     };
     char key[160];
     std::snprintf(key, sizeof(key), "grouped:%s:n%d:mean%d:work%d", CX ? "z" : "d", bucket((int64_t)probs.size()),
                   bucket((int64_t)mean), bucket(tot_mnk));
-    tuned_run(ctx, key, CX ? 2 : 3, mean >= 128 ? 0 : (CX ? 1 : 2), beta_zero, run);
+    tuned_run(ctx, key, CX ? 2 : 4, mean >= 128 ? (CX ? 0 : 1) : (CX ? 1 : 2), beta_zero, run);
 }
 
 void plain_grouped_f64(rstsr_hip_ctx* ctx, const std::vector<GemmProblem>& probs, double alpha, double beta,
@@ -733,7 +743,7 @@ void plain_grouped_f64(rstsr_hip_ctx* ctx, const std::vector<GemmProblem>& probs
     if (ta.empty()) return;
     // unconditional: the tuned configuration; conditional (a fallback inside a captured
     // sequence): the fixed one, since the autotuner cannot time a launch that may not run
-    if (run_if) launch_grouped_cfg<double, D0>(ctx, ta, alpha, beta, beta_zero, run_if);
+    if (run_if) launch_grouped_cfg<double, D1>(ctx, ta, alpha, beta, beta_zero, run_if);
     else grouped_t<double>(ctx, ta, alpha, beta, beta_zero);
 }
 

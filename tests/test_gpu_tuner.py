@@ -134,7 +134,7 @@ def test_gemm_tuner_candidates_all_in_parity(tmp_path):
             return 4
         if key.startswith("gemm:z"):
             return 1 if int(key.rsplit(":f", 1)[1]) & 8 else 3
-        return 3 if key.startswith("grouped:d") else 2
+        return 4 if key.startswith("grouped:d") else 2
 
     for c in range(4):
         cache.write_text("".join(f"{k} {min(c, ncand(k) - 1)}\n" for k in keys))
