@@ -367,7 +367,9 @@ struct GroupProblem {
     uint32_t flags;
 };
 
-template <class T, class CF>
+// VM: bit f set for each loader form f (GroupProblem::flags) the launch holds; a kernel with fewer
+// forms holds fewer inlined tile loops, so it needs fewer registers
+template <class T, class CF, uint32_t VM = 31>
 __global__ void __launch_bounds__(CF::NT, CF::OCC) k_gemm_grouped(const GroupProblem<T>* __restrict__ probs, int count,
                                                                   const uint32_t* __restrict__ prefix, uint32_t total,
                                                                   T alpha, T beta, int beta_zero,
@@ -393,13 +395,21 @@ __global__ void __launch_bounds__(CF::NT, CF::OCC) k_gemm_grouped(const GroupPro
         const uint32_t local = tile - prefix[lo];
         const uint32_t tm = local / gp.tiles_n, tn = local - (local / gp.tiles_n) * gp.tiles_n;
         switch (gp.flags & 7) {
-            case 3: mfma_tile<T, CF, true, true, false>(gp.t, tm, tn, alpha, beta, beta_zero, sm); break;
-            case 1: mfma_tile<T, CF, true, false, false>(gp.t, tm, tn, alpha, beta, beta_zero, sm); break;
-            case 2: mfma_tile<T, CF, false, true, false>(gp.t, tm, tn, alpha, beta, beta_zero, sm); break;
-            case 4:
-                if constexpr (RV) mfma_tile<T, CF, false, false, RV>(gp.t, tm, tn, alpha, beta, beta_zero, sm);
+            case 3:
+                if constexpr ((VM & 8) != 0) mfma_tile<T, CF, true, true, false>(gp.t, tm, tn, alpha, beta, beta_zero, sm);
                 break;
-            default: mfma_tile<T, CF, false, false, false>(gp.t, tm, tn, alpha, beta, beta_zero, sm); break;
+            case 1:
+                if constexpr ((VM & 2) != 0) mfma_tile<T, CF, true, false, false>(gp.t, tm, tn, alpha, beta, beta_zero, sm);
+                break;
+            case 2:
+                if constexpr ((VM & 4) != 0) mfma_tile<T, CF, false, true, false>(gp.t, tm, tn, alpha, beta, beta_zero, sm);
+                break;
+            case 4:
+                if constexpr (RV && (VM & 16) != 0) mfma_tile<T, CF, false, false, RV>(gp.t, tm, tn, alpha, beta, beta_zero, sm);
+                break;
+            default:
+                if constexpr ((VM & 1) != 0) mfma_tile<T, CF, false, false, false>(gp.t, tm, tn, alpha, beta, beta_zero, sm);
+                break;
         }
         if (threadIdx.x == 0) next_tile = claimed;
         __syncthreads();  // LDS reuse by the next tile; next_tile visible
@@ -666,6 +676,7 @@ static void launch_grouped_cfg(rstsr_hip_ctx* ctx, const std::vector<TileArgs<T>
     for (size_t i = 0; i < order.size(); ++i) order[i] = i;
     std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) { return probs[x].K > probs[y].K; });
     uint64_t total = 0;
+    uint32_t forms = 0;  // bit f: some problem loads with form f
     for (size_t i = 0; i < probs.size(); ++i) {
         const auto& t = probs[order[i]];
         uint32_t tm = ceil_div_u32(t.M, CF::BM), tn = ceil_div_u32(t.N, CF::BN);
@@ -676,6 +687,7 @@ static void launch_grouped_cfg(rstsr_hip_ctx* ctx, const std::vector<TileArgs<T>
         bool vec = !CplxOf<T>::value && !amc && !bkc && pair_loads_ok(t.A, t.csa, t.rsa, nullptr, 0) &&
                    pair_loads_ok(t.B, t.csb, t.rsb, nullptr, 0);
         gp[i].flags = vec ? 4u : ((amc ? 1u : 0u) | (bkc ? 2u : 0u));
+        forms |= 1u << gp[i].flags;
         prefix[i] = (uint32_t)total;
         total += (uint64_t)tm * tn;
     }
@@ -692,19 +704,26 @@ static void launch_grouped_cfg(rstsr_hip_ctx* ctx, const std::vector<TileArgs<T>
     const uint32_t zero = 0;
     std::memcpy(host + qoff, &zero, sizeof(zero));
     meta_upload(ctx, meta, bytes);
-    // persistent grid: as many blocks as can be resident at once. The launch bound sets waves per
-    // SIMD, not blocks per CU, so the count comes from the occupancy query: a block beyond it would
-    // start only once the queue is empty and then run its first tile (one of the longest) alone.
-    static const int per_cu = [] {
-        int v = 0;
-        HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&v, k_gemm_grouped<T, CF>, CF::NT, 0));
-        return std::max(v, 1);
-    }();
-    uint32_t grid = (uint32_t)std::min<uint64_t>(total, (uint64_t)std::max(ctx->num_cu, 1) * per_cu);
-    hipLaunchKernelGGL((k_gemm_grouped<T, CF>), dim3(grid), dim3(CF::NT), 0, ctx->stream,
-                       (const GroupProblem<T>*)meta, (int)gp.size(), (const uint32_t*)(meta + poff), (uint32_t)total,
-                       alpha, beta, beta_zero ? 1 : 0, run_if, (uint32_t*)(meta + qoff));
-    check_launch("grouped gemm");
+    auto launch = [&](auto vm) {
+        constexpr uint32_t VM = decltype(vm)::value;
+        // persistent grid: as many blocks as can be resident at once. The launch bound sets waves
+        // per SIMD, not blocks per CU, so the count comes from the occupancy query: a block beyond
+        // it would start only once the queue is empty and then run its first tile (one of the
+        // longest) alone.
+        static const int per_cu = [] {
+            int v = 0;
+            HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&v, k_gemm_grouped<T, CF, VM>, CF::NT, 0));
+            return std::max(v, 1);
+        }();
+        uint32_t grid = (uint32_t)std::min<uint64_t>(total, (uint64_t)std::max(ctx->num_cu, 1) * per_cu);
+        hipLaunchKernelGGL((k_gemm_grouped<T, CF, VM>), dim3(grid), dim3(CF::NT), 0, ctx->stream,
+                           (const GroupProblem<T>*)meta, (int)gp.size(), (const uint32_t*)(meta + poff),
+                           (uint32_t)total, alpha, beta, beta_zero ? 1 : 0, run_if, (uint32_t*)(meta + qoff));
+        check_launch("grouped gemm");
+    };
+    // every problem row-major in both operands (forms 0 and 4): the kernel with those two only
+    if ((forms & ~17u) == 0) launch(std::integral_constant<uint32_t, 17>{});
+    else launch(std::integral_constant<uint32_t, 31>{});
 }
 
 template <class T>
